@@ -1,0 +1,166 @@
+/*
+ * mpas_sw_amd: C ABI of the MI355X (gfx950) MPAS shallow-water RK4 core,
+ * its tangent-linear model (TLM) and its adjoint.
+ *
+ * This is the drop-in boundary for the reference's compiled module
+ * `module_sw_mpas` (xtian15/MPAS-SW-TL-AD, built by f2py from the src/ Fortran at
+ * src/Makefile:7-8).  The reference exposes argument-less Fortran module
+ * procedures acting on module globals; this library exposes the same
+ * computations on explicit device buffers.  The Python facade
+ * `module_sw_mpas.mpas_sw_module` (package mpas-sw-tl-ad_amd) restores the
+ * reference's attribute interface on top of it.
+ *
+ * Conventions
+ *  - All state pointers are DEVICE pointers owned by the caller, in the
+ *    library's internal entity order (see msw_mesh_desc perms), with batch
+ *    members stored innermost: element (entity i, member b) at [i*B + b].
+ *  - `prec` selects the member arithmetic: MSW_FP64 (double*) or MSW_FP32
+ *    (float*).  Nonlinear base states of the TLM/adjoint are always double.
+ *  - Every call is ordered on the stream given at creation (e.g. the caller's
+ *    torch stream) and returns 0 or a negative MSW_E* code; the library never
+ *    aborts.  msw_last_error() describes the last failure of this thread.
+ *  - A context is not thread-safe.  Use one per (mesh, batch, precision).
+ */
+#ifndef MPAS_SW_AMD_H
+#define MPAS_SW_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSW_ABI_VERSION 1
+
+enum msw_status {
+    MSW_OK = 0,
+    MSW_EINVAL = -1,      /* bad argument or inconsistent mesh      */
+    MSW_ENOMEM = -2,      /* device or host allocation failed       */
+    MSW_EHIP = -3,        /* HIP runtime error                      */
+    MSW_EUNSUPPORTED = -4 /* option the GPU path does not implement */
+};
+
+enum msw_precision { MSW_FP64 = 0, MSW_FP32 = 1 };
+
+/* Mesh in the reference's layout: the arrays src/mpas_sw_driver.py:87-119
+ * loads, already scaled to the sphere of radius a by the test-case routine
+ * (src/mpas_sw_testcases.f90:91-104).  Connectivity is 1-based;
+ * (slot, entity) arrays are flattened entity-major (Fortran column-major of
+ * the reference's (maxEdges, nCells) etc.). */
+typedef struct msw_mesh_desc {
+    int32_t nCells, nEdges, nVertices;
+    int32_t maxEdges, maxEdges2, vertexDegree;
+    const int32_t *cellsOnEdge;      /* (2, E)          */
+    const int32_t *verticesOnEdge;   /* (2, E)          */
+    const int32_t *nEdgesOnCell;     /* (C)             */
+    const int32_t *edgesOnCell;      /* (maxEdges, C)   */
+    const int32_t *nEdgesOnEdge;     /* (E)             */
+    const int32_t *edgesOnEdge;      /* (maxEdges2, E)  */
+    const double  *weightsOnEdge;    /* (maxEdges2, E)  */
+    const int32_t *cellsOnVertex;    /* (3, V)          */
+    const double  *kiteAreasOnVertex;/* (3, V)          */
+    const double  *dcEdge, *dvEdge;  /* (E)             */
+    const double  *areaCell;         /* (C)             */
+    const double  *areaTriangle;     /* (V)             */
+    const double  *fVertex;          /* (V)             */
+    const double  *h_s;              /* (C) bottom topography */
+    const double  *latCell, *lonCell;/* (C)             */
+    const double  *coeffs_reconstruct; /* (3, maxEdges, C), may be NULL */
+    /* Internal order: perm[k] = reference index (0-based) of internal entity
+     * k.  NULL means identity.  State buffers passed to the step functions
+     * use the internal order; msw_permute converts. */
+    const int32_t *cellPerm, *edgePerm, *vertexPerm;
+} msw_mesh_desc;
+
+/* Configuration scalars of src/mpas_sw_module.f90:23-42 that the stepped
+ * path reads. */
+typedef struct msw_config {
+    double dt;                 /* config_dt                   */
+    double apvm_upwinding;     /* config_apvm_upwinding       */
+    double visc2, visc4;       /* config_h_mom_eddy_visc2/4   */
+    int32_t test_case;         /* config_test_case (1 freezes u) */
+    int32_t thickness_adv_order;
+    int32_t monotonic, wind_stress, bottom_drag;
+} msw_config;
+
+typedef struct msw_ctx msw_ctx;
+
+/* Replaces: var_allocation/var_allocation_tlm/var_allocation_adj and the
+ * mesh part of sw_mpas_init_block (src/mpas_sw_module.f90:211-526).
+ * batch: members per entity, a power of two in [1, 256]. */
+int msw_create(const msw_mesh_desc *mesh, const msw_config *cfg, int prec, int batch,
+               void *hip_stream, msw_ctx **out);
+int msw_destroy(msw_ctx *ctx);
+/* Update dt/apvm/visc between calls (same mesh). */
+int msw_set_config(msw_ctx *ctx, const msw_config *cfg);
+/* Use a captured hipGraph per step for the step functions (default off). */
+int msw_set_graphs(msw_ctx *ctx, int enable);
+
+/* Replaces: sw_rk4 (src/mpas_sw_module.f90:528-635), nsteps times, on a
+ * batch of B independent states h[C][B], u[E][B] of type `prec`. */
+int msw_fwd_steps(msw_ctx *ctx, void *h, void *u, int nsteps);
+
+/* FP64 single-member forward (the nonlinear trajectory of the TLM/adjoint).
+ * If traj_h/traj_u are non-NULL the state at the START of step k is stored
+ * at traj_h + k*C, traj_u + k*E. */
+int msw_base_steps(msw_ctx *ctx, double *h, double *u, int nsteps, double *traj_h,
+                   double *traj_u);
+
+/* Replaces: sw_rk4_tlm (src/mpas_sw_module.f90:637-781), nsteps times.
+ * Advances the FP64 nonlinear state (h,u) [B=1] and the batched TL state
+ * (h_tl,u_tl) [B members, `prec`] together, as the reference does
+ * (:774-777).  Optional trajectory storage as in msw_base_steps. */
+int msw_tlm_steps(msw_ctx *ctx, double *h, double *u, void *h_tl, void *u_tl, int nsteps,
+                  double *traj_h, double *traj_u);
+
+/* Replaces: sw_rk4_adj (src/mpas_sw_module.f90:783-1002), one step.
+ * (h,u): FP64 nonlinear state at the step start (unchanged).
+ * (h_ad,u_ad): batched adjoint at the step end on entry, at the step start
+ * on return.  (h_end,u_end): optional FP64 nonlinear end state (the
+ * reference's h(2),u(2)). */
+int msw_adj_step(msw_ctx *ctx, const double *h, const double *u, void *h_ad, void *u_ad,
+                 double *h_end, double *u_end);
+
+/* nsteps adjoint steps backwards over a stored trajectory: step k (from
+ * nsteps-1 down to 0) uses traj_h + k*C, traj_u + k*E as its start state. */
+int msw_adj_steps(msw_ctx *ctx, const double *traj_h, const double *traj_u, void *h_ad,
+                  void *u_ad, int nsteps);
+
+/* Replaces: sw_compute_solve_diagnostics (src/mpas_sw_dyncore.f90:1-361) for
+ * one FP64 state; writes the 13 diagnostic fields into `out` (double,
+ * packed in the order of msw_diag_names, each of length C, E or V). */
+int msw_diagnostics(msw_ctx *ctx, const double *h, const double *u, double *out);
+/* comma-separated names and entity kinds ('c','e','v') of msw_diagnostics */
+const char *msw_diag_names(void);
+const char *msw_diag_kinds(void);
+
+/* Replaces: mpas_reconstruct_2d (src/mpas_sw_operators.f90:283-370) for the
+ * batch: out5 = [X, Y, Z, zonal, meridional], each [C][B] of `prec`. */
+int msw_reconstruct(msw_ctx *ctx, const void *u, void *out5);
+
+/* Deterministic FP64 inner products per member:
+ *   out[b] = sum_c wh[c] xh[c][b] yh[c][b] + sum_e wu[e] xu[e][b] yu[e][b]
+ * Any of wh/wu may be NULL (weight 1); yh/yu NULL means 1; xh or xu NULL
+ * drops that part.  out: device double[B]. */
+int msw_dot(msw_ctx *ctx, const void *xh, const void *xu, const void *yh, const void *yu,
+            const double *wh, const double *wu, double *out);
+
+/* Per-member non-finite detector: flags[b] = 1 if any NaN/Inf in member b
+ * of (h,u); flags is a device int[B] the caller zeroes. */
+int msw_nonfinite(msw_ctx *ctx, const void *h, const void *u, int *flags);
+
+/* Reorder rows between the reference order and the internal order.
+ * kind: 'c','e','v'; to_internal: 1 (reference -> internal) or 0.
+ * Rows hold `batch` values of precision `prec`. */
+int msw_permute(msw_ctx *ctx, int kind, int to_internal, int prec, int batch, const void *src,
+                void *dst);
+
+/* Device memory the context holds, in bytes. */
+int64_t msw_device_bytes(const msw_ctx *ctx);
+int msw_abi_version(void);
+const char *msw_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

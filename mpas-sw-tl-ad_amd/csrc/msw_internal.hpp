@@ -1,0 +1,150 @@
+// Internal declarations shared by the host runtime (msw_runtime.cpp) and the
+// HIP kernels (msw_kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace msw {
+
+constexpr int kMaxEdges = 6;
+constexpr int kMaxEdges2 = 12;
+constexpr int kVertexDegree = 3;
+constexpr double kGravity = 9.80616;   // src/mpas_sw_dyncore.f90:414
+
+// Device-resident mesh tables, entities in the library's internal order
+// (space-filling curve), 0-based.  "R" tables list neighbours in the order
+// the reference's scatter loops visit them (ascending reference index), so a
+// gather over them adds the terms in the reference's order.
+struct DMesh {
+    int nC, nE, nV;
+    // cells
+    const int* nEoC;      // [C]
+    const int* eoc;       // [C][6]  edgesOnCell, slot order
+    const double* sgnC;   // [C][6]  +1 if the cell is cell1 of eoc[slot]
+    const int* eocR;      // [C][6]  edges in ascending reference edge index
+    const double* sgnCR;  // [C][6]  +1 if the cell is cell1 of eocR[k]
+    const int* vocR;      // [C][6]  vertices in ascending reference vertex index
+    const double* kiteCR; // [C][6]  kite area of the cell at vocR[k]
+    const double* areaC;  // [C]
+    const double* hs;     // [C]
+    // edges
+    const int* coe;       // [E][2]
+    const int* voe;       // [E][2]
+    const int* nEoE;      // [E]
+    const int* eoe;       // [E][12]
+    const double* woe;    // [E][12]
+    const int* eoeT;      // [E][12] slot of this edge in eoe[eoe[e][j]]
+    const double* dc;     // [E]
+    const double* dv;     // [E]
+    // vertices
+    const int* eovR;      // [V][3]  edges in ascending reference edge index
+    const double* sgnV;   // [V][3]  +1 if the vertex is vertex2 of eovR[k]
+    const int* cov;       // [V][3]  cellsOnVertex, slot order
+    const double* kite;   // [V][3]
+    const double* areaT;  // [V]
+    const double* fV;     // [V]
+};
+
+// Physics configuration the stencil needs.
+struct DConfig {
+    double dt;
+    double apvm;
+    int frozen_u;   // test case 1: wind field held fixed (src/mpas_sw_module.f90:596-598)
+};
+
+// Per-stage base (nonlinear) fields kept for the tangent-linear and adjoint
+// kernels.  All double, B = 1.
+struct BaseStage {
+    double* h;      // [C]  provisional state
+    double* u;      // [E]
+    double* he;     // [E]  h_edge
+    double* pe;     // [E]  pv_edge
+    double* v;      // [E]  tangential velocity
+    double* gpt;    // [E]  gradPVt
+    double* gpn;    // [E]  gradPVn
+    double* ke;     // [C]
+    double* pc;     // [C]  pv_cell
+    double* pvv;    // [V]  pv_vertex
+    double* hv;     // [V]  h_vertex
+    double* vort;   // [V]  relative vorticity
+};
+
+// RK4 coefficients of src/mpas_sw_module.f90:567-575.
+struct RKCoeffs {
+    double w[4];
+    double sw[4];
+};
+RKCoeffs rk_coeffs(double dt);
+
+// ---- launchers (msw_kernels.hip).  B = 1 << lb members per entity, stored
+// innermost.  T is the member arithmetic type.
+
+// Nonlinear stage: diagnostics at state (h,u) into `st` (which may alias
+// `st.h/st.u` = (h,u)), then tendency and the RK4 update of stage `s`:
+//   acc = (s==0 ? x : acc) + w[s]*tend ;  pnext = x + sw[s]*tend (s<3)
+//   s==3: x = acc + w[3]*tend
+template <typename T>
+void fwd_diag(const DMesh& m, const DConfig& c, int lb, const T* h, const T* u,
+              T* pvv, T* hv, T* vort, T* ke, T* pc, T* he, T* pe, T* v, T* gpt, T* gpn,
+              hipStream_t s);
+template <typename T>
+void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
+                        const T* h, const T* u, const T* he, const T* pe, const T* ke,
+                        T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s);
+
+// Tangent-linear stage on top of base fields `b` of the same stage.
+template <typename T>
+void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
+              const BaseStage& b, const T* h_tl, const T* u_tl,
+              T* pvv_tl, T* ke_tl, T* pc_tl, T* pe_tl,
+              T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s);
+
+// Adjoint of one RK4 stage: pa = F'(p_s)^T g, then the epilogue
+//   s>0: accx = (first ? pa : accx + pa) ; g = w[s-1]*lam + sw[s-1]*pa
+//   s==0: lam = (accx + lam) + pa
+template <typename T>
+void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, const RKCoeffs& k,
+               const BaseStage& b, T* gh, T* gu, T* pe_ad, T* he_ad, T* ua_p, T* pc_ad, T* ha_p,
+               T* hv_ad, T* circ_ad, T* lamh, T* lamu, T* acch, T* accu, hipStream_t s);
+// g = w[3] * lam  (seed of the reverse sweep)
+template <typename T>
+void adj_seed(int n, T* g, const T* lam, double w3, hipStream_t s);
+
+// Reconstruction of the cell-centre velocity (src/mpas_sw_operators.f90:283-370)
+template <typename T>
+void reconstruct(int nC, int lb, const int* nEoC, const int* eoc, const double* coeffs,
+                 const double* trig, const T* u, T* out5, hipStream_t s);
+
+// Deterministic per-member weighted dot products: out[b] = sum_i w_i x[i][b] y[i][b]
+// (w may be null).  out is double[B].  `partial` holds nblk*B doubles.
+template <typename T>
+void dot_batched(long n, int lb, const T* x, const T* y, const double* w, double* partial,
+                 int nblk, double* out, int accumulate, hipStream_t s);
+int dot_blocks(long n);
+
+// copies and conversions
+template <typename TD, typename TS>
+void convert(long n, TD* dst, const TS* src, hipStream_t s);
+template <typename T>
+void fill(long n, T* dst, T val, hipStream_t s);
+// dst[i][b] = src[perm[i]][b]   (gather rows)
+template <typename T>
+void gather_rows(long n, int lb, const int* perm, const T* src, T* dst, hipStream_t s);
+// dst[perm[i]][b] = src[i][b]   (scatter rows)
+template <typename T>
+void scatter_rows(long n, int lb, const int* perm, const T* src, T* dst, hipStream_t s);
+// out[b] = 1 if any non-finite value in member b of x (n rows)
+template <typename T>
+void nonfinite_flags(long n, int lb, const T* x, int* flags, hipStream_t s);
+
+}  // namespace msw
+
+namespace msw {
+// circulation (V), divergence (C) and vorticity_cell (C) for the diagnostic
+// output (src/mpas_sw_dyncore.f90:208-242, :324-340); FP64, B = 1.
+void diag_extra(const DMesh& m, const double* u, const double* vort, double* circ, double* div,
+                double* vc, hipStream_t s);
+}  // namespace msw

@@ -1,0 +1,802 @@
+// Host runtime of libmpas_sw_amd: mesh tables, device buffers, the RK4
+// forward / tangent-linear / adjoint drivers and the extern "C" boundary
+// declared in include/mpas_sw_amd.h.
+//
+// The drivers restate the reference's RK4 routines
+// (src/mpas_sw_module.f90:528-1002) as sequences of stage kernels on device
+// buffers; the per-stage kernels live in msw_kernels.hip.
+#include "../../include/mpas_sw_amd.h"
+#include "msw_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using namespace msw;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return fail(MSW_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+    } while (0)
+
+// Device allocation owned by a context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct msw_ctx {
+    int prec = MSW_FP64;
+    int B = 1, lb = 0;
+    int nC = 0, nE = 0, nV = 0;
+    hipStream_t stream = nullptr;
+    msw_config cfg{};
+    DConfig dcfg{};
+    DMesh dm{};
+    std::vector<DevBuf> bufs;
+    int64_t bytes = 0;
+
+    // reconstruction tables
+    const double* coeffs = nullptr;   // [C][6][3]
+    const double* trig = nullptr;     // [C][4]
+    // permutations (device): perm[k] = reference index of internal entity k
+    const int* permC = nullptr;
+    const int* permE = nullptr;
+    const int* permV = nullptr;
+
+    // FP64 base (B = 1) buffers
+    BaseStage st[4]{};
+    double* bacc = nullptr;           // [C+E] accumulator of the base RK4
+    double* bx = nullptr;             // [C+E] step-start copy for the adjoint
+    double* bend = nullptr;           // [C+E] scratch end state
+    double* dextra = nullptr;         // [V + 2C] circulation, divergence, vorticity_cell
+
+    // member scratch ([entity][B] of prec)
+    void* macc = nullptr;             // [C+E]
+    void* mpA = nullptr;              // [C+E]
+    void* mpB = nullptr;              // [C+E]
+    void* mv1 = nullptr;              // [V]
+    void* mv2 = nullptr;              // [V]
+    void* mc1 = nullptr;              // [C]
+    void* mc2 = nullptr;              // [C]
+    void* me1 = nullptr;              // [E]
+    void* me2 = nullptr;              // [E]
+    void* me3 = nullptr;              // [E]
+    double* dpartial = nullptr;       // dot partials
+    int dot_nblk = 0;
+
+    // graphs
+    bool use_graphs = false;
+    hipStream_t cap_stream = nullptr;
+    std::map<std::tuple<int, const void*, const void*, const void*, const void*>, hipGraphExec_t> graphs;
+
+    ~msw_ctx()
+    {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        for (auto& b : bufs)
+            if (b.p) (void)hipFree(b.p);
+    }
+
+    template <typename T>
+    int alloc(T** out, size_t n)
+    {
+        void* p = nullptr;
+        size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            *out = nullptr;
+            return fail(MSW_ENOMEM, std::string("hipMalloc(") + std::to_string(bytes) +
+                                        "): " + hipGetErrorString(e));
+        }
+        bufs.push_back({p, bytes});
+        this->bytes += static_cast<int64_t>(bytes);
+        *out = static_cast<T*>(p);
+        return MSW_OK;
+    }
+
+    template <typename T>
+    int upload(const T** out, const std::vector<T>& host)
+    {
+        T* p = nullptr;
+        int rc = alloc(&p, host.size());
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+        *out = p;
+        return MSW_OK;
+    }
+
+    size_t esz() const { return prec == MSW_FP64 ? sizeof(double) : sizeof(float); }
+};
+
+namespace {
+
+bool is_pow2(int b) { return b > 0 && (b & (b - 1)) == 0; }
+
+// Build the internal-order mesh tables on the host and upload them.
+int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
+{
+    const int C = d->nCells, E = d->nEdges, V = d->nVertices;
+    const int mE = d->maxEdges, mE2 = d->maxEdges2, vd = d->vertexDegree;
+    if (C <= 0 || E <= 0 || V <= 0) return fail(MSW_EINVAL, "empty mesh");
+    if (vd != kVertexDegree) return fail(MSW_EUNSUPPORTED, "vertexDegree must be 3");
+    if (mE < 1 || mE2 < 1) return fail(MSW_EINVAL, "maxEdges/maxEdges2 must be positive");
+    const void* req[] = {d->cellsOnEdge, d->verticesOnEdge, d->nEdgesOnCell, d->edgesOnCell,
+                         d->nEdgesOnEdge, d->edgesOnEdge, d->weightsOnEdge, d->cellsOnVertex,
+                         d->kiteAreasOnVertex, d->dcEdge, d->dvEdge, d->areaCell,
+                         d->areaTriangle, d->fVertex, d->h_s, d->latCell, d->lonCell};
+    for (const void* p : req)
+        if (!p) return fail(MSW_EINVAL, "a required mesh array is NULL");
+
+    // permutations: new -> old, and inverse
+    auto mkperm = [](const int32_t* p, int n, std::vector<int>& fwd, std::vector<int>& inv) {
+        fwd.resize(n);
+        inv.assign(n, -1);
+        for (int k = 0; k < n; ++k) {
+            int o = p ? p[k] : k;
+            if (o < 0 || o >= n || inv[o] != -1) return false;
+            fwd[k] = o;
+            inv[o] = k;
+        }
+        return true;
+    };
+    std::vector<int> pc, ic, pe, ie, pv, iv;
+    if (!mkperm(d->cellPerm, C, pc, ic) || !mkperm(d->edgePerm, E, pe, ie) ||
+        !mkperm(d->vertexPerm, V, pv, iv))
+        return fail(MSW_EINVAL, "a permutation is not a bijection");
+
+    auto chk = [](int32_t x, int n) { return x >= 1 && x <= n; };
+    // ---- validate reference connectivity
+    for (int e = 0; e < E; ++e) {
+        for (int k = 0; k < 2; ++k) {
+            if (!chk(d->cellsOnEdge[2 * e + k], C))
+                return fail(MSW_EINVAL, "cellsOnEdge out of range at edge " + std::to_string(e + 1));
+            if (!chk(d->verticesOnEdge[2 * e + k], V))
+                return fail(MSW_EINVAL, "verticesOnEdge out of range at edge " + std::to_string(e + 1));
+        }
+        int n = d->nEdgesOnEdge[e];
+        if (n < 0 || n > std::min(mE2, kMaxEdges2))
+            return fail(n > kMaxEdges2 ? MSW_EUNSUPPORTED : MSW_EINVAL,
+                        "nEdgesOnEdge out of range at edge " + std::to_string(e + 1));
+        for (int j = 0; j < n; ++j)
+            if (!chk(d->edgesOnEdge[static_cast<long>(mE2) * e + j], E))
+                return fail(MSW_EINVAL, "edgesOnEdge out of range at edge " + std::to_string(e + 1));
+        if (!(d->dcEdge[e] > 0.0) || !(d->dvEdge[e] > 0.0))
+            return fail(MSW_EINVAL, "non-positive dcEdge/dvEdge at edge " + std::to_string(e + 1));
+    }
+    for (int i = 0; i < C; ++i) {
+        int n = d->nEdgesOnCell[i];
+        if (n < 1 || n > std::min(mE, kMaxEdges))
+            return fail(n > kMaxEdges ? MSW_EUNSUPPORTED : MSW_EINVAL,
+                        "nEdgesOnCell out of range at cell " + std::to_string(i + 1));
+        for (int k = 0; k < n; ++k)
+            if (!chk(d->edgesOnCell[static_cast<long>(mE) * i + k], E))
+                return fail(MSW_EINVAL, "edgesOnCell out of range at cell " + std::to_string(i + 1));
+        if (!(d->areaCell[i] > 0.0))
+            return fail(MSW_EINVAL, "non-positive areaCell at cell " + std::to_string(i + 1));
+    }
+    for (int v = 0; v < V; ++v) {
+        for (int k = 0; k < 3; ++k)
+            if (!chk(d->cellsOnVertex[3 * v + k], C))
+                return fail(MSW_EINVAL, "cellsOnVertex out of range at vertex " + std::to_string(v + 1));
+        if (!(d->areaTriangle[v] > 0.0))
+            return fail(MSW_EINVAL, "non-positive areaTriangle at vertex " + std::to_string(v + 1));
+    }
+
+    // ---- cells
+    std::vector<int> nEoC(C), eoc(static_cast<size_t>(C) * kMaxEdges, 0);
+    std::vector<double> sgnC(static_cast<size_t>(C) * kMaxEdges, 0.0);
+    std::vector<int> eocR(static_cast<size_t>(C) * kMaxEdges, 0), vocR(static_cast<size_t>(C) * kMaxEdges, 0);
+    std::vector<double> sgnCR(static_cast<size_t>(C) * kMaxEdges, 0.0), kiteCR(static_cast<size_t>(C) * kMaxEdges, 0.0);
+    std::vector<double> areaC(C), hs(C), trig(static_cast<size_t>(C) * 4),
+        coeffs(static_cast<size_t>(C) * kMaxEdges * 3, 0.0);
+    // edges of each reference cell in ascending reference edge index
+    std::vector<std::vector<int>> cellEdges(C), cellVerts(C);
+    for (int e = 0; e < E; ++e) {
+        int c1 = d->cellsOnEdge[2 * e] - 1, c2 = d->cellsOnEdge[2 * e + 1] - 1;
+        cellEdges[c1].push_back(e);
+        if (c2 != c1) cellEdges[c2].push_back(e);
+    }
+    for (int v = 0; v < V; ++v)
+        for (int k = 0; k < 3; ++k) {
+            int cc = d->cellsOnVertex[3 * v + k] - 1;
+            if (cellVerts[cc].empty() || cellVerts[cc].back() != v) cellVerts[cc].push_back(v);
+        }
+    for (int k = 0; k < C; ++k) {
+        const int o = pc[k];
+        const int n = d->nEdgesOnCell[o];
+        nEoC[k] = n;
+        if (static_cast<int>(cellEdges[o].size()) != n || static_cast<int>(cellVerts[o].size()) != n)
+            return fail(MSW_EINVAL, "cell " + std::to_string(o + 1) +
+                                        ": edgesOnCell, cellsOnEdge and cellsOnVertex disagree");
+        for (int i = 0; i < n; ++i) {
+            int e = d->edgesOnCell[static_cast<long>(mE) * o + i] - 1;
+            eoc[kMaxEdges * k + i] = ie[e];
+            sgnC[kMaxEdges * k + i] = (d->cellsOnEdge[2 * e] - 1 == o) ? 1.0 : -1.0;
+            int er = cellEdges[o][i];
+            eocR[kMaxEdges * k + i] = ie[er];
+            sgnCR[kMaxEdges * k + i] = (d->cellsOnEdge[2 * er] - 1 == o) ? 1.0 : -1.0;
+            int vr = cellVerts[o][i];
+            vocR[kMaxEdges * k + i] = iv[vr];
+            double kite = 0.0;
+            for (int s = 0; s < 3; ++s)
+                if (d->cellsOnVertex[3 * vr + s] - 1 == o) kite = d->kiteAreasOnVertex[3 * vr + s];
+            kiteCR[kMaxEdges * k + i] = kite;
+            if (d->coeffs_reconstruct)
+                for (int r = 0; r < 3; ++r)
+                    coeffs[(static_cast<size_t>(k) * kMaxEdges + i) * 3 + r] =
+                        d->coeffs_reconstruct[(static_cast<size_t>(o) * mE + i) * 3 + r];
+        }
+        areaC[k] = d->areaCell[o];
+        hs[k] = d->h_s[o];
+        // one sincos per angle, as the reference's compiled reconstruction
+        // evaluates them (src/mpas_sw_operators.f90:350-353)
+        sincos(d->latCell[o], &trig[4 * k + 1], &trig[4 * k]);
+        sincos(d->lonCell[o], &trig[4 * k + 3], &trig[4 * k + 2]);
+    }
+    // ---- edges
+    std::vector<int> coe(2 * static_cast<size_t>(E)), voe(2 * static_cast<size_t>(E)), nEoE(E),
+        eoe(static_cast<size_t>(E) * kMaxEdges2, 0), eoeT(static_cast<size_t>(E) * kMaxEdges2, 0);
+    std::vector<double> woe(static_cast<size_t>(E) * kMaxEdges2, 0.0), dc(E), dv(E);
+    for (int k = 0; k < E; ++k) {
+        const int o = pe[k];
+        coe[2 * k] = ic[d->cellsOnEdge[2 * o] - 1];
+        coe[2 * k + 1] = ic[d->cellsOnEdge[2 * o + 1] - 1];
+        voe[2 * k] = iv[d->verticesOnEdge[2 * o] - 1];
+        voe[2 * k + 1] = iv[d->verticesOnEdge[2 * o + 1] - 1];
+        const int n = d->nEdgesOnEdge[o];
+        nEoE[k] = n;
+        for (int j = 0; j < n; ++j) {
+            const int eo = d->edgesOnEdge[static_cast<long>(mE2) * o + j] - 1;
+            eoe[kMaxEdges2 * k + j] = ie[eo];
+            woe[kMaxEdges2 * k + j] = d->weightsOnEdge[static_cast<long>(mE2) * o + j];
+            // slot of o in eo's list
+            int jt = -1;
+            const int n2 = d->nEdgesOnEdge[eo];
+            for (int jj = 0; jj < n2; ++jj)
+                if (d->edgesOnEdge[static_cast<long>(mE2) * eo + jj] - 1 == o) { jt = jj; break; }
+            if (jt < 0)
+                return fail(MSW_EINVAL, "edgesOnEdge is not symmetric at edge " + std::to_string(o + 1));
+            eoeT[kMaxEdges2 * k + j] = jt;
+        }
+        dc[k] = d->dcEdge[o];
+        dv[k] = d->dvEdge[o];
+    }
+    // ---- vertices
+    std::vector<std::vector<int>> vertEdges(V);
+    for (int e = 0; e < E; ++e) {
+        int v1 = d->verticesOnEdge[2 * e] - 1, v2 = d->verticesOnEdge[2 * e + 1] - 1;
+        vertEdges[v1].push_back(e);
+        if (v2 != v1) vertEdges[v2].push_back(e);
+    }
+    std::vector<int> eovR(3 * static_cast<size_t>(V)), cov(3 * static_cast<size_t>(V));
+    std::vector<double> sgnV(3 * static_cast<size_t>(V)), kite(3 * static_cast<size_t>(V)), areaT(V), fV(V);
+    for (int k = 0; k < V; ++k) {
+        const int o = pv[k];
+        if (vertEdges[o].size() != 3)
+            return fail(MSW_EINVAL, "vertex " + std::to_string(o + 1) + " is not on exactly 3 edges");
+        for (int i = 0; i < 3; ++i) {
+            const int e = vertEdges[o][i];
+            eovR[3 * k + i] = ie[e];
+            sgnV[3 * k + i] = (d->verticesOnEdge[2 * e + 1] - 1 == o) ? 1.0 : -1.0;
+            cov[3 * k + i] = ic[d->cellsOnVertex[3 * o + i] - 1];
+            kite[3 * k + i] = d->kiteAreasOnVertex[3 * o + i];
+        }
+        areaT[k] = d->areaTriangle[o];
+        fV[k] = d->fVertex[o];
+    }
+
+    DMesh& m = c->dm;
+    m.nC = C; m.nE = E; m.nV = V;
+    int rc = 0;
+#define UP(dst, vec) if ((rc = c->upload(&(dst), vec))) return rc
+    UP(m.nEoC, nEoC); UP(m.eoc, eoc); UP(m.sgnC, sgnC); UP(m.eocR, eocR); UP(m.sgnCR, sgnCR);
+    UP(m.vocR, vocR); UP(m.kiteCR, kiteCR); UP(m.areaC, areaC); UP(m.hs, hs);
+    UP(m.coe, coe); UP(m.voe, voe); UP(m.nEoE, nEoE); UP(m.eoe, eoe); UP(m.woe, woe);
+    UP(m.eoeT, eoeT); UP(m.dc, dc); UP(m.dv, dv);
+    UP(m.eovR, eovR); UP(m.sgnV, sgnV); UP(m.cov, cov); UP(m.kite, kite); UP(m.areaT, areaT);
+    UP(m.fV, fV);
+    UP(c->coeffs, coeffs); UP(c->trig, trig);
+    std::vector<int> vpc(pc.begin(), pc.end()), vpe(pe.begin(), pe.end()), vpv(pv.begin(), pv.end());
+    UP(c->permC, vpc); UP(c->permE, vpe); UP(c->permV, vpv);
+#undef UP
+    return MSW_OK;
+}
+
+int alloc_buffers(msw_ctx* c)
+{
+    const size_t C = c->nC, E = c->nE, V = c->nV, B = c->B;
+    int rc;
+    for (int s = 0; s < 4; ++s) {
+        BaseStage& b = c->st[s];
+        if (s > 0) {
+            if ((rc = c->alloc(&b.h, C + E))) return rc;
+            b.u = b.h + C;
+        }
+        double* eb = nullptr;
+        if ((rc = c->alloc(&eb, 5 * E + 2 * C + 3 * V))) return rc;
+        b.he = eb; b.pe = eb + E; b.v = eb + 2 * E; b.gpt = eb + 3 * E; b.gpn = eb + 4 * E;
+        b.ke = eb + 5 * E; b.pc = b.ke + C;
+        b.pvv = b.pc + C; b.hv = b.pvv + V; b.vort = b.hv + V;
+    }
+    if ((rc = c->alloc(&c->bacc, C + E))) return rc;
+    if ((rc = c->alloc(&c->bx, C + E))) return rc;
+    if ((rc = c->alloc(&c->bend, C + E))) return rc;
+    if ((rc = c->alloc(&c->dextra, V + 2 * C))) return rc;
+    const size_t es = c->esz();
+    char* mem = nullptr;
+    const size_t total = (3 * (C + E) + 2 * V + 2 * C + 3 * E) * B * es;
+    if ((rc = c->alloc(&mem, total))) return rc;
+    size_t off = 0;
+    auto take = [&](void** p, size_t rows) { *p = mem + off; off += rows * B * es; };
+    take(&c->macc, C + E); take(&c->mpA, C + E); take(&c->mpB, C + E);
+    take(&c->mv1, V); take(&c->mv2, V); take(&c->mc1, C); take(&c->mc2, C);
+    take(&c->me1, E); take(&c->me2, E); take(&c->me3, E);
+    c->dot_nblk = dot_blocks(std::max<long>(C, E));
+    if ((rc = c->alloc(&c->dpartial, static_cast<size_t>(c->dot_nblk) * B + 2 * B))) return rc;
+    return MSW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// drivers
+
+template <typename T>
+void enqueue_fwd_step(msw_ctx* c, T* h, T* u)
+{
+    const DMesh& m = c->dm;
+    const RKCoeffs k = rk_coeffs(c->cfg.dt);
+    const size_t C = c->nC, E = c->nE, B = c->B;
+    T* acc = static_cast<T*>(c->macc);
+    T* pA = static_cast<T*>(c->mpA);
+    T* pB = static_cast<T*>(c->mpB);
+    T* pvv = static_cast<T*>(c->mv1);
+    T* ke = static_cast<T*>(c->mc1);
+    T* pc = static_cast<T*>(c->mc2);
+    T* he = static_cast<T*>(c->me1);
+    T* pe = static_cast<T*>(c->me2);
+    const T* rh = h;
+    const T* ru = u;
+    for (int s = 0; s < 4; ++s) {
+        fwd_diag<T>(m, c->dcfg, c->lb, rh, ru, pvv, nullptr, nullptr, ke, pc, he, pe, nullptr,
+                    nullptr, nullptr, c->stream);
+        T* nb = (s % 2 == 0) ? pA : pB;
+        fwd_tend_update_he<T>(m, c->dcfg, c->lb, s, k, rh, ru, he, pe, ke, h, u, acc, acc + C * B, nb,
+                              nb + C * B, c->stream);
+        rh = nb;
+        ru = nb + C * B;
+    }
+    (void)E;
+}
+
+// FP64 B=1 base step storing per-stage fields in c->st (stage 0 reads x).
+void enqueue_base_step(msw_ctx* c, double* h, double* u)
+{
+    const DMesh& m = c->dm;
+    const RKCoeffs k = rk_coeffs(c->cfg.dt);
+    const size_t C = c->nC;
+    c->st[0].h = h;
+    c->st[0].u = u;
+    for (int s = 0; s < 4; ++s) {
+        BaseStage& b = c->st[s];
+        fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
+                         b.gpt, b.gpn, c->stream);
+        double* nh = s < 3 ? c->st[s + 1].h : c->bend;
+        fwd_tend_update_he<double>(m, c->dcfg, 0, s, k, b.h, b.u, b.he, b.pe, b.ke, h, u, c->bacc,
+                                   c->bacc + C, nh, nh + C, c->stream);
+    }
+}
+
+template <typename T>
+void enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
+{
+    const DMesh& m = c->dm;
+    const RKCoeffs k = rk_coeffs(c->cfg.dt);
+    const size_t C = c->nC, B = c->B;
+    T* acc = static_cast<T*>(c->macc);
+    T* pA = static_cast<T*>(c->mpA);
+    T* pB = static_cast<T*>(c->mpB);
+    T* pvv_tl = static_cast<T*>(c->mv1);
+    T* ke_tl = static_cast<T*>(c->mc1);
+    T* pc_tl = static_cast<T*>(c->mc2);
+    T* pe_tl = static_cast<T*>(c->me1);
+    c->st[0].h = h;
+    c->st[0].u = u;
+    const T* th = htl;
+    const T* tu = utl;
+    for (int s = 0; s < 4; ++s) {
+        BaseStage& b = c->st[s];
+        fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
+                         b.gpt, b.gpn, c->stream);
+        T* nb = (s % 2 == 0) ? pA : pB;
+        tl_stage<T>(m, c->dcfg, c->lb, s, k, b, th, tu, pvv_tl, ke_tl, pc_tl, pe_tl, htl, utl, acc,
+                    acc + C * B, nb, nb + C * B, c->stream);
+        double* nh = s < 3 ? c->st[s + 1].h : c->bend;
+        fwd_tend_update_he<double>(m, c->dcfg, 0, s, k, b.h, b.u, b.he, b.pe, b.ke, h, u, c->bacc,
+                                   c->bacc + C, nh, nh + C, c->stream);
+        th = nb;
+        tu = nb + C * B;
+    }
+}
+
+// One adjoint step; the base step-start state must already be in c->bx.
+template <typename T>
+void enqueue_adj_step(msw_ctx* c, T* lamh, T* lamu, bool want_end)
+{
+    const DMesh& m = c->dm;
+    const RKCoeffs k = rk_coeffs(c->cfg.dt);
+    const size_t C = c->nC, E = c->nE, B = c->B;
+    double* xh = c->bx;
+    double* xu = c->bx + C;
+    c->st[0].h = xh;
+    c->st[0].u = xu;
+    for (int s = 0; s < 4; ++s) {
+        BaseStage& b = c->st[s];
+        fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
+                         b.gpt, b.gpn, c->stream);
+        if (s < 3 || want_end) {
+            // stage 3 writes x_new into its "x" argument: give it the end buffer
+            double* nh = s < 3 ? c->st[s + 1].h : c->bend;
+            double* wxh = s < 3 ? xh : c->bend;
+            double* wxu = s < 3 ? xu : c->bend + C;
+            fwd_tend_update_he<double>(m, c->dcfg, 0, s, k, b.h, b.u, b.he, b.pe, b.ke, wxh, wxu,
+                                       c->bacc, c->bacc + C, nh, nh + C, c->stream);
+        }
+    }
+    T* g = static_cast<T*>(c->mpA);
+    T* acc = static_cast<T*>(c->macc);
+    adj_seed<T>(static_cast<int>(C * B), g, lamh, k.w[3], c->stream);
+    adj_seed<T>(static_cast<int>(E * B), g + C * B, lamu, k.w[3], c->stream);
+    for (int s = 3; s >= 0; --s)
+        adj_stage<T>(m, c->dcfg, c->lb, s, s == 3, k, c->st[s], g, g + C * B,
+                     static_cast<T*>(c->me1), static_cast<T*>(c->me2), static_cast<T*>(c->me3),
+                     static_cast<T*>(c->mc1), static_cast<T*>(c->mc2), static_cast<T*>(c->mv1),
+                     static_cast<T*>(c->mv2), lamh, lamu, acc, acc + C * B, c->stream);
+}
+
+int check_cfg(msw_ctx* c)
+{
+    const msw_config& g = c->cfg;
+    if (g.visc2 > 0.0 || g.visc4 > 0.0)
+        return fail(MSW_EUNSUPPORTED, "del2/del4 viscosity is not implemented on the GPU path");
+    if (g.wind_stress || g.bottom_drag)
+        return fail(MSW_EUNSUPPORTED, "wind stress / bottom drag are not implemented on the GPU path");
+    if (!(g.dt > 0.0)) return fail(MSW_EINVAL, "dt must be positive");
+    return MSW_OK;
+}
+
+// Run `body` once per step, either directly or through a captured graph
+// keyed by (kind, pointers).
+template <typename F>
+int run_steps(msw_ctx* c, int kind, const void* p0, const void* p1, const void* p2, const void* p3,
+              int nsteps, F&& body, const std::function<void(int)>* pre = nullptr)
+{
+    if (!c->use_graphs) {
+        for (int i = 0; i < nsteps; ++i) {
+            if (pre) (*pre)(i);
+            body();
+        }
+        HIP_TRY(hipGetLastError());
+        return MSW_OK;
+    }
+    auto key = std::make_tuple(kind, p0, p1, p2, p3);
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        if (!c->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+        hipStream_t user = c->stream;
+        c->stream = c->cap_stream;
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            body();
+            e = hipStreamEndCapture(c->cap_stream, &graph);
+        }
+        c->stream = user;
+        if (e != hipSuccess) return fail(MSW_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(MSW_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        it = c->graphs.emplace(key, exec).first;
+    }
+    for (int i = 0; i < nsteps; ++i) {
+        if (pre) (*pre)(i);
+        HIP_TRY(hipGraphLaunch(it->second, c->stream));
+    }
+    HIP_TRY(hipGetLastError());
+    return MSW_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int msw_abi_version(void) { return MSW_ABI_VERSION; }
+
+const char* msw_last_error(void) { return g_err.c_str(); }
+
+int msw_create(const msw_mesh_desc* mesh, const msw_config* cfg, int prec, int batch,
+               void* hip_stream, msw_ctx** out)
+{
+    if (!out) return fail(MSW_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!mesh || !cfg) return fail(MSW_EINVAL, "mesh or cfg is NULL");
+    if (prec != MSW_FP64 && prec != MSW_FP32) return fail(MSW_EINVAL, "prec must be MSW_FP64 or MSW_FP32");
+    if (!is_pow2(batch) || batch > 256) return fail(MSW_EINVAL, "batch must be a power of two in [1,256]");
+    std::unique_ptr<msw_ctx> c(new (std::nothrow) msw_ctx());
+    if (!c) return fail(MSW_ENOMEM, "host allocation failed");
+    c->prec = prec;
+    c->B = batch;
+    c->lb = 0;
+    while ((1 << c->lb) < batch) ++c->lb;
+    c->nC = mesh->nCells;
+    c->nE = mesh->nEdges;
+    c->nV = mesh->nVertices;
+    c->stream = static_cast<hipStream_t>(hip_stream);
+    c->cfg = *cfg;
+    c->dcfg.dt = cfg->dt;
+    c->dcfg.apvm = cfg->apvm_upwinding;
+    c->dcfg.frozen_u = cfg->test_case == 1 ? 1 : 0;
+    int rc = build_mesh(c.get(), mesh);
+    if (rc) return rc;
+    if ((rc = alloc_buffers(c.get()))) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    *out = c.release();
+    return MSW_OK;
+}
+
+int msw_destroy(msw_ctx* ctx)
+{
+    if (!ctx) return MSW_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+    return MSW_OK;
+}
+
+int msw_set_config(msw_ctx* ctx, const msw_config* cfg)
+{
+    if (!ctx || !cfg) return fail(MSW_EINVAL, "NULL argument");
+    ctx->cfg = *cfg;
+    ctx->dcfg.dt = cfg->dt;
+    ctx->dcfg.apvm = cfg->apvm_upwinding;
+    ctx->dcfg.frozen_u = cfg->test_case == 1 ? 1 : 0;
+    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+    return MSW_OK;
+}
+
+int msw_set_graphs(msw_ctx* ctx, int enable)
+{
+    if (!ctx) return fail(MSW_EINVAL, "NULL ctx");
+    ctx->use_graphs = enable != 0;
+    return MSW_OK;
+}
+
+int msw_fwd_steps(msw_ctx* ctx, void* h, void* u, int nsteps)
+{
+    if (!ctx || !h || !u || nsteps < 0) return fail(MSW_EINVAL, "bad argument");
+    int rc = check_cfg(ctx);
+    if (rc) return rc;
+    if (ctx->prec == MSW_FP64)
+        return run_steps(ctx, 1, h, u, nullptr, nullptr, nsteps,
+                         [&] { enqueue_fwd_step<double>(ctx, static_cast<double*>(h), static_cast<double*>(u)); });
+    return run_steps(ctx, 2, h, u, nullptr, nullptr, nsteps,
+                     [&] { enqueue_fwd_step<float>(ctx, static_cast<float*>(h), static_cast<float*>(u)); });
+}
+
+int msw_base_steps(msw_ctx* ctx, double* h, double* u, int nsteps, double* traj_h, double* traj_u)
+{
+    if (!ctx || !h || !u || nsteps < 0 || (!traj_h) != (!traj_u)) return fail(MSW_EINVAL, "bad argument");
+    int rc = check_cfg(ctx);
+    if (rc) return rc;
+    const size_t C = ctx->nC, E = ctx->nE;
+    std::function<void(int)> pre = [&](int i) {
+        (void)hipMemcpyAsync(traj_h + i * C, h, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+        (void)hipMemcpyAsync(traj_u + i * E, u, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+    };
+    return run_steps(ctx, 3, h, u, nullptr, nullptr, nsteps, [&] { enqueue_base_step(ctx, h, u); },
+                     traj_h ? &pre : nullptr);
+}
+
+int msw_tlm_steps(msw_ctx* ctx, double* h, double* u, void* h_tl, void* u_tl, int nsteps,
+                  double* traj_h, double* traj_u)
+{
+    if (!ctx || !h || !u || !h_tl || !u_tl || nsteps < 0 || (!traj_h) != (!traj_u))
+        return fail(MSW_EINVAL, "bad argument");
+    int rc = check_cfg(ctx);
+    if (rc) return rc;
+    const size_t C = ctx->nC, E = ctx->nE;
+    std::function<void(int)> pre = [&](int i) {
+        (void)hipMemcpyAsync(traj_h + i * C, h, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+        (void)hipMemcpyAsync(traj_u + i * E, u, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+    };
+    const std::function<void(int)>* pp = traj_h ? &pre : nullptr;
+    if (ctx->prec == MSW_FP64)
+        return run_steps(ctx, 4, h, u, h_tl, u_tl, nsteps, [&] {
+            enqueue_tlm_step<double>(ctx, h, u, static_cast<double*>(h_tl), static_cast<double*>(u_tl));
+        }, pp);
+    return run_steps(ctx, 5, h, u, h_tl, u_tl, nsteps, [&] {
+        enqueue_tlm_step<float>(ctx, h, u, static_cast<float*>(h_tl), static_cast<float*>(u_tl));
+    }, pp);
+}
+
+static int adj_common(msw_ctx* ctx, void* h_ad, void* u_ad, int nsteps, bool want_end,
+                      const std::function<void(int)>& pre)
+{
+    if (ctx->dcfg.frozen_u) return fail(MSW_EUNSUPPORTED, "adjoint of test case 1 is not implemented");
+    if (ctx->prec == MSW_FP64)
+        return run_steps(ctx, want_end ? 6 : 7, h_ad, u_ad, nullptr, nullptr, nsteps, [&] {
+            enqueue_adj_step<double>(ctx, static_cast<double*>(h_ad), static_cast<double*>(u_ad), want_end);
+        }, &pre);
+    return run_steps(ctx, want_end ? 8 : 9, h_ad, u_ad, nullptr, nullptr, nsteps, [&] {
+        enqueue_adj_step<float>(ctx, static_cast<float*>(h_ad), static_cast<float*>(u_ad), want_end);
+    }, &pre);
+}
+
+int msw_adj_step(msw_ctx* ctx, const double* h, const double* u, void* h_ad, void* u_ad,
+                 double* h_end, double* u_end)
+{
+    if (!ctx || !h || !u || !h_ad || !u_ad) return fail(MSW_EINVAL, "bad argument");
+    int rc = check_cfg(ctx);
+    if (rc) return rc;
+    const size_t C = ctx->nC, E = ctx->nE;
+    std::function<void(int)> pre = [&](int) {
+        (void)hipMemcpyAsync(ctx->bx, h, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+        (void)hipMemcpyAsync(ctx->bx + C, u, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+    };
+    const bool want_end = h_end || u_end;
+    rc = adj_common(ctx, h_ad, u_ad, 1, want_end, pre);
+    if (rc) return rc;
+    if (h_end) HIP_TRY(hipMemcpyAsync(h_end, ctx->bend, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if (u_end) HIP_TRY(hipMemcpyAsync(u_end, ctx->bend + C, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return MSW_OK;
+}
+
+int msw_adj_steps(msw_ctx* ctx, const double* traj_h, const double* traj_u, void* h_ad, void* u_ad,
+                  int nsteps)
+{
+    if (!ctx || !traj_h || !traj_u || !h_ad || !u_ad || nsteps < 0) return fail(MSW_EINVAL, "bad argument");
+    int rc = check_cfg(ctx);
+    if (rc) return rc;
+    const size_t C = ctx->nC, E = ctx->nE;
+    std::function<void(int)> pre = [&](int i) {
+        const size_t k = static_cast<size_t>(nsteps - 1 - i);
+        (void)hipMemcpyAsync(ctx->bx, traj_h + k * C, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+        (void)hipMemcpyAsync(ctx->bx + C, traj_u + k * E, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+    };
+    return adj_common(ctx, h_ad, u_ad, nsteps, false, pre);
+}
+
+static const char* kDiagNames =
+    "v,h_edge,h_vertex,circulation,vorticity,divergence,ke,pv_edge,pv_vertex,pv_cell,"
+    "vorticity_cell,gradPVn,gradPVt";
+static const char* kDiagKinds = "e,e,v,v,v,c,c,e,v,c,c,e,e";
+
+const char* msw_diag_names(void) { return kDiagNames; }
+const char* msw_diag_kinds(void) { return kDiagKinds; }
+
+int msw_diagnostics(msw_ctx* ctx, const double* h, const double* u, double* out)
+{
+    if (!ctx || !h || !u || !out) return fail(MSW_EINVAL, "bad argument");
+    const size_t C = ctx->nC, E = ctx->nE, V = ctx->nV;
+    double* o = out;
+    double* v = o; o += E;
+    double* he = o; o += E;
+    double* hv = o; o += V;
+    double* circ = o; o += V;
+    double* vort = o; o += V;
+    double* div = o; o += C;
+    double* ke = o; o += C;
+    double* pe = o; o += E;
+    double* pvv = o; o += V;
+    double* pc = o; o += C;
+    double* vc = o; o += C;
+    double* gpn = o; o += E;
+    double* gpt = o; o += E;
+    fwd_diag<double>(ctx->dm, ctx->dcfg, 0, h, u, pvv, hv, vort, ke, pc, he, pe, v, gpt, gpn, ctx->stream);
+    diag_extra(ctx->dm, u, vort, circ, div, vc, ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return MSW_OK;
+}
+
+int msw_reconstruct(msw_ctx* ctx, const void* u, void* out5)
+{
+    if (!ctx || !u || !out5) return fail(MSW_EINVAL, "bad argument");
+    if (ctx->prec == MSW_FP64)
+        reconstruct<double>(ctx->nC, ctx->lb, ctx->dm.nEoC, ctx->dm.eoc, ctx->coeffs, ctx->trig,
+                            static_cast<const double*>(u), static_cast<double*>(out5), ctx->stream);
+    else
+        reconstruct<float>(ctx->nC, ctx->lb, ctx->dm.nEoC, ctx->dm.eoc, ctx->coeffs, ctx->trig,
+                           static_cast<const float*>(u), static_cast<float*>(out5), ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return MSW_OK;
+}
+
+int msw_dot(msw_ctx* ctx, const void* xh, const void* xu, const void* yh, const void* yu,
+            const double* wh, const double* wu, double* out)
+{
+    if (!ctx || !out || (!xh && !xu)) return fail(MSW_EINVAL, "bad argument");
+    bool acc = false;
+    auto run = [&](long n, const void* x, const void* y, const double* w) {
+        if (ctx->prec == MSW_FP64)
+            dot_batched<double>(n, ctx->lb, static_cast<const double*>(x), static_cast<const double*>(y), w,
+                                ctx->dpartial, ctx->dot_nblk, out, acc ? 1 : 0, ctx->stream);
+        else
+            dot_batched<float>(n, ctx->lb, static_cast<const float*>(x), static_cast<const float*>(y), w,
+                               ctx->dpartial, ctx->dot_nblk, out, acc ? 1 : 0, ctx->stream);
+        acc = true;
+    };
+    if (xh) run(ctx->nC, xh, yh, wh);
+    if (xu) run(ctx->nE, xu, yu, wu);
+    HIP_TRY(hipGetLastError());
+    return MSW_OK;
+}
+
+int msw_nonfinite(msw_ctx* ctx, const void* h, const void* u, int* flags)
+{
+    if (!ctx || !flags) return fail(MSW_EINVAL, "bad argument");
+    if (ctx->prec == MSW_FP64) {
+        if (h) nonfinite_flags<double>(ctx->nC, ctx->lb, static_cast<const double*>(h), flags, ctx->stream);
+        if (u) nonfinite_flags<double>(ctx->nE, ctx->lb, static_cast<const double*>(u), flags, ctx->stream);
+    } else {
+        if (h) nonfinite_flags<float>(ctx->nC, ctx->lb, static_cast<const float*>(h), flags, ctx->stream);
+        if (u) nonfinite_flags<float>(ctx->nE, ctx->lb, static_cast<const float*>(u), flags, ctx->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    return MSW_OK;
+}
+
+int msw_permute(msw_ctx* ctx, int kind, int to_internal, int prec, int batch, const void* src, void* dst)
+{
+    if (!ctx || !src || !dst || src == dst) return fail(MSW_EINVAL, "bad argument");
+    if (!is_pow2(batch) || batch > 1024) return fail(MSW_EINVAL, "batch must be a power of two <= 1024");
+    int lb = 0;
+    while ((1 << lb) < batch) ++lb;
+    const int* perm = nullptr;
+    long n = 0;
+    if (kind == 'c') { perm = ctx->permC; n = ctx->nC; }
+    else if (kind == 'e') { perm = ctx->permE; n = ctx->nE; }
+    else if (kind == 'v') { perm = ctx->permV; n = ctx->nV; }
+    else return fail(MSW_EINVAL, "kind must be 'c', 'e' or 'v'");
+    if (prec == MSW_FP64) {
+        if (to_internal) gather_rows<double>(n, lb, perm, static_cast<const double*>(src), static_cast<double*>(dst), ctx->stream);
+        else scatter_rows<double>(n, lb, perm, static_cast<const double*>(src), static_cast<double*>(dst), ctx->stream);
+    } else if (prec == MSW_FP32) {
+        if (to_internal) gather_rows<float>(n, lb, perm, static_cast<const float*>(src), static_cast<float*>(dst), ctx->stream);
+        else scatter_rows<float>(n, lb, perm, static_cast<const float*>(src), static_cast<float*>(dst), ctx->stream);
+    } else {
+        return fail(MSW_EINVAL, "bad precision");
+    }
+    HIP_TRY(hipGetLastError());
+    return MSW_OK;
+}
+
+int64_t msw_device_bytes(const msw_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+"""ctypes binding of libmpas_sw_amd.so (the C ABI in include/mpas_sw_amd.h).
+
+The shared library is built in-tree by :func:`build` (hipcc, gfx950).  Loading
+fails loudly when it is missing: there is no CPU fallback for the stepped
+path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(PKG_ROOT, "csrc")
+REPO_ROOT = os.path.dirname(PKG_ROOT)
+LIB_PATH = os.path.join(HERE, "libmpas_sw_amd.so")
+
+SOURCES = ("msw_kernels.hip", "msw_runtime.cpp")
+HIPCC_FLAGS = [
+    "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",
+    # FP contraction off: gathers reproduce the reference's rounding exactly
+    "-ffp-contract=off",
+    # no packed FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+]
+
+MSW_FP64 = 0
+MSW_FP32 = 1
+
+
+def build(verbose: bool = False) -> str:
+    """Compile the HIP library in-tree; returns its path."""
+    hipcc = os.environ.get("HIPCC", "hipcc")
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB_PATH + ".tmp"
+    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, *srcs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, LIB_PATH)
+    if verbose:
+        print(" ".join(cmd))
+    return LIB_PATH
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("nCells", C.c_int32), ("nEdges", C.c_int32), ("nVertices", C.c_int32),
+        ("maxEdges", C.c_int32), ("maxEdges2", C.c_int32), ("vertexDegree", C.c_int32),
+        ("cellsOnEdge", C.c_void_p), ("verticesOnEdge", C.c_void_p),
+        ("nEdgesOnCell", C.c_void_p), ("edgesOnCell", C.c_void_p),
+        ("nEdgesOnEdge", C.c_void_p), ("edgesOnEdge", C.c_void_p),
+        ("weightsOnEdge", C.c_void_p), ("cellsOnVertex", C.c_void_p),
+        ("kiteAreasOnVertex", C.c_void_p), ("dcEdge", C.c_void_p), ("dvEdge", C.c_void_p),
+        ("areaCell", C.c_void_p), ("areaTriangle", C.c_void_p), ("fVertex", C.c_void_p),
+        ("h_s", C.c_void_p), ("latCell", C.c_void_p), ("lonCell", C.c_void_p),
+        ("coeffs_reconstruct", C.c_void_p),
+        ("cellPerm", C.c_void_p), ("edgePerm", C.c_void_p), ("vertexPerm", C.c_void_p),
+    ]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("apvm_upwinding", C.c_double), ("visc2", C.c_double),
+        ("visc4", C.c_double), ("test_case", C.c_int32), ("thickness_adv_order", C.c_int32),
+        ("monotonic", C.c_int32), ("wind_stress", C.c_int32), ("bottom_drag", C.c_int32),
+    ]
+
+
+# every symbol include/mpas_sw_amd.h declares
+EXPORTS = (
+    "msw_create", "msw_destroy", "msw_set_config", "msw_set_graphs", "msw_fwd_steps",
+    "msw_base_steps", "msw_tlm_steps", "msw_adj_step", "msw_adj_steps", "msw_diagnostics",
+    "msw_diag_names", "msw_diag_kinds", "msw_reconstruct", "msw_dot", "msw_nonfinite",
+    "msw_permute", "msw_device_bytes", "msw_abi_version", "msw_last_error",
+)
+
+_lib = None
+
+
+class MSWError(RuntimeError):
+    """A negative status from the HIP library, with its message."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (status {code})")
+        self.code = code
+
+
+def load():
+    """Load the library (no GPU call happens here)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.isfile(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(the GPU path has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    vp, i32, d, i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+    sig = {
+        "msw_create": ([C.POINTER(MeshDesc), C.POINTER(Config), i32, i32, vp, C.POINTER(vp)], i32),
+        "msw_destroy": ([vp], i32),
+        "msw_set_config": ([vp, C.POINTER(Config)], i32),
+        "msw_set_graphs": ([vp, i32], i32),
+        "msw_fwd_steps": ([vp, vp, vp, i32], i32),
+        "msw_base_steps": ([vp, vp, vp, i32, vp, vp], i32),
+        "msw_tlm_steps": ([vp, vp, vp, vp, vp, i32, vp, vp], i32),
+        "msw_adj_step": ([vp, vp, vp, vp, vp, vp, vp], i32),
+        "msw_adj_steps": ([vp, vp, vp, vp, vp, i32], i32),
+        "msw_diagnostics": ([vp, vp, vp, vp], i32),
+        "msw_diag_names": ([], C.c_char_p),
+        "msw_diag_kinds": ([], C.c_char_p),
+        "msw_reconstruct": ([vp, vp, vp], i32),
+        "msw_dot": ([vp, vp, vp, vp, vp, vp, vp, vp], i32),
+        "msw_nonfinite": ([vp, vp, vp, vp], i32),
+        "msw_permute": ([vp, i32, i32, i32, i32, vp, vp], i32),
+        "msw_device_bytes": ([vp], i64),
+        "msw_abi_version": ([], i32),
+        "msw_last_error": ([], C.c_char_p),
+    }
+    for name, (argt, rest) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = argt
+        f.restype = rest
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().msw_last_error().decode(errors="replace")
+        raise MSWError(rc, msg)
+    return rc

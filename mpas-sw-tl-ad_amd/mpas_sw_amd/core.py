@@ -1,0 +1,268 @@
+"""Torch-facing host API over the HIP library.
+
+``SWModel`` owns one library context (one mesh, one batch size, one member
+precision) and steps caller-owned torch tensors on the GPU:
+
+* states are in the library's internal (space-filling-curve) entity order,
+  shape ``[n]`` for a single member or ``[n, B]`` with members innermost;
+* nonlinear base states of the TLM/adjoint are float64 ``[n]``;
+* member arrays are float64 or float32 (``precision``).
+
+Host helpers convert reference-order numpy arrays to internal-order device
+tensors and back.  Nothing here computes model physics on the CPU: every
+step, diagnostic and reduction is a library call.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .mesh import Mesh, sfc_permutation
+
+_DT = {"fp64": (torch.float64, _lib.MSW_FP64), "fp32": (torch.float32, _lib.MSW_FP32)}
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class SWModel:
+    """One mesh + configuration on the GPU.
+
+    Parameters
+    ----------
+    mesh : Mesh in reference (generator/file) order, unit sphere.
+    phys : dict with the test-case-scaled fields: dcEdge, dvEdge, areaCell,
+        areaTriangle, kiteAreasOnVertex (3,V), fVertex, h_s, and optionally
+        lonCell and coeffs_reconstruct (3, maxEdges, C).
+    dt, apvm, test_case, visc2, visc4, ... : the reference's config_* values.
+    batch : members per entity (power of two, 1..256).
+    precision : "fp64" or "fp32" for member arrays.
+    sfc : reorder entities along a 3-D Hilbert curve (default) or keep the
+        reference order.
+    """
+
+    def __init__(self, mesh: Mesh, phys: dict, dt: float, *, apvm: float = 0.5, test_case: int = 2,
+                 visc2: float = 0.0, visc4: float = 0.0, thickness_adv_order: int = 2,
+                 monotonic: bool = False, wind_stress: bool = False, bottom_drag: bool = False,
+                 batch: int = 1, precision: str = "fp64", sfc: bool = True, device=None,
+                 graphs: bool = False):
+        if precision not in _DT:
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.lib = _lib.load()
+        self.mesh = mesh
+        self.batch = int(batch)
+        self.precision = precision
+        self.dtype, self._prec = _DT[precision]
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.nC, self.nE, self.nV = mesh.nCells, mesh.nEdges, mesh.nVertices
+        if sfc:
+            pc, pe, pv = sfc_permutation(mesh)
+        else:
+            pc, pe, pv = (np.arange(self.nC), np.arange(self.nE), np.arange(self.nV))
+        self.perm = {"c": np.ascontiguousarray(pc, np.int32), "e": np.ascontiguousarray(pe, np.int32),
+                     "v": np.ascontiguousarray(pv, np.int32)}
+        self.cfg = _lib.Config(dt, apvm, visc2, visc4, test_case, thickness_adv_order,
+                               int(monotonic), int(wind_stress), int(bottom_drag))
+        keep = []
+
+        def f8(x):
+            a = np.ascontiguousarray(x, dtype=np.float64)
+            keep.append(a)
+            return a.ctypes.data
+
+        def i4(x):
+            a = np.ascontiguousarray(x, dtype=np.int32)
+            keep.append(a)
+            return a.ctypes.data
+
+        d = _lib.MeshDesc()
+        d.nCells, d.nEdges, d.nVertices = self.nC, self.nE, self.nV
+        d.maxEdges, d.maxEdges2, d.vertexDegree = mesh.maxEdges, mesh.maxEdges2, mesh.vertexDegree
+        d.cellsOnEdge = i4(mesh.cellsOnEdge.T)
+        d.verticesOnEdge = i4(mesh.verticesOnEdge.T)
+        d.nEdgesOnCell = i4(mesh.nEdgesOnCell)
+        d.edgesOnCell = i4(mesh.edgesOnCell.T)
+        d.nEdgesOnEdge = i4(mesh.nEdgesOnEdge)
+        d.edgesOnEdge = i4(mesh.edgesOnEdge.T)
+        d.weightsOnEdge = f8(mesh.weightsOnEdge.T)
+        d.cellsOnVertex = i4(mesh.cellsOnVertex.T)
+        kite = np.asarray(phys["kiteAreasOnVertex"])
+        d.kiteAreasOnVertex = f8(kite.T if kite.ndim == 2 else kite)
+        for name in ("dcEdge", "dvEdge", "areaCell", "areaTriangle", "fVertex", "h_s"):
+            setattr(d, name, f8(phys[name]))
+        d.latCell = f8(mesh.latCell)
+        d.lonCell = f8(phys.get("lonCell", mesh.lonCell))
+        cr = phys.get("coeffs_reconstruct")
+        if cr is not None:
+            cr = np.asarray(cr, np.float64)
+            # (3, maxEdges, C) reference layout -> entity-major flat
+            d.coeffs_reconstruct = f8(np.transpose(cr, (2, 1, 0)) if cr.ndim == 3 else cr)
+        d.cellPerm = self.perm["c"].ctypes.data
+        d.edgePerm = self.perm["e"].ctypes.data
+        d.vertexPerm = self.perm["v"].ctypes.data
+        with torch.cuda.device(self.device):
+            self.stream = torch.cuda.current_stream(self.device)
+            h = C.c_void_p()
+            _lib.check(self.lib.msw_create(C.byref(d), C.byref(self.cfg), self._prec, self.batch,
+                                           C.c_void_p(self.stream.cuda_stream), C.byref(h)))
+        self._h = h
+        if graphs:
+            self.set_graphs(True)
+        self._inv = {k: np.argsort(v).astype(np.int64) for k, v in self.perm.items()}
+
+    # ------------------------------------------------------------------ utils
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.msw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_graphs(self, on: bool):
+        _lib.check(self.lib.msw_set_graphs(self._h, int(bool(on))))
+
+    def set_config(self, dt=None, apvm=None):
+        if dt is not None:
+            self.cfg.dt = dt
+        if apvm is not None:
+            self.cfg.apvm_upwinding = apvm
+        _lib.check(self.lib.msw_set_config(self._h, C.byref(self.cfg)))
+
+    @property
+    def device_bytes(self) -> int:
+        return int(self.lib.msw_device_bytes(self._h))
+
+    def n(self, kind):
+        return {"c": self.nC, "e": self.nE, "v": self.nV}[kind]
+
+    def _shape(self, kind, batch=None):
+        b = self.batch if batch is None else batch
+        n = self.n(kind)
+        return (n,) if b == 1 else (n, b)
+
+    def empty(self, kind, member=True, batch=None):
+        """Uninitialised internal-order device tensor for entity kind c/e/v."""
+        dt = self.dtype if member else torch.float64
+        b = (self.batch if batch is None else batch) if member else 1
+        return torch.empty(self._shape(kind, b), dtype=dt, device=self.device)
+
+    def zeros(self, kind, member=True, batch=None):
+        t = self.empty(kind, member, batch)
+        t.zero_()
+        return t
+
+    def to_internal(self, arr, kind, member=True):
+        """Reference-order host array ([n] or [n,B]) -> internal-order device tensor."""
+        a = np.asarray(arr)[self.perm[kind]]
+        dt = self.dtype if member else torch.float64
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=self.device)
+
+    def to_reference(self, t, kind):
+        """Internal-order device tensor -> reference-order host float64 array."""
+        a = t.detach().to("cpu", torch.float64).numpy()
+        return a[self._inv[kind]]
+
+    def _check(self, t, kind, member=True, name="array"):
+        dt = self.dtype if member else torch.float64
+        shp = self._shape(kind, self.batch if member else 1)
+        if t.dtype != dt or tuple(t.shape) != shp or not t.is_contiguous() or t.device != self.device:
+            raise ValueError(f"{name}: expected contiguous {dt} {shp} on {self.device}, got "
+                             f"{t.dtype} {tuple(t.shape)} on {t.device}")
+
+    # ----------------------------------------------------------------- steps
+    def fwd(self, h, u, nsteps=1):
+        """sw_rk4 x nsteps on the batch (h [C,B], u [E,B] of `precision`), in place."""
+        self._check(h, "c", name="h"); self._check(u, "e", name="u")
+        _lib.check(self.lib.msw_fwd_steps(self._h, _ptr(h), _ptr(u), int(nsteps)))
+
+    def base(self, h, u, nsteps=1, traj_h=None, traj_u=None):
+        """FP64 single-member forward, in place; optional trajectory storage
+        (traj_h [nsteps, C], traj_u [nsteps, E] get the step-start states)."""
+        self._check(h, "c", False, "h"); self._check(u, "e", False, "u")
+        self._check_traj(traj_h, traj_u, nsteps)
+        _lib.check(self.lib.msw_base_steps(self._h, _ptr(h), _ptr(u), int(nsteps), _ptr(traj_h), _ptr(traj_u)))
+
+    def tlm(self, h, u, h_tl, u_tl, nsteps=1, traj_h=None, traj_u=None):
+        """sw_rk4_tlm x nsteps: advances the FP64 base (h,u) and the batched
+        TL state (h_tl,u_tl) in place."""
+        self._check(h, "c", False, "h"); self._check(u, "e", False, "u")
+        self._check(h_tl, "c", name="h_tl"); self._check(u_tl, "e", name="u_tl")
+        self._check_traj(traj_h, traj_u, nsteps)
+        _lib.check(self.lib.msw_tlm_steps(self._h, _ptr(h), _ptr(u), _ptr(h_tl), _ptr(u_tl), int(nsteps),
+                                          _ptr(traj_h), _ptr(traj_u)))
+
+    def adj_step(self, h, u, h_ad, u_ad, want_end=False):
+        """sw_rk4_adj: (h,u) base at the step start; (h_ad,u_ad) updated in
+        place from the step-end adjoint to the step-start adjoint.  Returns
+        the nonlinear end state if ``want_end``."""
+        self._check(h, "c", False, "h"); self._check(u, "e", False, "u")
+        self._check(h_ad, "c", name="h_ad"); self._check(u_ad, "e", name="u_ad")
+        he = ue = None
+        if want_end:
+            he = torch.empty_like(h)
+            ue = torch.empty_like(u)
+        _lib.check(self.lib.msw_adj_step(self._h, _ptr(h), _ptr(u), _ptr(h_ad), _ptr(u_ad), _ptr(he), _ptr(ue)))
+        return (he, ue) if want_end else None
+
+    def adj(self, traj_h, traj_u, h_ad, u_ad, nsteps):
+        """nsteps adjoint steps backwards over a stored trajectory, in place."""
+        self._check_traj(traj_h, traj_u, nsteps, required=True)
+        self._check(h_ad, "c", name="h_ad"); self._check(u_ad, "e", name="u_ad")
+        _lib.check(self.lib.msw_adj_steps(self._h, _ptr(traj_h), _ptr(traj_u), _ptr(h_ad), _ptr(u_ad), int(nsteps)))
+
+    def _check_traj(self, th, tu, nsteps, required=False):
+        if th is None and tu is None and not required:
+            return
+        if th is None or tu is None:
+            raise ValueError("traj_h and traj_u go together")
+        for t, n in ((th, self.nC), (tu, self.nE)):
+            if t.dtype != torch.float64 or not t.is_contiguous() or t.device != self.device or t.numel() < nsteps * n:
+                raise ValueError("trajectory buffers must be contiguous float64 [nsteps, n] on the device")
+
+    # ------------------------------------------------------------ diagnostics
+    def diagnostics(self, h, u):
+        """All 13 diagnostic fields of one FP64 state, internal order."""
+        self._check(h, "c", False, "h"); self._check(u, "e", False, "u")
+        names = self.lib.msw_diag_names().decode().split(",")
+        kinds = self.lib.msw_diag_kinds().decode().split(",")
+        sizes = [self.n(k) for k in kinds]
+        out = torch.empty(sum(sizes), dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.msw_diagnostics(self._h, _ptr(h), _ptr(u), _ptr(out)))
+        res, off = {}, 0
+        for nm, k, s in zip(names, kinds, sizes):
+            res[nm] = (out[off:off + s], k)
+            off += s
+        return res
+
+    def reconstruct(self, u):
+        """[5, C(, B)] = X, Y, Z, zonal, meridional of the batch `u`."""
+        self._check(u, "e", name="u")
+        out = torch.empty((5,) + self._shape("c"), dtype=self.dtype, device=self.device)
+        _lib.check(self.lib.msw_reconstruct(self._h, _ptr(u), _ptr(out)))
+        return out
+
+    def dot(self, xh=None, xu=None, yh=None, yu=None, wh=None, wu=None):
+        """Per-member FP64 inner products (see msw_dot); returns float64 [B]."""
+        out = torch.empty(self.batch, dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.msw_dot(self._h, _ptr(xh), _ptr(xu), _ptr(yh), _ptr(yu), _ptr(wh), _ptr(wu), _ptr(out)))
+        return out
+
+    def nonfinite(self, h, u):
+        flags = torch.zeros(self.batch, dtype=torch.int32, device=self.device)
+        _lib.check(self.lib.msw_nonfinite(self._h, _ptr(h), _ptr(u), _ptr(flags)))
+        return flags
+
+    def permute(self, src, kind, to_internal, batch=1):
+        dst = torch.empty_like(src)
+        prec = _lib.MSW_FP64 if src.dtype == torch.float64 else _lib.MSW_FP32
+        _lib.check(self.lib.msw_permute(self._h, ord(kind), int(to_internal), prec, int(batch), _ptr(src), _ptr(dst)))
+        return dst

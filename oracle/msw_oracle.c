@@ -9,6 +9,7 @@
  * golden vectors; closed forms (RK4 on a linear ODE, mass conservation) and
  * the TLM/adjoint identities in tests/test_oracle_math.py.
  */
+#define _GNU_SOURCE
 #include "msw_oracle.h"
 
 #include <math.h>
@@ -816,8 +817,9 @@ static void recon_core(mswo_ctx *c, const double *u, reconset *r)
         }
     }
     for (int i = 0; i < c->C; ++i) {
-        double clat = cos(c->latC[i]), slat = sin(c->latC[i]);
-        double clon = cos(c->lonC[i]), slon = sin(c->lonC[i]);
+        double clat, slat, clon, slon;  /* the reference's compiler emits sincos */
+        sincos(c->latC[i], &slat, &clat);
+        sincos(c->lonC[i], &slon, &clon);
         r->Zonal[i] = -r->X[i] * slon + r->Y[i] * clon;
         r->Meridional[i] = -(r->X[i] * clon + r->Y[i] * slon) * slat + r->Z[i] * clat;
     }
@@ -837,8 +839,9 @@ void mswo_reconstruct_adj(mswo_ctx *c, const double *u, double *u_ad)
 {
     reconset *a = &c->rad;
     for (int i = c->C - 1; i >= 0; --i) {
-        double clat = cos(c->latC[i]), slat = sin(c->latC[i]);
-        double clon = cos(c->lonC[i]), slon = sin(c->lonC[i]);
+        double clat, slat, clon, slon;  /* the reference's compiler emits sincos */
+        sincos(c->latC[i], &slat, &clat);
+        sincos(c->lonC[i], &slon, &clon);
         a->Z[i] = a->Z[i] + clat * a->Meridional[i];
         a->Y[i] = a->Y[i] - slon * slat * a->Meridional[i];
         a->X[i] = a->X[i] - clon * slat * a->Meridional[i];

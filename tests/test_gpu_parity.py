@@ -1,0 +1,158 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle (which is
+bit-exact with the reference Fortran, see test_oracle_vs_reference.py).
+
+Tolerances: the FP64 forward and TLM gather in the reference's accumulation
+order with FP contraction off, so they must match to the bit.  The adjoint is
+a gather-form transpose whose sums run in a different order than the
+reference's reverse scatter loops; it must match to 1e-11 relative L2.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import case, make_oracle, perturbations, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+LEVEL = 3
+CASES = [(2, 1200.0), (5, 900.0), (6, 600.0)]
+
+
+def model(level, tc, dt, **kw):
+    from mpas_sw_amd.core import SWModel
+    mesh, phys = case(level, tc)
+    return SWModel(mesh, phys, dt, test_case=tc, **kw)
+
+
+@pytest.mark.parametrize("tc,dt", CASES)
+def test_forward_bitwise(tc, dt):
+    print(f"forward L{LEVEL} tc{tc}", flush=True)
+    mesh, phys = case(LEVEL, tc)
+    o = make_oracle(LEVEL, tc, dt)
+    o.rk4(10)
+    m = model(LEVEL, tc, dt)
+    h = m.to_internal(phys["h"], "c")
+    u = m.to_internal(phys["u"], "e")
+    m.fwd(h, u, 10)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(m.to_reference(h, "c"), o["h1"])
+    np.testing.assert_array_equal(m.to_reference(u, "e"), o["u1"])
+
+
+@pytest.mark.parametrize("tc,dt", CASES)
+def test_base_and_diagnostics_bitwise(tc, dt):
+    print(f"base+diag L{LEVEL} tc{tc}", flush=True)
+    mesh, phys = case(LEVEL, tc)
+    o = make_oracle(LEVEL, tc, dt)
+    o.rk4(3)
+    m = model(LEVEL, tc, dt, batch=4)
+    h = m.to_internal(phys["h"], "c", member=False)
+    u = m.to_internal(phys["u"], "e", member=False)
+    m.base(h, u, 3)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(m.to_reference(h, "c"), o["h1"])
+    np.testing.assert_array_equal(m.to_reference(u, "e"), o["u1"])
+    # the reference's diag after sw_rk4 is the diag of the new state
+    d = m.diagnostics(h, u)
+    for name, (t, kind) in d.items():
+        np.testing.assert_array_equal(m.to_reference(t, kind), o[name], err_msg=name)
+
+
+@pytest.mark.parametrize("tc,dt", CASES)
+def test_tlm_bitwise(tc, dt):
+    print(f"tlm L{LEVEL} tc{tc}", flush=True)
+    mesh, phys = case(LEVEL, tc)
+    dh, du = perturbations(mesh, 1)
+    o = make_oracle(LEVEL, tc, dt)
+    o["h_tl1"][:] = dh[:, 0]
+    o["u_tl1"][:] = du[:, 0]
+    o.rk4_tlm(6)
+    m = model(LEVEL, tc, dt)
+    h = m.to_internal(phys["h"], "c", member=False)
+    u = m.to_internal(phys["u"], "e", member=False)
+    ht = m.to_internal(dh[:, 0], "c")
+    ut = m.to_internal(du[:, 0], "e")
+    m.tlm(h, u, ht, ut, 6)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(m.to_reference(ht, "c"), o["h_tl1"])
+    np.testing.assert_array_equal(m.to_reference(ut, "e"), o["u_tl1"])
+    np.testing.assert_array_equal(m.to_reference(h, "c"), o["h1"])
+
+
+@pytest.mark.parametrize("tc,dt", CASES)
+def test_adjoint_matches_oracle(tc, dt):
+    print(f"adj L{LEVEL} tc{tc}", flush=True)
+    mesh, phys = case(LEVEL, tc)
+    dh, du = perturbations(mesh, 1)
+    nsteps = 4
+    o = make_oracle(LEVEL, tc, dt)
+    traj = []
+    for _ in range(nsteps):
+        traj.append((o["h1"].copy(), o["u1"].copy()))
+        o.rk4()
+    o["h_ad1"][:] = dh[:, 0]
+    o["u_ad1"][:] = du[:, 0]
+    for s in range(nsteps - 1, -1, -1):
+        o["h1"][:] = traj[s][0]
+        o["u1"][:] = traj[s][1]
+        o.rk4_adj()
+    m = model(LEVEL, tc, dt)
+    h = m.to_internal(phys["h"], "c", member=False)
+    u = m.to_internal(phys["u"], "e", member=False)
+    th = torch.empty((nsteps, m.nC), dtype=torch.float64, device=m.device)
+    tu = torch.empty((nsteps, m.nE), dtype=torch.float64, device=m.device)
+    m.base(h, u, nsteps, th, tu)
+    ha = m.to_internal(dh[:, 0], "c")
+    ua = m.to_internal(du[:, 0], "e")
+    m.adj(th, tu, ha, ua, nsteps)
+    torch.cuda.synchronize()
+    assert rel_l2(m.to_reference(ha, "c"), o["h_ad1"]) < 1e-11
+    assert rel_l2(m.to_reference(ua, "e"), o["u_ad1"]) < 1e-11
+
+
+def test_batched_members_equal_single_runs():
+    print("batched vs single L3 tc5", flush=True)
+    tc, dt = 5, 900.0
+    mesh, phys = case(LEVEL, tc)
+    B = 8
+    dh, du = perturbations(mesh, B)
+    mb = model(LEVEL, tc, dt, batch=B)
+    h = mb.to_internal(phys["h"], "c", member=False)
+    u = mb.to_internal(phys["u"], "e", member=False)
+    ht = mb.to_internal(dh, "c")
+    ut = mb.to_internal(du, "e")
+    mb.tlm(h, u, ht, ut, 4)
+    m1 = model(LEVEL, tc, dt, batch=1)
+    for b in (0, 3, 7):
+        h1 = m1.to_internal(phys["h"], "c", member=False)
+        u1 = m1.to_internal(phys["u"], "e", member=False)
+        ht1 = m1.to_internal(dh[:, b], "c")
+        ut1 = m1.to_internal(du[:, b], "e")
+        m1.tlm(h1, u1, ht1, ut1, 4)
+        torch.cuda.synchronize()
+        assert torch.equal(ht1, ht[:, b].contiguous())
+        assert torch.equal(ut1, ut[:, b].contiguous())
+
+
+def test_dot_product_identity_fp64():
+    """<L x, y> = <x, L^T y> for one RK4 step of the batched TLM/adjoint."""
+    print("dot-product test L3 tc5 B=4", flush=True)
+    tc, dt = 5, 900.0
+    mesh, phys = case(LEVEL, tc)
+    B = 4
+    xh, xu = perturbations(mesh, B, seed=1)
+    yh, yu = perturbations(mesh, B, seed=100)
+    m = model(LEVEL, tc, dt, batch=B)
+    h0 = m.to_internal(phys["h"], "c", member=False)
+    u0 = m.to_internal(phys["u"], "e", member=False)
+    Lxh, Lxu = m.to_internal(xh, "c"), m.to_internal(xu, "e")
+    h, u = h0.clone(), u0.clone()
+    m.tlm(h, u, Lxh, Lxu, 1)
+    LTyh, LTyu = m.to_internal(yh, "c"), m.to_internal(yu, "e")
+    yh_d, yu_d = LTyh.clone(), LTyu.clone()
+    m.adj_step(h0, u0, LTyh, LTyu)
+    xh_d, xu_d = m.to_internal(xh, "c"), m.to_internal(xu, "e")
+    lhs = m.dot(Lxh, Lxu, yh_d, yu_d).cpu().numpy()
+    rhs = m.dot(xh_d, xu_d, LTyh, LTyu).cpu().numpy()
+    assert np.all(np.abs(lhs - rhs) <= 1e-12 * np.abs(lhs))

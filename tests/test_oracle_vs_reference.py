@@ -1,0 +1,82 @@
+"""Pins the CPU oracle to the reference Fortran.
+
+The golden vectors in tests/golden/ were written by the reference model
+itself (oracle/_ref/msw_ref, compiled from the reference's src/*.f90 files,
+see tests/golden/make_golden.py).  The oracle restates the same loops in the
+same order, so every comparison here is bit-for-bit.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import case, golden, make_oracle, oracle_phys
+
+
+def _oracle_from(level, tc, dt, h0, u0):
+    from oracle.oracle import Oracle
+    mesh, phys = case(level, tc)
+    o = Oracle(mesh, oracle_phys(phys), dt, test_case=tc)
+    o["h1"][:] = h0
+    o["u1"][:] = u0
+    o.init_block()
+    return o
+
+
+def test_tc2_l5_one_day_bitwise():
+    g, meta = golden("tc2_l5_fwd_1day")
+    o = _oracle_from(5, 2, meta["dt"], g["h0"], g["u0"])
+    o.rk4(meta["nsteps"])
+    np.testing.assert_array_equal(o["h1"], g["h"])
+    np.testing.assert_array_equal(o["u1"], g["u"])
+
+
+def test_routines_bitwise():
+    g, meta = golden("tc5_l2_routines")
+    mesh, phys = case(2, 5)
+    o = _oracle_from(2, 5, meta["dt"], g["ic_h"], g["ic_u"])
+    o.diag(g["ic_h"], g["ic_u"])
+    o.tend(g["ic_h"], g["ic_u"])
+    for k in ("v", "h_edge", "h_vertex", "circulation", "vorticity", "divergence", "ke", "pv_edge",
+              "pv_vertex", "pv_cell", "vorticity_cell", "gradPVn", "gradPVt", "tend_h", "tend_u"):
+        np.testing.assert_array_equal(o[k], g[f"diag_{k}"], err_msg=k)
+    o.diag_tlm(g["ic_h"], g["dh"], g["ic_u"], g["du"])
+    o.tend_tlm(g["ic_h"], g["dh"], g["ic_u"], g["du"])
+    for k in ("v_tl", "h_edge_tl", "h_vertex_tl", "vorticity_tl", "divergence_tl", "ke_tl",
+              "pv_edge_tl", "pv_vertex_tl", "pv_cell_tl", "gradPVn_tl", "gradPVt_tl"):
+        np.testing.assert_array_equal(o[k], g[f"diagtl_{k}"], err_msg=k)
+    ha, ua = o.tend_full_adj(g["ic_h"], g["ic_u"], g["gh"], g["gu"])
+    np.testing.assert_array_equal(ha, g["tad_h_ad"])
+    np.testing.assert_array_equal(ua, g["tad_u_ad"])
+
+
+@pytest.mark.parametrize("tc", [5, 6])
+def test_tlm_and_adjoint_windows_bitwise(tc):
+    g, meta = golden(f"tc{tc}_l4_tlm_adj")
+    n, dt = meta["nsteps"], meta["dt"]
+    o = _oracle_from(4, tc, dt, g["h0"], g["u0"])
+    o["h_tl1"][:] = g["dh"]
+    o["u_tl1"][:] = g["du"]
+    o.rk4_tlm(n)
+    np.testing.assert_array_equal(o["h_tl1"], g["h_tl"])
+    np.testing.assert_array_equal(o["u_tl1"], g["u_tl"])
+    np.testing.assert_array_equal(o["h1"], g["h"])
+    o = _oracle_from(4, tc, dt, g["h0"], g["u0"])
+    traj = []
+    for _ in range(n):
+        traj.append((o["h1"].copy(), o["u1"].copy()))
+        o.rk4()
+    o["h_ad1"][:] = g["dh"]
+    o["u_ad1"][:] = g["du"]
+    for s in range(n - 1, -1, -1):
+        o["h1"][:] = traj[s][0]
+        o["u1"][:] = traj[s][1]
+        o.rk4_adj()
+    np.testing.assert_array_equal(o["h_ad1"], g["h_ad"])
+    np.testing.assert_array_equal(o["u_ad1"], g["u_ad"])
+
+
+def test_reconstruction_bitwise():
+    g, meta = golden("l3_initial_conditions")
+    o = make_oracle(3, 2, 600.0)
+    np.testing.assert_array_equal(o["uReconstructZonal"], g["tc2_uReconstructZonal"])
+    np.testing.assert_array_equal(o["uReconstructMeridional"], g["tc2_uReconstructMeridional"])
