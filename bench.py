@@ -1,0 +1,198 @@
+"""Benchmark: batched TLM+adjoint member cell-steps/s on one MI355X.
+
+Workload (the reference configuration the metric is quoted on): level-7
+icosahedral Voronoi mesh (163842 cells, 491520 edges, 327680 vertices),
+Williamson TC6 (Rossby-Haurwitz wave, f = 0 as in the reference), dt = 75 s,
+64 seeded Gaussian perturbations (dh sigma 1 m, du sigma 0.1 m/s) stored
+members-innermost, FP64.
+
+One step = one tangent-linear RK4 step of all 64 members (which also
+advances the shared FP64 nonlinear state and stores it) followed, after the
+K-step TLM sweep, by the adjoint RK4 steps of all 64 members over the same K
+steps in reverse (each recomputing its four nonlinear stage states from the
+stored step-start state).  value = C * B * K / T with T the wall time of both
+sweeps, inputs resident in HBM.
+
+Also reported: the plain unbatched FP64 path (B = 1, same kernels) and its
+ratio, the FP32-member variant, the algorithmic-bytes roofline, and the
+reference Fortran timed on this host's CPU (one core) as cpu_baseline.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mpas-sw-tl-ad_amd"))
+sys.path.insert(0, ROOT)
+
+LEVEL, TC, DT, BATCH = 7, 6, 75.0, 64
+SEED = 20261203
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_STEP = {"f64": 1024.0, "f32": 512.0}   # algorithmic, per member TLM+ADJ cell-step
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def perturbations(mesh, nmem):
+    import numpy as np
+    import torch
+    hs, us = [], []
+    for m in range(nmem):
+        g = torch.Generator("cpu").manual_seed(SEED + m)
+        hs.append(torch.randn(mesh.nCells, generator=g, dtype=torch.float64).numpy() * 1.0)
+        us.append(torch.randn(mesh.nEdges, generator=g, dtype=torch.float64).numpy() * 0.1)
+    return np.stack(hs, 1), np.stack(us, 1)
+
+
+def run_window(m, h0, u0, ht, ut, nsteps, timed):
+    """TLM sweep of nsteps (storing the trajectory) + adjoint sweep back."""
+    import torch
+    h = h0.clone()
+    u = u0.clone()
+    th = torch.empty((nsteps, m.nC), dtype=torch.float64, device=m.device)
+    tu = torch.empty((nsteps, m.nE), dtype=torch.float64, device=m.device)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t0 = time.perf_counter()
+    ev[0].record()
+    m.tlm(h, u, ht, ut, nsteps, th, tu)
+    ev[1].record()
+    # adjoint seeded with the final TL state (the gradient of 1/2|dx(T)|^2)
+    m.adj(th, tu, ht, ut, nsteps)
+    ev[2].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, ev[0].elapsed_time(ev[1]) / 1e3, ev[1].elapsed_time(ev[2]) / 1e3
+
+
+def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True):
+    import torch
+    from mpas_sw_amd.core import SWModel
+    m = SWModel(mesh, phys, DT, test_case=TC, batch=batch, precision=precision, graphs=graphs)
+    h0 = m.to_internal(phys["h"], "c", member=False)
+    u0 = m.to_internal(phys["u"], "e", member=False)
+    sel = slice(0, batch)
+    ht0 = m.to_internal(dh[:, sel] if batch > 1 else dh[:, 0], "c")
+    ut0 = m.to_internal(du[:, sel] if batch > 1 else du[:, 0], "e")
+    if warmup > 0:
+        run_window(m, h0, u0, ht0.clone(), ut0.clone(), warmup, False)
+    ht, ut = ht0.clone(), ut0.clone()
+    wall, t_tlm, t_adj = run_window(m, h0, u0, ht, ut, steps, True)
+    flags = m.nonfinite(ht, ut)
+    ok = int(flags.sum().item()) == 0
+    del m
+    torch.cuda.empty_cache()
+    T = t_tlm + t_adj
+    return dict(seconds=T, wall=wall, t_tlm=t_tlm, t_adj=t_adj,
+                rate=mesh.nCells * batch * steps / T, finite=ok)
+
+
+def cpu_baseline(mesh, phys, dh, du, steps=2):
+    """Reference Fortran (oracle/_ref/msw_ref, one core) TLM and adjoint
+    steps on the same mesh and case; member cell-steps/s of TLM+ADJ."""
+    from oracle import ref
+    if not ref.available():
+        return None
+    t_tlm = ref.run(mesh, "time_tlm", nsteps=steps, test_case=TC, dt=DT, tl=(dh[:, 0], du[:, 0]))["seconds"][0]
+    t_adj = ref.run(mesh, "time_adj", nsteps=steps, test_case=TC, dt=DT, ad=(dh[:, 0], du[:, 0]))["seconds"][0]
+    per_step = (t_tlm + t_adj) / steps
+    return dict(value=mesh.nCells / per_step, unit="member cell-steps/s (TLM+ADJ)", cores=1,
+                kind="reference",
+                sample=f"reference Fortran sw_rk4_tlm x{steps} + sw_rk4_adj x{steps}, level 7 TC6 "
+                       f"dt 75 s, 1 member, amdflang -O2, 1 thread; TLM {t_tlm / steps:.3f} s/step, "
+                       f"ADJ {t_adj / steps:.3f} s/step")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=96)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--level", type=int, default=LEVEL)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; the batched members would shard "
+              "across GPUs with no per-step exchange (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from mpas_sw_amd.mesh import get_mesh
+    from mpas_sw_amd.testcases import initial_conditions
+
+    t = time.time()
+    mesh = get_mesh(args.level)
+    phys = initial_conditions(mesh, TC)
+    dh, du = perturbations(mesh, BATCH)
+    log(f"setup L{args.level}: {time.time() - t:.1f}s; C={mesh.nCells} E={mesh.nEdges} V={mesh.nVertices}")
+
+    res = gpu_rate(mesh, phys, dh, du, BATCH, "fp64", args.steps, args.warmup)
+    log(f"batched fp64: {res}")
+    plain = None
+    if not args.no_plain:
+        plain = gpu_rate(mesh, phys, dh, du, 1, "fp64", args.steps, args.warmup)
+        log(f"plain fp64: {plain}")
+    f32 = gpu_rate(mesh, phys, dh, du, BATCH, "fp32", args.steps, args.warmup)
+    log(f"batched fp32: {f32}")
+
+    K = args.steps
+    bytes_step = BYTES_PER_CELL_STEP["f64"] * mesh.nCells * BATCH
+    achieved = bytes_step * K / res["seconds"] / 1e9
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "kernel": "one TLM+ADJ RK4 step of 64 members (all stage kernels, hipGraph)",
+        "algorithmic_bytes_per_step": bytes_step,
+        "avg_step_ms": round(1e3 * res["seconds"] / K, 4),
+    }
+    prof = os.path.join(ROOT, "profiles", "traffic_tlm_adj_l7_b64_fp64.json")
+    if os.path.isfile(prof):
+        with open(prof) as f:
+            roofline["traffic"] = json.load(f).get("hbm_bytes_per_step")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(mesh, phys, dh, du)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline failed: {e}")
+    out = {
+        "metric": "TLM+ADJ cell-steps/s at 163842 cells, batch 64 vs plain; % HBM peak",
+        "value": res["rate"],
+        "unit": "member cell-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * res["seconds"] / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated level-7 mesh, Williamson TC6, seeded perturbations)",
+        "config": {"workload": "batched TLM+ADJ RK4, level-7 icosahedral Voronoi mesh, TC6, dt 75 s",
+                   "cells": mesh.nCells, "edges": mesh.nEdges, "vertices": mesh.nVertices,
+                   "batch": BATCH, "dt": DT, "window_steps": K},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_fp64": None if plain is None else {"value": plain["rate"], "ms_per_step": 1e3 * plain["seconds"] / K},
+        "batched_vs_plain": None if plain is None else res["rate"] / plain["rate"],
+        "fp32": {"value": f32["rate"], "ms_per_step": 1e3 * f32["seconds"] / K,
+                 "roofline_frac": round(BYTES_PER_CELL_STEP["f32"] * mesh.nCells * BATCH * K /
+                                        f32["seconds"] / 1e9 / HBM_PEAK_GBS, 4)},
+        "tlm_s": res["t_tlm"], "adj_s": res["t_adj"], "finite": res["finite"] and f32["finite"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

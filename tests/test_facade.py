@@ -1,0 +1,113 @@
+"""The drop-in module object: f2py attribute semantics (CPU) and the GPU
+stepping routines against the oracle."""
+
+import importlib
+
+import numpy as np
+import pytest
+
+from helpers import case, make_oracle, perturbations, rel_l2
+
+
+def fresh_module():
+    import module_sw_mpas._facade as f
+    importlib.reload(f)
+    return f.mpas_sw_module
+
+
+def load_mesh(mpsw, mesh):
+    """What src/mpas_sw_driver.py:33-119 does, from a generated mesh."""
+    mpsw.nvertlevels = 1
+    mpsw.ncells, mpsw.nedges, mpsw.nvertices = mesh.nCells, mesh.nEdges, mesh.nVertices
+    mpsw.maxedges, mpsw.maxedges2, mpsw.vertexdegree = 6, 12, 3
+    mpsw.r3 = 3
+    mpsw.ntracers = 1
+    mpsw.nverticessolve = mesh.nVertices          # not a module variable: goes aside
+    mpsw.var_allocation()
+    mpsw.on_a_sphere = True
+    mpsw.is_periodic = False
+    mpsw.sphere_radius = 1.0
+    for name in ("latCell", "lonCell", "xCell", "yCell", "zCell", "latEdge", "lonEdge", "xEdge",
+                 "yEdge", "zEdge", "latVertex", "lonVertex", "xVertex", "yVertex", "zVertex",
+                 "cellsOnEdge", "nEdgesOnCell", "nEdgesOnEdge", "edgesOnCell", "edgesOnEdge",
+                 "weightsOnEdge", "dvEdge", "dcEdge", "angleEdge", "areaCell", "areaTriangle",
+                 "cellsOnCell", "verticesOnCell", "verticesOnEdge", "edgesOnVertex",
+                 "cellsOnVertex", "kiteAreasOnVertex", "meshDensity"):
+        setattr(mpsw, name.lower(), getattr(mesh, name))
+
+
+def test_attribute_semantics():
+    mpsw = fresh_module()
+    assert mpsw.a == 6371229.0 and mpsw.gravity == 9.80616 and mpsw.omega == 7.29212e-5
+    assert mpsw.u is None                      # declared, not allocated
+    mesh, _ = case(2, 2)
+    load_mesh(mpsw, mesh)
+    assert mpsw.u.shape == (2, 1, mesh.nEdges) and mpsw.u.flags.f_contiguous
+    assert mpsw.cellsonedge.dtype == np.int32 and mpsw.cellsonedge.shape == (2, mesh.nEdges)
+    mpsw.u[0, 0] = 3.0                         # aliasing view
+    assert np.all(mpsw.u[0, 0] == 3.0)
+    mpsw.config_dt = "600"                     # cast to the Fortran type
+    assert mpsw.config_dt == 600.0
+    mpsw.config_h_ScaleWithMesh = True         # mixed case never reaches the module
+    assert mpsw.config_h_scalewithmesh is False
+    assert mpsw.nverticessolve == mesh.nVertices
+    with pytest.raises(RuntimeError):
+        mpsw.var_allocation()                  # allocating twice is an error
+    with pytest.raises(AttributeError):
+        mpsw.a = 1.0
+
+
+@pytest.mark.parametrize("tc", [2, 5, 6])
+def test_test_cases_match_reference(tc):
+    from helpers import golden
+    g, _ = golden("l3_initial_conditions")
+    mpsw = fresh_module()
+    mesh, _ = case(3, tc)
+    load_mesh(mpsw, mesh)
+    getattr(mpsw, f"sw_test_case_{tc}")()
+    np.testing.assert_array_equal(mpsw.h[0, 0], g[f"tc{tc}_h"])
+    np.testing.assert_array_equal(mpsw.u[0, 0], g[f"tc{tc}_u"])
+    np.testing.assert_array_equal(mpsw.fvertex, g[f"tc{tc}_fVertex"])
+
+
+@pytest.mark.gpu
+def test_facade_steps_on_gpu():
+    print("facade L3 tc5", flush=True)
+    mesh, phys = case(3, 5)
+    mpsw = fresh_module()
+    load_mesh(mpsw, mesh)
+    mpsw.config_test_case = 5
+    mpsw.config_dt = 900.0
+    mpsw.sw_test_case_5()
+    mpsw.var_allocation_tlm()
+    mpsw.var_allocation_adj()
+    mpsw.sw_mpas_init_block()
+    o = make_oracle(3, 5, 900.0)
+    np.testing.assert_array_equal(mpsw.ureconstructzonal[0], o["uReconstructZonal"])
+    for _ in range(3):
+        mpsw.sw_rk4()
+    o.rk4(3)
+    np.testing.assert_array_equal(mpsw.h[0, 0], o["h1"])
+    np.testing.assert_array_equal(mpsw.u[1, 0], o["u1"])
+    np.testing.assert_array_equal(mpsw.pv_edge[0], o["pv_edge"])
+    np.testing.assert_array_equal(mpsw.ureconstructmeridional[0], o["uReconstructMeridional"])
+    # TLM from the current state
+    dh, du = perturbations(mesh, 1)
+    mpsw.h_tl[0, 0] = dh[:, 0]
+    mpsw.u_tl[0, 0] = du[:, 0]
+    o["h_tl1"][:] = dh[:, 0]
+    o["u_tl1"][:] = du[:, 0]
+    mpsw.sw_rk4_tlm()
+    o.rk4_tlm()
+    np.testing.assert_array_equal(mpsw.h_tl[0, 0], o["h_tl1"])
+    # adjoint of the next step
+    o["h_ad1"][:] = dh[:, 0]
+    o["u_ad1"][:] = du[:, 0]
+    mpsw.h_ad[0, 0] = dh[:, 0]
+    mpsw.u_ad[0, 0] = du[:, 0]
+    mpsw.sw_rk4_adj()
+    o.rk4_adj()
+    assert rel_l2(mpsw.h_ad[0, 0], o["h_ad1"]) < 1e-11
+    np.testing.assert_array_equal(mpsw.h[1, 0], o["h2"])
+    lhs, rhs = mpsw.link_tlm_adj()
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
