@@ -1,0 +1,251 @@
+"""Python driver with the interface of the reference's ``src/mpas_sw_driver.py``,
+over the GPU ``module_sw_mpas``.
+
+Differences, all forced by the environment or by scope:
+
+* grid and output files are NetCDF-3 through ``scipy.io.netcdf_file``
+  (netCDF4 is not available); ``mpas_sw_amd.gridio.write_grid`` writes the
+  generated meshes in that format;
+* test case 7 (ERA5 initialisation, ``:135-217``) needs data files that are
+  not available and is outside the GPU hot path: it raises
+  ``NotImplementedError``;
+* ``run_sw_adj`` keeps the nonlinear trajectory in memory instead of reading
+  a per-step state file (the reference demo's ``x1.10242.state.nc``, which no
+  script writes; ``demo/run-adj.py:17-18``).
+"""
+
+import numpy as np
+
+from datetime import datetime, timedelta
+from pandas import to_timedelta
+from scipy.io import netcdf_file
+
+from module_sw_mpas import mpas_sw_module as mpsw
+from mpas_namelist import namelist
+from mpas_sw_amd import gridio
+
+
+def read_configs(nml):
+    """src/mpas_sw_driver.py:12-30"""
+    nlatm = nml.nlatm
+    mpsw.config_test_case = nlatm['sw_model']['config_test_case']
+    mpsw.config_dt = nlatm['sw_model']['config_dt']
+    mpsw.config_calendar_type = nlatm['sw_model']['config_calendar_type']
+    mpsw.config_stats_interval = nlatm['sw_model']['config_stats_interval']
+    mpsw.config_h_ScaleWithMesh = nlatm['sw_model']['config_h_ScaleWithMesh']
+    mpsw.config_h_mom_eddy_visc2 = nlatm['sw_model']['config_h_mom_eddy_visc2']
+    mpsw.config_h_mom_eddy_visc4 = nlatm['sw_model']['config_h_mom_eddy_visc4']
+    mpsw.config_h_tracer_eddy_diff2 = nlatm['sw_model']['config_h_tracer_eddy_diff2']
+    mpsw.config_h_tracer_eddy_diff4 = nlatm['sw_model']['config_h_tracer_eddy_diff4']
+    mpsw.config_thickness_adv_order = nlatm['sw_model']['config_thickness_adv_order']
+    mpsw.config_tracer_adv_order = nlatm['sw_model']['config_tracer_adv_order']
+    mpsw.config_positive_definite = nlatm['sw_model']['config_positive_definite']
+    mpsw.config_monotonic = nlatm['sw_model']['config_monotonic']
+    mpsw.config_wind_stress = nlatm['sw_model']['config_wind_stress']
+    mpsw.config_bottom_drag = nlatm['sw_model']['config_bottom_drag']
+    mpsw.config_apvm_upwinding = nlatm['sw_model']['config_apvm_upwinding']
+    mpsw.config_num_halos = nlatm['sw_model']['config_num_halos']
+
+
+def read_dims(nml):
+    """src/mpas_sw_driver.py:33-61"""
+    d = gridio.dims(nml.file_t0)
+    mpsw.nvertlevels = d.get('nVertLevels', 1)
+    mpsw.ncells = d['nCells']
+    mpsw.nedges = d['nEdges']
+    mpsw.nvertices = d['nVertices']
+    mpsw.maxedges = d['maxEdges']
+    mpsw.maxedges2 = d['maxEdges2']
+    mpsw.vertexdegree = d['vertexDegree']
+    mpsw.r3 = 3
+    mpsw.ntracers = 1
+    mpsw.ncellssolve = mpsw.ncells
+    mpsw.nedgessolve = mpsw.nedges
+    mpsw.nverticessolve = mpsw.nvertices
+    mpsw.cellstart = 1
+    mpsw.cellend = mpsw.ncells
+    mpsw.vertexstart = 1
+    mpsw.vertexend = mpsw.nvertices
+    mpsw.edgestart = 1
+    mpsw.edgeend = mpsw.nedges
+    mpsw.cellsolvestart = 1
+    mpsw.cellsolveend = mpsw.ncells
+    mpsw.vertexsolvestart = 1
+    mpsw.vertexsolveend = mpsw.nvertices
+    mpsw.edgesolvestart = 1
+    mpsw.edgesolveend = mpsw.nedges
+
+
+def read_vars(nml):
+    """src/mpas_sw_driver.py:64-119"""
+    mpsw.var_allocation()
+    mpsw.on_a_sphere = True
+    mpsw.is_periodic = False
+    mpsw.sphere_radius = gridio.attribute(nml.file_t0, 'sphere_radius', 1.)
+    mesh = gridio.read_grid(nml.file_t0)
+    for name in ('latCell', 'lonCell', 'xCell', 'yCell', 'zCell', 'latEdge', 'lonEdge', 'xEdge',
+                 'yEdge', 'zEdge', 'latVertex', 'lonVertex', 'xVertex', 'yVertex', 'zVertex',
+                 'cellsOnEdge', 'nEdgesOnCell', 'nEdgesOnEdge', 'edgesOnCell', 'edgesOnEdge',
+                 'weightsOnEdge', 'dvEdge', 'dcEdge', 'angleEdge', 'areaCell', 'areaTriangle',
+                 'cellsOnCell', 'verticesOnCell', 'verticesOnEdge', 'edgesOnVertex',
+                 'cellsOnVertex', 'kiteAreasOnVertex', 'meshDensity'):
+        setattr(mpsw, name.lower(), getattr(mesh, name))
+
+
+def clock_namelist(nml):
+    """src/mpas_sw_driver.py:122-132"""
+    nlatm = nml.nlatm
+    clock_start = datetime.strptime(nlatm['sw_model']['config_start_time'], '%Y-%m-%d_%H:%M:%S')
+    clock_duration_str = nlatm['sw_model']['config_run_duration'].replace('_', ' days ')
+    clock_duration = to_timedelta(clock_duration_str)
+    clock_end = clock_start + clock_duration
+    return [clock_start, clock_end]
+
+
+def initial_conditions(nml):
+    """src/mpas_sw_driver.py:220-234"""
+    case_no = nml.nlatm['sw_model']['config_test_case']
+    if case_no == 1:
+        mpsw.sw_test_case_1()
+    elif case_no == 2:
+        mpsw.sw_test_case_2()
+    elif case_no == 5:
+        mpsw.sw_test_case_5()
+    elif case_no == 6:
+        mpsw.sw_test_case_6()
+    elif case_no == 7:
+        raise NotImplementedError('test case 7 (ERA5 initialisation) is not part of this build')
+    else:
+        exit('other cases not supported yet')
+
+
+def _write_output(path, lat, lon, late, lone, names, lists):
+    r2d = 180. / np.pi
+    with netcdf_file(path, 'w', version=2) as of:
+        of.createDimension('nCell', lat.shape[0])
+        of.createDimension('nTime', len(lists[0]))
+        of.createDimension('nEdge', late.shape[0])
+        of.createVariable('latCell', 'f', ('nCell',))[:] = (lat * r2d).astype(np.float32)
+        of.createVariable('lonCell', 'f', ('nCell',))[:] = (lon * r2d).astype(np.float32)
+        of.createVariable('latEdge', 'f', ('nEdge',))[:] = (late * r2d).astype(np.float32)
+        of.createVariable('lonEdge', 'f', ('nEdge',))[:] = (lone * r2d).astype(np.float32)
+        for (name, typ, dims), data in zip(names, lists):
+            of.createVariable(name, typ, dims)[:] = np.array(data)
+
+
+def run_sw_fwd(nmlname='namelist.sw.x1.10242'):
+    """src/mpas_sw_driver.py:265-308"""
+    nml = namelist(nmlname=nmlname)
+    read_configs(nml)
+    read_dims(nml)
+    read_vars(nml)
+    initial_conditions(nml)
+    mpsw.sw_mpas_init_block()
+    clock_start, clock_end = clock_namelist(nml)
+    today = clock_start
+    ulist, vlist, hlist, unorm = [], [], [], []
+    while today <= clock_end:
+        if (today - clock_start).total_seconds() % nml.output_interval == 0:
+            print(today, mpsw.u[0].min(), mpsw.u[0].max())
+            ulist.append(mpsw.ureconstructzonal[0].copy())
+            vlist.append(mpsw.ureconstructmeridional[0].copy())
+            hlist.append(mpsw.h[0, 0].copy())
+            unorm.append(mpsw.u[0, 0].copy())
+        mpsw.sw_rk4()
+        today += timedelta(seconds=int(mpsw.config_dt))
+    _write_output(nml.file_output, mpsw.latcell, mpsw.loncell, mpsw.latedge, mpsw.lonedge,
+                  [('ux', 'd', ('nTime', 'nCell')), ('uy', 'd', ('nTime', 'nCell')),
+                   ('h', 'd', ('nTime', 'nCell')), ('u', 'd', ('nTime', 'nEdge'))],
+                  [ulist, vlist, hlist, unorm])
+
+
+def run_sw_tlm(pt=1.E-3, nmlname='namelist.sw.x1.10242'):
+    """src/mpas_sw_driver.py:311-332"""
+    nml = namelist(nmlname=nmlname)
+    read_configs(nml)
+    read_dims(nml)
+    read_vars(nml)
+    initial_conditions(nml)
+    mpsw.var_allocation_tlm()
+    mpsw.sw_mpas_init_block()
+    mpsw.u_tl[0] = mpsw.u[0] * pt
+    mpsw.h_tl[0] = mpsw.h[0] * pt
+    clock_start, clock_end = clock_namelist(nml)
+    today = clock_start
+    while today <= clock_end:
+        mpsw.sw_rk4_tlm()
+        today += timedelta(seconds=int(mpsw.config_dt))
+
+
+def run_sw_adj(nmlname='namelist.sw.x1.10242', seed_index=0):
+    """The flow of demo/run-adj.py:16-80 with the nonlinear trajectory kept
+    in memory: a forward sweep stores the state at every step, the adjoint
+    sweep walks it backwards.  Returns the adjoint (u_ad, h_ad) at the start."""
+    nml = namelist(nmlname=nmlname)
+    read_configs(nml)
+    read_dims(nml)
+    read_vars(nml)
+    initial_conditions(nml)
+    mpsw.var_allocation_adj()
+    mpsw.sw_mpas_init_block()
+    clock_start, clock_end = clock_namelist(nml)
+    traj_u, traj_h = [], []
+    today = clock_start
+    while today <= clock_end:
+        traj_u.append(mpsw.u[0, 0].copy())
+        traj_h.append(mpsw.h[0, 0].copy())
+        mpsw.sw_rk4()
+        today += timedelta(seconds=int(mpsw.config_dt))
+    mpsw.u_ad[0, 0, seed_index] = 1.
+    mpsw.h_ad[0, 0, seed_index] = 1.
+    itimestep = len(traj_u) - 1
+    today = clock_end
+    while today >= clock_start:
+        mpsw.u[0, 0] = traj_u[itimestep]
+        mpsw.h[0, 0] = traj_h[itimestep]
+        mpsw.sw_rk4_adj()
+        itimestep -= 1
+        today -= timedelta(seconds=int(mpsw.config_dt))
+    return mpsw.u_ad[0, 0].copy(), mpsw.h_ad[0, 0].copy()
+
+
+def check_tlm(nmlname='namelist.sw.tlmcheck', idx1=0, idx2=500, npts=15):
+    """Linearisation test, src/mpas_sw_driver.py:335-377: prints, for
+    pt = 10**-ipt, the ratio (M(x(1+pt)) - M(x)) / (L x pt) at one cell and
+    one edge.  Returns the list of (ipt, ratio_h, ratio_u)."""
+    nml = namelist(nmlname=nmlname)
+    read_configs(nml)
+    read_dims(nml)
+    read_vars(nml)
+    initial_conditions(nml)
+    mpsw.var_allocation_tlm()
+    mpsw.sw_mpas_init_block()
+    clock_start, clock_end = clock_namelist(nml)
+    u_back, h_back = mpsw.u[0].copy(), mpsw.h[0].copy()
+    out = []
+    for ipt in range(npts):
+        pt = 10. ** (-ipt)
+        mpsw.u[0] = u_back * (1. + pt)
+        mpsw.h[0] = h_back * (1. + pt)
+        today = clock_start
+        while today <= clock_end:
+            mpsw.sw_rk4()
+            today += timedelta(seconds=int(mpsw.config_dt))
+        u1, h1 = mpsw.u[0, idx1, idx2].copy(), mpsw.h[0, idx1, idx2].copy()
+        mpsw.u[0] = u_back.copy()
+        mpsw.h[0] = h_back.copy()
+        mpsw.u_tl[0] = mpsw.u[0] * pt
+        mpsw.h_tl[0] = mpsw.h[0] * pt
+        today = clock_start
+        while today <= clock_end:
+            mpsw.sw_rk4_tlm()
+            today += timedelta(seconds=int(mpsw.config_dt))
+        u0, u_tl = mpsw.u[0, idx1, idx2].copy(), mpsw.u_tl[0, idx1, idx2].copy()
+        h0, h_tl = mpsw.h[0, idx1, idx2].copy(), mpsw.h_tl[0, idx1, idx2].copy()
+        print(ipt, (h1 - h0) / h_tl, (u1 - u0) / u_tl)
+        out.append((ipt, (h1 - h0) / h_tl, (u1 - u0) / u_tl))
+    return out
+
+
+if __name__ == '__main__':
+    check_tlm()
