@@ -53,13 +53,16 @@ def perturbations(mesh, nmem):
     return np.stack(hs, 1), np.stack(us, 1)
 
 
-def run_window(m, h0, u0, ht, ut, nsteps, timed):
-    """TLM sweep of nsteps (storing the trajectory) + adjoint sweep back."""
+def run_window(m, bufs, h0, u0, ht0, ut0, nsteps):
+    """TLM sweep of nsteps (storing the trajectory) + adjoint sweep back.
+    The same device buffers are reused on every call, so cached step graphs
+    stay valid."""
     import torch
-    h = h0.clone()
-    u = u0.clone()
-    th = torch.empty((nsteps, m.nC), dtype=torch.float64, device=m.device)
-    tu = torch.empty((nsteps, m.nE), dtype=torch.float64, device=m.device)
+    h, u, ht, ut, th, tu = bufs
+    h.copy_(h0)
+    u.copy_(u0)
+    ht.copy_(ht0)
+    ut.copy_(ut0)
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     t0 = time.perf_counter()
@@ -83,13 +86,16 @@ def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True):
     sel = slice(0, batch)
     ht0 = m.to_internal(dh[:, sel] if batch > 1 else dh[:, 0], "c")
     ut0 = m.to_internal(du[:, sel] if batch > 1 else du[:, 0], "e")
+    n = max(steps, warmup)
+    bufs = (torch.empty_like(h0), torch.empty_like(u0), torch.empty_like(ht0), torch.empty_like(ut0),
+            torch.empty((n, m.nC), dtype=torch.float64, device=m.device),
+            torch.empty((n, m.nE), dtype=torch.float64, device=m.device))
     if warmup > 0:
-        run_window(m, h0, u0, ht0.clone(), ut0.clone(), warmup, False)
-    ht, ut = ht0.clone(), ut0.clone()
-    wall, t_tlm, t_adj = run_window(m, h0, u0, ht, ut, steps, True)
-    flags = m.nonfinite(ht, ut)
+        run_window(m, bufs, h0, u0, ht0, ut0, warmup)
+    wall, t_tlm, t_adj = run_window(m, bufs, h0, u0, ht0, ut0, steps)
+    flags = m.nonfinite(bufs[2], bufs[3])
     ok = int(flags.sum().item()) == 0
-    del m
+    del m, bufs
     torch.cuda.empty_cache()
     T = t_tlm + t_adj
     return dict(seconds=T, wall=wall, t_tlm=t_tlm, t_adj=t_adj,
@@ -120,6 +126,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--level", type=int, default=LEVEL)
+    ap.add_argument("--no-graphs", action="store_true", help="launch stage kernels directly")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the batched members would shard "
@@ -136,13 +143,14 @@ def main():
     dh, du = perturbations(mesh, BATCH)
     log(f"setup L{args.level}: {time.time() - t:.1f}s; C={mesh.nCells} E={mesh.nEdges} V={mesh.nVertices}")
 
-    res = gpu_rate(mesh, phys, dh, du, BATCH, "fp64", args.steps, args.warmup)
+    g = not args.no_graphs
+    res = gpu_rate(mesh, phys, dh, du, BATCH, "fp64", args.steps, args.warmup, g)
     log(f"batched fp64: {res}")
     plain = None
     if not args.no_plain:
-        plain = gpu_rate(mesh, phys, dh, du, 1, "fp64", args.steps, args.warmup)
+        plain = gpu_rate(mesh, phys, dh, du, 1, "fp64", args.steps, args.warmup, g)
         log(f"plain fp64: {plain}")
-    f32 = gpu_rate(mesh, phys, dh, du, BATCH, "fp32", args.steps, args.warmup)
+    f32 = gpu_rate(mesh, phys, dh, du, BATCH, "fp32", args.steps, args.warmup, g)
     log(f"batched fp32: {f32}")
 
     K = args.steps

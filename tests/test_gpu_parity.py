@@ -156,3 +156,54 @@ def test_dot_product_identity_fp64():
     lhs = m.dot(Lxh, Lxu, yh_d, yu_d).cpu().numpy()
     rhs = m.dot(xh_d, xu_d, LTyh, LTyu).cpu().numpy()
     assert np.all(np.abs(lhs - rhs) <= 1e-12 * np.abs(lhs))
+
+
+def test_graph_replays_match_eager():
+    """Captured per-step hipGraphs give the eager results, replay after replay."""
+    print("graphs vs eager L3 tc6 B=4", flush=True)
+    tc, dt = 6, 600.0
+    mesh, phys = case(LEVEL, tc)
+    B, n = 4, 5
+    dh, du = perturbations(mesh, B)
+    res = []
+    for graphs in (False, True):
+        m = model(LEVEL, tc, dt, batch=B, graphs=graphs)
+        h = m.to_internal(phys["h"], "c", member=False)
+        u = m.to_internal(phys["u"], "e", member=False)
+        ht, ut = m.to_internal(dh, "c"), m.to_internal(du, "e")
+        th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+        tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+        outs = []
+        for rep in range(2):                  # the second pass replays cached graphs
+            m.tlm(h, u, ht, ut, n, th, tu)
+            torch.cuda.synchronize()
+            outs.append(ht.clone())
+            m.adj(th, tu, ht, ut, n)
+            torch.cuda.synchronize()
+            outs.append(ut.clone())
+        res.append(outs)
+    for a, b in zip(res[0], res[1]):
+        assert torch.equal(a, b)
+
+
+def test_fp32_members_track_fp64():
+    """FP32 TL/adjoint members on the FP64 base trajectory stay close to FP64."""
+    print("fp32 members L3 tc5 B=4", flush=True)
+    tc, dt = 5, 900.0
+    mesh, phys = case(LEVEL, tc)
+    B, n = 4, 8
+    dh, du = perturbations(mesh, B)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        m = model(LEVEL, tc, dt, batch=B, precision=prec)
+        h = m.to_internal(phys["h"], "c", member=False)
+        u = m.to_internal(phys["u"], "e", member=False)
+        ht, ut = m.to_internal(dh, "c"), m.to_internal(du, "e")
+        th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+        tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+        m.tlm(h, u, ht, ut, n, th, tu)
+        tl = m.to_reference(ht, "c")
+        m.adj(th, tu, ht, ut, n)
+        out[prec] = (tl, m.to_reference(ht, "c"), m.to_reference(ut, "e"))
+    for a, b in zip(out["fp32"], out["fp64"]):
+        assert rel_l2(a, b) < 1e-5
