@@ -17,7 +17,9 @@ constexpr double kGravity = 9.80616;   // src/mpas_sw_dyncore.f90:414
 // Device-resident mesh tables, entities in the library's internal order
 // (space-filling curve), 0-based.  "R" tables list neighbours in the order
 // the reference's scatter loops visit them (ascending reference index), so a
-// gather over them adds the terms in the reference's order.
+// gather over them adds the terms in the reference's order.  Lists are padded
+// to their fixed width with a valid index (the entity's own first neighbour,
+// or the edge itself) and a zero sign/weight/kite, so padded terms are 0.
 struct DMesh {
     int nC, nE, nV;
     // cells
@@ -99,7 +101,7 @@ void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, con
 template <typename T>
 void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
               const BaseStage& b, const T* h_tl, const T* u_tl,
-              T* pvv_tl, T* ke_tl, T* pc_tl, T* pe_tl,
+              T* pvv_tl, T* ke_tl, T* pc_tl, T* pe_tl, T* he_tl,
               T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s);
 
 // Adjoint of one RK4 stage: pa = F'(p_s)^T g, then the epilogue

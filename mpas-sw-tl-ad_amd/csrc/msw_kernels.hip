@@ -7,6 +7,16 @@
 // loads, one SGPR) and each neighbour gather is one contiguous B*sizeof(T)
 // row.  For B = 1 (the plain path) each lane is one entity.
 //
+// Neighbour lists are padded to fixed widths (6 per cell, 12 per edge, 3
+// per vertex) with zero weights, and every gather loop is fully unrolled:
+// all index loads and then all row loads of an entity are issued before any
+// arithmetic, so a wavefront keeps many loads in flight instead of walking a
+// chain of dependent loads.  Padded terms add exact zeros.
+//
+// Blocks are mapped XCD-aware: the hardware deals block i to XCD i % 8, and
+// xcd_block() turns that into one contiguous range of the space-filling-curve
+// order per XCD, so the neighbours a gather touches sit in the same L2.
+//
 // All gathers visit neighbours in the order the reference's scatter loops
 // add them (the "R" tables of DMesh) and every expression keeps the
 // reference's association, so with FP contraction off the FP64 forward and
@@ -19,6 +29,8 @@ namespace msw {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kE2 = kMaxEdges2;
+constexpr int kC6 = kMaxEdges;
 
 template <bool WU>
 __device__ __forceinline__ int entity_of(long t, int lb)
@@ -28,14 +40,46 @@ __device__ __forceinline__ int entity_of(long t, int lb)
     return e;
 }
 
+// logical block index: XCD x (= blockIdx % 8) owns the contiguous range
+// [x * gridDim/8, (x+1) * gridDim/8); gridDim is a multiple of 8
+__device__ __forceinline__ long xcd_block()
+{
+    const long per = static_cast<long>(gridDim.x >> 3);
+    return static_cast<long>(blockIdx.x & 7u) * per + static_cast<long>(blockIdx.x >> 3);
+}
+
 inline int grid_for(long threads)
 {
     long blocks = (threads + kBlock - 1) / kBlock;
-    const long cap = 1L << 20;   // grid-stride beyond ~268M threads
+    const long cap = 1L << 20;   // block-stride beyond ~268M threads
     if (blocks > cap) blocks = cap;
-    if (blocks < 1) blocks = 1;
+    blocks = (blocks + 7) & ~7L;
+    if (blocks < 8) blocks = 8;
     return static_cast<int>(blocks);
 }
+
+inline long nblocks(long threads) { return (threads + kBlock - 1) / kBlock; }
+
+// one entity range
+#define ENTITY_LOOP(count)                                                               \
+    const long total_ = static_cast<long>(count) << lb;                                 \
+    const long nb_ = (total_ + kBlock - 1) / kBlock;                                     \
+    for (long L_ = xcd_block(); L_ < nb_; L_ += gridDim.x)                               \
+        for (long t = L_ * kBlock + threadIdx.x; t < total_; t = total_)
+
+// edges and cells interleaved in proportion along the curve, so one XCD
+// updates the edges and the cells of the same region
+#define EDGE_CELL_LOOP                                                                  \
+    const long tE_ = static_cast<long>(m.nE) << lb, tC_ = static_cast<long>(m.nC) << lb; \
+    const long nbE_ = (tE_ + kBlock - 1) / kBlock, nbC_ = (tC_ + kBlock - 1) / kBlock;   \
+    const long N_ = nbE_ + nbC_;                                                         \
+    for (long L_ = xcd_block(); L_ < N_; L_ += gridDim.x)
+
+#define EDGE_CELL_SPLIT(is_edge, t)                                                     \
+    const long e0_ = (L_ * nbE_) / N_, e1_ = ((L_ + 1) * nbE_) / N_;                    \
+    const bool is_edge = e1_ > e0_;                                                      \
+    const long t = (is_edge ? e0_ : L_ - e0_) * kBlock + threadIdx.x;                   \
+    if (t >= (is_edge ? tE_ : tC_)) continue;
 
 #define GRID_LOOP(total)                                                               \
     for (long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x; t < (total); \
@@ -57,22 +101,24 @@ __global__ __launch_bounds__(kBlock) void k_fwd_vertex(DMesh m, int lb, const T*
                                                        T* __restrict__ hv, T* __restrict__ vort)
 {
     const int B = 1 << lb;
-    GRID_LOOP(static_cast<long>(m.nV) << lb)
+    ENTITY_LOOP(m.nV)
     {
         const int iv = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
+        int ie[3], ic[3];
+        T ue[3], hc[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { ie[i] = m.eovR[3 * iv + i]; ic[i] = m.cov[3 * iv + i]; }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { ue[i] = ld(u, ie[i], B, b); hc[i] = ld(h, ic[i], B, b); }
         T circ = T(0);
 #pragma unroll
-        for (int i = 0; i < kVertexDegree; ++i) {
-            const int e = m.eovR[3 * iv + i];
-            circ = circ + T(m.sgnV[3 * iv + i]) * (T(m.dc[e]) * ld(u, e, B, b));
-        }
+        for (int i = 0; i < 3; ++i) circ = circ + T(m.sgnV[3 * iv + i]) * (T(m.dc[ie[i]]) * ue[i]);
         const T at = T(m.areaT[iv]);
         const T vo = circ / at;
         T hh = T(0);
 #pragma unroll
-        for (int i = 0; i < kVertexDegree; ++i)
-            hh = hh + ld(h, m.cov[3 * iv + i], B, b) * T(m.kite[3 * iv + i]);
+        for (int i = 0; i < 3; ++i) hh = hh + hc[i] * T(m.kite[3 * iv + i]);
         hh = hh / at;
         const long o = static_cast<long>(iv) * B + b;
         pvv[o] = (T(m.fV[iv]) + vo) / hh;
@@ -88,24 +134,27 @@ __global__ __launch_bounds__(kBlock) void k_fwd_cell(DMesh m, int lb, const T* _
                                                      T* __restrict__ pc)
 {
     const int B = 1 << lb;
-    GRID_LOOP(static_cast<long>(m.nC) << lb)
+    ENTITY_LOOP(m.nC)
     {
         const int ic = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
-        const int n = m.nEoC[ic];
+        int ie[kC6], iv[kC6];
+        T ue[kC6], pv[kC6];
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) { ie[i] = m.eoc[kC6 * ic + i]; iv[i] = m.vocR[kC6 * ic + i]; }
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) { ue[i] = ld(u, ie[i], B, b); pv[i] = ld(pvv, iv[i], B, b); }
         const T area = T(m.areaC[ic]);
         T k = T(0);
-        for (int i = 0; i < n; ++i) {
-            const int e = m.eoc[kMaxEdges * ic + i];
-            const T uu = ld(u, e, B, b);
-            k = k + T(0.25) * T(m.dc[e]) * T(m.dv[e]) * (uu * uu);
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) {
+            const T mask = T(fabs(m.sgnC[kC6 * ic + i]));   // 1, or 0 on a padded slot
+            k = k + mask * (T(0.25) * T(m.dc[ie[i]]) * T(m.dv[ie[i]]) * (ue[i] * ue[i]));
         }
         k = k / area;
         T p = T(0);
-        for (int i = 0; i < n; ++i) {
-            const int iv = m.vocR[kMaxEdges * ic + i];
-            p = p + T(m.kiteCR[kMaxEdges * ic + i]) * ld(pvv, iv, B, b) / area;
-        }
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
         const long o = static_cast<long>(ic) * B + b;
         ke[o] = k;
         pc[o] = p;
@@ -126,25 +175,32 @@ __global__ __launch_bounds__(kBlock) void k_fwd_edge(DMesh m, DConfig c, int lb,
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    GRID_LOOP(static_cast<long>(m.nE) << lb)
+    ENTITY_LOOP(m.nE)
     {
         const int e = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
         const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
         const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
-        const T hh = T(0.5) * (ld(h, c1, B, b) + ld(h, c2, B, b));
-        const int n = m.nEoE[e];
-        T vv = T(0);
-        for (int j = 0; j < n; ++j)
-            vv = vv + T(m.woe[kMaxEdges2 * e + j]) * ld(u, m.eoe[kMaxEdges2 * e + j], B, b);
+        int ix[kE2];
+        T ux[kE2];
+#pragma unroll
+        for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+#pragma unroll
+        for (int j = 0; j < kE2; ++j) ux[j] = ld(u, ix[j], B, b);
+        const T h1 = ld(h, c1, B, b), h2 = ld(h, c2, B, b);
         const T p1 = ld(pvv, v1, B, b), p2 = ld(pvv, v2, B, b);
+        const T q1 = ld(pc, c1, B, b), q2 = ld(pc, c2, B, b);
+        const T ue = ld(u, e, B, b);
+        T vv = T(0);
+#pragma unroll
+        for (int j = 0; j < kE2; ++j) vv = vv + T(m.woe[kE2 * e + j]) * ux[j];
         const T gt = (p2 - p1) / T(m.dv[e]);
-        const T gn = (ld(pc, c2, B, b) - ld(pc, c1, B, b)) / T(m.dc[e]);
+        const T gn = (q2 - q1) / T(m.dc[e]);
         T p = T(0.5) * p1 + T(0.5) * p2;
         p = p - apvm * vv * dt * gt;
-        p = p - apvm * ld(u, e, B, b) * dt * gn;
+        p = p - apvm * ue * dt * gn;
         const long o = static_cast<long>(e) * B + b;
-        he[o] = hh;
+        he[o] = T(0.5) * (h1 + h2);
         pe[o] = p;
         if (vout) vout[o] = vv;
         if (gpt_out) gpt_out[o] = gt;
@@ -157,11 +213,12 @@ template <typename T>
 __device__ __forceinline__ void rk_update(int stage, T w, T sw, T tend, T* x, T* acc, T* pnext,
                                           long o, bool frozen)
 {
-    const T xv = x[o];
     if (stage == 0) {
+        const T xv = x[o];
         acc[o] = xv + w * tend;
         pnext[o] = frozen ? xv : xv + sw * tend;
     } else if (stage < 3) {
+        const T xv = x[o];
         acc[o] = acc[o] + w * tend;
         pnext[o] = frozen ? xv : xv + sw * tend;
     } else {
@@ -180,35 +237,49 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
                                                      T* acch, T* accu, T* ph, T* pu)
 {
     const int B = 1 << lb;
-    const long nEt = static_cast<long>(m.nE) << lb;
-    GRID_LOOP(nEt + (static_cast<long>(m.nC) << lb))
+    EDGE_CELL_LOOP
     {
+        EDGE_CELL_SPLIT(is_edge, t)
         const int b = static_cast<int>(t & (B - 1));
-        if (t < nEt) {
+        if (is_edge) {
             const int e = entity_of<WU>(t, lb);
             const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
-            const T pee = ld(pe, e, B, b);
-            const int n = m.nEoE[e];
-            T q = T(0);
-            for (int j = 0; j < n; ++j) {
-                const int eo = m.eoe[kMaxEdges2 * e + j];
-                const T workpv = T(0.5) * (pee + ld(pe, eo, B, b));
-                q = q + T(m.woe[kMaxEdges2 * e + j]) * ld(u, eo, B, b) * workpv * ld(he, eo, B, b);
+            int ix[kE2];
+            T ux[kE2], px[kE2], hx[kE2];
+#pragma unroll
+            for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+#pragma unroll
+            for (int j = 0; j < kE2; ++j) {
+                ux[j] = ld(u, ix[j], B, b);
+                px[j] = ld(pe, ix[j], B, b);
+                hx[j] = ld(he, ix[j], B, b);
             }
-            const T tend = q - (ld(ke, c2, B, b) - ld(ke, c1, B, b) +
-                                T(kGravity) * (ld(h, c2, B, b) + T(m.hs[c2]) - ld(h, c1, B, b) -
-                                               T(m.hs[c1]))) /
+            const T pee = ld(pe, e, B, b);
+            const T k1 = ld(ke, c1, B, b), k2 = ld(ke, c2, B, b);
+            const T h1 = ld(h, c1, B, b), h2 = ld(h, c2, B, b);
+            T q = T(0);
+#pragma unroll
+            for (int j = 0; j < kE2; ++j) {
+                const T workpv = T(0.5) * (pee + px[j]);
+                q = q + T(m.woe[kE2 * e + j]) * ux[j] * workpv * hx[j];
+            }
+            const T tend = q - (k2 - k1 + T(kGravity) * (h2 + T(m.hs[c2]) - h1 - T(m.hs[c1]))) /
                                    T(m.dc[e]);
             rk_update(stage, w, sw, tend, xu, accu, pu, static_cast<long>(e) * B + b,
                       c.frozen_u != 0);
         } else {
-            const int ic = entity_of<WU>(t - nEt, lb);
-            const int n = m.nEoC[ic];
+            const int ic = entity_of<WU>(t, lb);
+            int ie[kC6];
+            T ue[kC6], hh[kC6];
+#pragma unroll
+            for (int k = 0; k < kC6; ++k) ie[k] = m.eocR[kC6 * ic + k];
+#pragma unroll
+            for (int k = 0; k < kC6; ++k) { ue[k] = ld(u, ie[k], B, b); hh[k] = ld(he, ie[k], B, b); }
             T th = T(0);
-            for (int k = 0; k < n; ++k) {
-                const int e = m.eocR[kMaxEdges * ic + k];
-                const T flux = ld(u, e, B, b) * T(m.dv[e]) * ld(he, e, B, b);
-                th = th + T(-m.sgnCR[kMaxEdges * ic + k]) * flux;
+#pragma unroll
+            for (int k = 0; k < kC6; ++k) {
+                const T flux = ue[k] * T(m.dv[ie[k]]) * hh[k];
+                th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
             }
             th = th / T(m.areaC[ic]);
             rk_update(stage, w, sw, th, xh, acch, ph, static_cast<long>(ic) * B + b, false);
@@ -229,22 +300,24 @@ __global__ __launch_bounds__(kBlock) void k_tl_vertex(DMesh m, int lb, const dou
                                                       T* __restrict__ pvv_tl)
 {
     const int B = 1 << lb;
-    GRID_LOOP(static_cast<long>(m.nV) << lb)
+    ENTITY_LOOP(m.nV)
     {
         const int iv = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
+        int ie[3], ic[3];
+        T ue[3], hc[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { ie[i] = m.eovR[3 * iv + i]; ic[i] = m.cov[3 * iv + i]; }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { ue[i] = ld(u_tl, ie[i], B, b); hc[i] = ld(h_tl, ic[i], B, b); }
         T ct = T(0);
 #pragma unroll
-        for (int i = 0; i < kVertexDegree; ++i) {
-            const int e = m.eovR[3 * iv + i];
-            ct = ct + T(m.sgnV[3 * iv + i]) * (T(m.dc[e]) * ld(u_tl, e, B, b));
-        }
+        for (int i = 0; i < 3; ++i) ct = ct + T(m.sgnV[3 * iv + i]) * (T(m.dc[ie[i]]) * ue[i]);
         const T at = T(m.areaT[iv]);
         const T vt = ct / at;
         T ht = T(0);
 #pragma unroll
-        for (int i = 0; i < kVertexDegree; ++i)
-            ht = ht + ld(h_tl, m.cov[3 * iv + i], B, b) * T(m.kite[3 * iv + i]);
+        for (int i = 0; i < 3; ++i) ht = ht + hc[i] * T(m.kite[3 * iv + i]);
         ht = ht / at;
         const T hv = T(hvb[iv]);
         pvv_tl[static_cast<long>(iv) * B + b] =
@@ -260,56 +333,71 @@ __global__ __launch_bounds__(kBlock) void k_tl_cell(DMesh m, int lb, const doubl
                                                     T* __restrict__ ke_tl, T* __restrict__ pc_tl)
 {
     const int B = 1 << lb;
-    GRID_LOOP(static_cast<long>(m.nC) << lb)
+    ENTITY_LOOP(m.nC)
     {
         const int ic = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
-        const int n = m.nEoC[ic];
+        int ie[kC6], iv[kC6];
+        T ue[kC6], pv[kC6];
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) { ie[i] = m.eoc[kC6 * ic + i]; iv[i] = m.vocR[kC6 * ic + i]; }
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) { ue[i] = ld(u_tl, ie[i], B, b); pv[i] = ld(pvv_tl, iv[i], B, b); }
         const T area = T(m.areaC[ic]);
         T k = T(0);
-        for (int i = 0; i < n; ++i) {
-            const int e = m.eoc[kMaxEdges * ic + i];
-            k = k + T(0.25) * T(m.dc[e]) * T(m.dv[e]) * (T(2.0) * T(ub[e]) * ld(u_tl, e, B, b));
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) {
+            const T mask = T(fabs(m.sgnC[kC6 * ic + i]));
+            k = k + mask * (T(0.25) * T(m.dc[ie[i]]) * T(m.dv[ie[i]]) * (T(2.0) * T(ub[ie[i]]) * ue[i]));
         }
         k = k / area;
         T p = T(0);
-        for (int i = 0; i < n; ++i) {
-            const int iv = m.vocR[kMaxEdges * ic + i];
-            p = p + T(m.kiteCR[kMaxEdges * ic + i]) * ld(pvv_tl, iv, B, b) / area;
-        }
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
         const long o = static_cast<long>(ic) * B + b;
         ke_tl[o] = k;
         pc_tl[o] = p;
     }
 }
 
-// :319-329, :364-393, :425-445
+// :117-126 (h_edge_tl), :319-329, :364-393, :425-445
 template <typename T, bool WU>
 __global__ __launch_bounds__(kBlock) void k_tl_edge(DMesh m, DConfig c, int lb, BaseStage bs,
+                                                    const T* __restrict__ h_tl,
                                                     const T* __restrict__ u_tl,
                                                     const T* __restrict__ pvv_tl,
                                                     const T* __restrict__ pc_tl,
-                                                    T* __restrict__ pe_tl)
+                                                    T* __restrict__ pe_tl, T* __restrict__ he_tl)
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    GRID_LOOP(static_cast<long>(m.nE) << lb)
+    ENTITY_LOOP(m.nE)
     {
         const int e = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
         const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
         const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
-        const int n = m.nEoE[e];
-        T vt = T(0);
-        for (int j = 0; j < n; ++j)
-            vt = vt + T(m.woe[kMaxEdges2 * e + j]) * ld(u_tl, m.eoe[kMaxEdges2 * e + j], B, b);
+        int ix[kE2];
+        T ux[kE2];
+#pragma unroll
+        for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+#pragma unroll
+        for (int j = 0; j < kE2; ++j) ux[j] = ld(u_tl, ix[j], B, b);
         const T p1 = ld(pvv_tl, v1, B, b), p2 = ld(pvv_tl, v2, B, b);
+        const T q1 = ld(pc_tl, c1, B, b), q2 = ld(pc_tl, c2, B, b);
+        const T h1 = ld(h_tl, c1, B, b), h2 = ld(h_tl, c2, B, b);
+        const T ue = ld(u_tl, e, B, b);
+        T vt = T(0);
+#pragma unroll
+        for (int j = 0; j < kE2; ++j) vt = vt + T(m.woe[kE2 * e + j]) * ux[j];
         const T gtt = (p2 - p1) / T(m.dv[e]);
-        const T gnt = (ld(pc_tl, c2, B, b) - ld(pc_tl, c1, B, b)) / T(m.dc[e]);
+        const T gnt = (q2 - q1) / T(m.dc[e]);
         T p = T(0.5) * p1 + T(0.5) * p2;
         p = p - apvm * vt * dt * T(bs.gpt[e]) - apvm * T(bs.v[e]) * dt * gtt;
-        p = p - apvm * ld(u_tl, e, B, b) * dt * T(bs.gpn[e]) - apvm * T(bs.u[e]) * dt * gnt;
-        pe_tl[static_cast<long>(e) * B + b] = p;
+        p = p - apvm * ue * dt * T(bs.gpn[e]) - apvm * T(bs.u[e]) * dt * gnt;
+        const long o = static_cast<long>(e) * B + b;
+        pe_tl[o] = p;
+        he_tl[o] = T(0.5) * (h1 + h2);
     }
 }
 
@@ -320,49 +408,60 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
                                                     const T* __restrict__ h_tl,
                                                     const T* __restrict__ u_tl,
                                                     const T* __restrict__ pe_tl,
+                                                    const T* __restrict__ he_tl,
                                                     const T* __restrict__ ke_tl, T* xh, T* xu,
                                                     T* acch, T* accu, T* ph, T* pu)
 {
     const int B = 1 << lb;
-    const long nEt = static_cast<long>(m.nE) << lb;
-    GRID_LOOP(nEt + (static_cast<long>(m.nC) << lb))
+    EDGE_CELL_LOOP
     {
+        EDGE_CELL_SPLIT(is_edge, t)
         const int b = static_cast<int>(t & (B - 1));
-        if (t < nEt) {
+        if (is_edge) {
             const int e = entity_of<WU>(t, lb);
             const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+            int ix[kE2];
+            T ux[kE2], px[kE2], hx[kE2];
+#pragma unroll
+            for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+#pragma unroll
+            for (int j = 0; j < kE2; ++j) {
+                ux[j] = ld(u_tl, ix[j], B, b);
+                px[j] = ld(pe_tl, ix[j], B, b);
+                hx[j] = ld(he_tl, ix[j], B, b);
+            }
             const T pte = ld(pe_tl, e, B, b);
+            const T k1 = ld(ke_tl, c1, B, b), k2 = ld(ke_tl, c2, B, b);
+            const T h1 = ld(h_tl, c1, B, b), h2 = ld(h_tl, c2, B, b);
             const T pbe = T(bs.pe[e]);
-            const int n = m.nEoE[e];
             T qt = T(0);
-            for (int j = 0; j < n; ++j) {
-                const int eo = m.eoe[kMaxEdges2 * e + j];
-                const T wgt = T(m.woe[kMaxEdges2 * e + j]);
-                const T wpvt = T(0.5) * (pte + ld(pe_tl, eo, B, b));
+#pragma unroll
+            for (int j = 0; j < kE2; ++j) {
+                const int eo = ix[j];
+                const T wgt = T(m.woe[kE2 * e + j]);
+                const T wpvt = T(0.5) * (pte + px[j]);
                 const T wpv = T(0.5) * (pbe + T(bs.pe[eo]));
                 const T heb = T(bs.he[eo]);
-                const T het = T(0.5) * (ld(h_tl, m.coe[2 * eo], B, b) +
-                                        ld(h_tl, m.coe[2 * eo + 1], B, b));
                 const T ubo = T(bs.u[eo]);
-                qt = qt + wgt * ld(u_tl, eo, B, b) * wpv * heb + wgt * ubo * wpvt * heb +
-                     wgt * ubo * wpv * het;
+                qt = qt + wgt * ux[j] * wpv * heb + wgt * ubo * wpvt * heb + wgt * ubo * wpv * hx[j];
             }
-            const T tend = qt - (ld(ke_tl, c2, B, b) - ld(ke_tl, c1, B, b) +
-                                 T(kGravity) * (ld(h_tl, c2, B, b) - ld(h_tl, c1, B, b))) /
-                                    T(m.dc[e]);
+            const T tend = qt - (k2 - k1 + T(kGravity) * (h2 - h1)) / T(m.dc[e]);
             rk_update(stage, w, sw, tend, xu, accu, pu, static_cast<long>(e) * B + b,
                       c.frozen_u != 0);
         } else {
-            const int ic = entity_of<WU>(t - nEt, lb);
-            const int n = m.nEoC[ic];
+            const int ic = entity_of<WU>(t, lb);
+            int ie[kC6];
+            T ue[kC6], hh[kC6];
+#pragma unroll
+            for (int k = 0; k < kC6; ++k) ie[k] = m.eocR[kC6 * ic + k];
+#pragma unroll
+            for (int k = 0; k < kC6; ++k) { ue[k] = ld(u_tl, ie[k], B, b); hh[k] = ld(he_tl, ie[k], B, b); }
             T th = T(0);
-            for (int k = 0; k < n; ++k) {
-                const int e = m.eocR[kMaxEdges * ic + k];
-                const T het = T(0.5) * (ld(h_tl, m.coe[2 * e], B, b) +
-                                        ld(h_tl, m.coe[2 * e + 1], B, b));
-                const T flux = ld(u_tl, e, B, b) * T(m.dv[e]) * T(bs.he[e]) +
-                               T(bs.u[e]) * T(m.dv[e]) * het;
-                th = th + T(-m.sgnCR[kMaxEdges * ic + k]) * flux;
+#pragma unroll
+            for (int k = 0; k < kC6; ++k) {
+                const int e = ie[k];
+                const T flux = ue[k] * T(m.dv[e]) * T(bs.he[e]) + T(bs.u[e]) * T(m.dv[e]) * hh[k];
+                th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
             }
             th = th / T(m.areaC[ic]);
             rk_update(stage, w, sw, th, xh, acch, ph, static_cast<long>(ic) * B + b, false);
@@ -388,44 +487,50 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, DConfig c, int lb
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    GRID_LOOP(static_cast<long>(m.nE) << lb)
+    ENTITY_LOOP(m.nE)
     {
         const int e = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
         const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+        int ix[kE2], i1[kC6], i2[kC6];
+        T gx[kE2], g1[kC6], g2[kC6];
+#pragma unroll
+        for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) { i1[i] = m.eoc[kC6 * c1 + i]; i2[i] = m.eoc[kC6 * c2 + i]; }
+#pragma unroll
+        for (int j = 0; j < kE2; ++j) gx[j] = ld(gu, ix[j], B, b);
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) { g1[i] = ld(gu, i1[i], B, b); g2[i] = ld(gu, i2[i], B, b); }
+        const T gh1 = ld(gh, c1, B, b), gh2 = ld(gh, c2, B, b);
+        const T ge = ld(gu, e, B, b);
         const T a1 = T(m.areaC[c1]), a2 = T(m.areaC[c2]);
         const T dv = T(m.dv[e]), dc = T(m.dc[e]);
         const T ub = T(bs.u[e]), heb = T(bs.he[e]), peb = T(bs.pe[e]);
-        const T flux_ad = ld(gh, c2, B, b) / a2 - ld(gh, c1, B, b) / a1;
-        const T ge = ld(gu, e, B, b);
+        const T flux_ad = gh2 / a2 - gh1 / a1;
         T pad = T(0);
         T had = ub * dv * flux_ad;
         T uad = dv * heb * flux_ad;
-        const int n = m.nEoE[e];
-        for (int j = 0; j < n; ++j) {
-            const int eo = m.eoe[kMaxEdges2 * e + j];
-            const int jt = m.eoeT[kMaxEdges2 * e + j];
-            const T wgt = T(m.woe[kMaxEdges2 * e + j]);
-            const T wt = T(m.woe[kMaxEdges2 * eo + jt]);
+#pragma unroll
+        for (int j = 0; j < kE2; ++j) {
+            const int eo = ix[j];
+            const int jt = m.eoeT[kE2 * e + j];
+            const T wgt = T(m.woe[kE2 * e + j]);
+            const T wt = T(m.woe[kE2 * eo + jt]);
             const T wpv = T(0.5) * (peb + T(bs.pe[eo]));
             // this edge's own q: pv_edge_ad(e) += 0.5*w*u(eo)*he(eo)*q_ad(e)
             pad = pad + T(0.5) * (wgt * T(bs.u[eo]) * T(bs.he[eo]) * ge);
             // edge eo's q, in which this edge is neighbour jt
-            const T go = ld(gu, eo, B, b);
-            had = had + wt * ub * wpv * go;
-            uad = uad + wt * wpv * heb * go;
-            pad = pad + T(0.5) * (wt * ub * heb * go);
+            had = had + wt * ub * wpv * gx[j];
+            uad = uad + wt * wpv * heb * gx[j];
+            pad = pad + T(0.5) * (wt * ub * heb * gx[j]);
         }
         // ke_ad of both cells from tend_u's kinetic-energy gradient
         T k1 = T(0), k2 = T(0);
-        const int n1 = m.nEoC[c1], n2 = m.nEoC[c2];
-        for (int i = 0; i < n1; ++i) {
-            const int ee = m.eoc[kMaxEdges * c1 + i];
-            k1 = k1 + T(m.sgnC[kMaxEdges * c1 + i]) * ld(gu, ee, B, b) / T(m.dc[ee]);
-        }
-        for (int i = 0; i < n2; ++i) {
-            const int ee = m.eoc[kMaxEdges * c2 + i];
-            k2 = k2 + T(m.sgnC[kMaxEdges * c2 + i]) * ld(gu, ee, B, b) / T(m.dc[ee]);
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) {
+            k1 = k1 + T(m.sgnC[kC6 * c1 + i]) * g1[i] / T(m.dc[i1[i]]);
+            k2 = k2 + T(m.sgnC[kC6 * c2 + i]) * g2[i] / T(m.dc[i2[i]]);
         }
         const T kc = T(0.25) * dc * dv * T(2.0) * ub;
         uad = uad + kc * (k1 / a1) + kc * (k2 / a2);
@@ -449,19 +554,30 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    GRID_LOOP(static_cast<long>(m.nC) << lb)
+    ENTITY_LOOP(m.nC)
     {
         const int ic = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
-        const int n = m.nEoC[ic];
+        int ie[kC6];
+        T ge[kC6], pa[kC6], ha[kC6];
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) ie[i] = m.eoc[kC6 * ic + i];
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) {
+            ge[i] = ld(gu, ie[i], B, b);
+            pa[i] = ld(pe_ad, ie[i], B, b);
+            ha[i] = ld(he_ad, ie[i], B, b);
+        }
         T pcad = T(0), had = T(0);
-        for (int i = 0; i < n; ++i) {
-            const int e = m.eoc[kMaxEdges * ic + i];
-            const T s = T(m.sgnC[kMaxEdges * ic + i]);
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) {
+            const int e = ie[i];
+            const T s = T(m.sgnC[kC6 * ic + i]);
+            const T mask = T(fabs(m.sgnC[kC6 * ic + i]));
             const T dc = T(m.dc[e]);
-            const T gpn_ad = -apvm * T(bs.u[e]) * dt * ld(pe_ad, e, B, b);
+            const T gpn_ad = -apvm * T(bs.u[e]) * dt * pa[i];
             pcad = pcad - s * gpn_ad / dc;
-            had = had + s * (T(kGravity) / dc * ld(gu, e, B, b)) + T(0.5) * ld(he_ad, e, B, b);
+            had = had + s * (T(kGravity) / dc * ge[i]) + mask * (T(0.5) * ha[i]);
         }
         const long o = static_cast<long>(ic) * B + b;
         pc_ad[o] = pcad;
@@ -481,22 +597,24 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    GRID_LOOP(static_cast<long>(m.nV) << lb)
+    ENTITY_LOOP(m.nV)
     {
         const int iv = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
+        int ic[3], ie[3];
+        T pc[3], pe[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { ic[i] = m.cov[3 * iv + i]; ie[i] = m.eovR[3 * iv + i]; }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { pc[i] = ld(pc_ad, ic[i], B, b); pe[i] = ld(pe_ad, ie[i], B, b); }
         T pa = T(0);
 #pragma unroll
-        for (int i = 0; i < kVertexDegree; ++i) {
-            const int ic = m.cov[3 * iv + i];
-            pa = pa + T(m.kite[3 * iv + i]) / T(m.areaC[ic]) * ld(pc_ad, ic, B, b);
-        }
+        for (int i = 0; i < 3; ++i) pa = pa + T(m.kite[3 * iv + i]) / T(m.areaC[ic[i]]) * pc[i];
 #pragma unroll
-        for (int i = 0; i < kVertexDegree; ++i) {
-            const int e = m.eovR[3 * iv + i];
-            const T pad = ld(pe_ad, e, B, b);
-            const T gpt_ad = -apvm * T(bs.v[e]) * dt * pad;
-            pa = pa + T(0.5) * pad + T(m.sgnV[3 * iv + i]) * gpt_ad / T(m.dv[e]);
+        for (int i = 0; i < 3; ++i) {
+            const int e = ie[i];
+            const T gpt_ad = -apvm * T(bs.v[e]) * dt * pe[i];
+            pa = pa + T(0.5) * pe[i] + T(m.sgnV[3 * iv + i]) * gpt_ad / T(m.dv[e]);
         }
         const T hv = T(bs.hv[iv]);
         const T at = T(m.areaT[iv]);
@@ -537,29 +655,40 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    const long nEt = static_cast<long>(m.nE) << lb;
-    GRID_LOOP(nEt + (static_cast<long>(m.nC) << lb))
+    EDGE_CELL_LOOP
     {
+        EDGE_CELL_SPLIT(is_edge, t)
         const int b = static_cast<int>(t & (B - 1));
-        if (t < nEt) {
+        if (is_edge) {
             const int e = entity_of<WU>(t, lb);
+            int ix[kE2];
+            T px[kE2];
+#pragma unroll
+            for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+#pragma unroll
+            for (int j = 0; j < kE2; ++j) px[j] = ld(pe_ad, ix[j], B, b);
+            const T cv2 = ld(circ_ad, m.voe[2 * e + 1], B, b), cv1 = ld(circ_ad, m.voe[2 * e], B, b);
             T ua = ld(ua_p, e, B, b);
-            const int n = m.nEoE[e];
-            for (int j = 0; j < n; ++j) {
-                const int eo = m.eoe[kMaxEdges2 * e + j];
-                const int jt = m.eoeT[kMaxEdges2 * e + j];
-                const T vad = -apvm * dt * T(bs.gpt[eo]) * ld(pe_ad, eo, B, b);
-                ua = ua + T(m.woe[kMaxEdges2 * eo + jt]) * vad;
+#pragma unroll
+            for (int j = 0; j < kE2; ++j) {
+                const int eo = ix[j];
+                const int jt = m.eoeT[kE2 * e + j];
+                const T vad = -apvm * dt * T(bs.gpt[eo]) * px[j];
+                ua = ua + T(m.woe[kE2 * eo + jt]) * vad;
             }
-            ua = ua + T(m.dc[e]) * (ld(circ_ad, m.voe[2 * e + 1], B, b) -
-                                    ld(circ_ad, m.voe[2 * e], B, b));
+            ua = ua + T(m.dc[e]) * (cv2 - cv1);
             adj_update(stage, first, w, sw, ua, lamu, accu, gu, static_cast<long>(e) * B + b);
         } else {
-            const int ic = entity_of<WU>(t - nEt, lb);
-            const int n = m.nEoC[ic];
+            const int ic = entity_of<WU>(t, lb);
+            int iv[kC6];
+            T hx[kC6];
+#pragma unroll
+            for (int k = 0; k < kC6; ++k) iv[k] = m.vocR[kC6 * ic + k];
+#pragma unroll
+            for (int k = 0; k < kC6; ++k) hx[k] = ld(hv_ad, iv[k], B, b);
             T ha = ld(ha_p, ic, B, b);
-            for (int k = 0; k < n; ++k)
-                ha = ha + T(m.kiteCR[kMaxEdges * ic + k]) * ld(hv_ad, m.vocR[kMaxEdges * ic + k], B, b);
+#pragma unroll
+            for (int k = 0; k < kC6; ++k) ha = ha + T(m.kiteCR[kC6 * ic + k]) * hx[k];
             adj_update(stage, first, w, sw, ha, lamh, acch, gh, static_cast<long>(ic) * B + b);
         }
     }
@@ -739,17 +868,17 @@ void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, con
 template <typename T>
 void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
               const BaseStage& b, const T* h_tl, const T* u_tl, T* pvv_tl, T* ke_tl, T* pc_tl,
-              T* pe_tl, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s)
+              T* pe_tl, T* he_tl, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s)
 {
     LAUNCH_WU(k_tl_vertex, T, lb, static_cast<long>(m.nV) << lb, s, m, lb, b.hv, b.vort, h_tl, u_tl,
               pvv_tl);
     LAUNCH_WU(k_tl_cell, T, lb, static_cast<long>(m.nC) << lb, s, m, lb, b.u, u_tl, pvv_tl, ke_tl,
               pc_tl);
-    LAUNCH_WU(k_tl_edge, T, lb, static_cast<long>(m.nE) << lb, s, m, c, lb, b, u_tl, pvv_tl, pc_tl,
-              pe_tl);
+    LAUNCH_WU(k_tl_edge, T, lb, static_cast<long>(m.nE) << lb, s, m, c, lb, b, h_tl, u_tl, pvv_tl,
+              pc_tl, pe_tl, he_tl);
     const long threads = (static_cast<long>(m.nE) + m.nC) << lb;
     LAUNCH_WU(k_tl_tend, T, lb, threads, s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), b, h_tl,
-              u_tl, pe_tl, ke_tl, xh, xu, acch, accu, ph, pu);
+              u_tl, pe_tl, he_tl, ke_tl, xh, xu, acch, accu, ph, pu);
 }
 
 template <typename T>
@@ -845,7 +974,7 @@ void nonfinite_flags(long n, int lb, const T* x, int* flags, hipStream_t s)
                                         T*, T*, T*, T*, hipStream_t);                               \
     template void tl_stage<T>(const DMesh&, const DConfig&, int, int, const RKCoeffs&,              \
                               const BaseStage&, const T*, const T*, T*, T*, T*, T*, T*, T*, T*, T*, \
-                              T*, T*, hipStream_t);                                                 \
+                              T*, T*, T*, hipStream_t);                                             \
     template void adj_stage<T>(const DMesh&, const DConfig&, int, int, bool, const RKCoeffs&,       \
                                const BaseStage&, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, \
                                hipStream_t);                                                        \

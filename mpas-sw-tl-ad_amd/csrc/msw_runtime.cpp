@@ -233,6 +233,12 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
         if (static_cast<int>(cellEdges[o].size()) != n || static_cast<int>(cellVerts[o].size()) != n)
             return fail(MSW_EINVAL, "cell " + std::to_string(o + 1) +
                                         ": edgesOnCell, cellsOnEdge and cellsOnVertex disagree");
+        for (int i = n; i < kMaxEdges; ++i) {
+            // padding: a valid neighbour with a zero sign / kite
+            eoc[kMaxEdges * k + i] = ie[d->edgesOnCell[static_cast<long>(mE) * o] - 1];
+            eocR[kMaxEdges * k + i] = ie[cellEdges[o][0]];
+            vocR[kMaxEdges * k + i] = iv[cellVerts[o][0]];
+        }
         for (int i = 0; i < n; ++i) {
             int e = d->edgesOnCell[static_cast<long>(mE) * o + i] - 1;
             eoc[kMaxEdges * k + i] = ie[e];
@@ -270,6 +276,12 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
         voe[2 * k + 1] = iv[d->verticesOnEdge[2 * o + 1] - 1];
         const int n = d->nEdgesOnEdge[o];
         nEoE[k] = n;
+        for (int j = n; j < kMaxEdges2; ++j) {
+            // padding: the edge itself with a zero weight; the transposed
+            // slot points at this edge's first padded (zero-weight) slot
+            eoe[kMaxEdges2 * k + j] = k;
+            eoeT[kMaxEdges2 * k + j] = n;
+        }
         for (int j = 0; j < n; ++j) {
             const int eo = d->edgesOnEdge[static_cast<long>(mE2) * o + j] - 1;
             eoe[kMaxEdges2 * k + j] = ie[eo];
@@ -423,6 +435,7 @@ void enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
     T* ke_tl = static_cast<T*>(c->mc1);
     T* pc_tl = static_cast<T*>(c->mc2);
     T* pe_tl = static_cast<T*>(c->me1);
+    T* he_tl = static_cast<T*>(c->me2);
     c->st[0].h = h;
     c->st[0].u = u;
     const T* th = htl;
@@ -432,8 +445,8 @@ void enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
                          b.gpt, b.gpn, c->stream);
         T* nb = (s % 2 == 0) ? pA : pB;
-        tl_stage<T>(m, c->dcfg, c->lb, s, k, b, th, tu, pvv_tl, ke_tl, pc_tl, pe_tl, htl, utl, acc,
-                    acc + C * B, nb, nb + C * B, c->stream);
+        tl_stage<T>(m, c->dcfg, c->lb, s, k, b, th, tu, pvv_tl, ke_tl, pc_tl, pe_tl, he_tl, htl, utl,
+                    acc, acc + C * B, nb, nb + C * B, c->stream);
         double* nh = s < 3 ? c->st[s + 1].h : c->bend;
         fwd_tend_update_he<double>(m, c->dcfg, 0, s, k, b.h, b.u, b.he, b.pe, b.ke, h, u, c->bacc,
                                    c->bacc + C, nh, nh + C, c->stream);

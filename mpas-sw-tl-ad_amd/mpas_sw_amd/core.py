@@ -44,13 +44,14 @@ class SWModel:
     precision : "fp64" or "fp32" for member arrays.
     sfc : reorder entities along a 3-D Hilbert curve (default) or keep the
         reference order.
+    perm : explicit (cell, edge, vertex) permutations, new -> old (overrides sfc).
     """
 
     def __init__(self, mesh: Mesh, phys: dict, dt: float, *, apvm: float = 0.5, test_case: int = 2,
                  visc2: float = 0.0, visc4: float = 0.0, thickness_adv_order: int = 2,
                  monotonic: bool = False, wind_stress: bool = False, bottom_drag: bool = False,
                  batch: int = 1, precision: str = "fp64", sfc: bool = True, device=None,
-                 graphs: bool = False):
+                 graphs: bool = False, perm=None):
         if precision not in _DT:
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.lib = _lib.load()
@@ -60,7 +61,9 @@ class SWModel:
         self.dtype, self._prec = _DT[precision]
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.nC, self.nE, self.nV = mesh.nCells, mesh.nEdges, mesh.nVertices
-        if sfc:
+        if perm is not None:
+            pc, pe, pv = perm
+        elif sfc:
             pc, pe, pv = sfc_permutation(mesh)
         else:
             pc, pe, pv = (np.arange(self.nC), np.arange(self.nE), np.arange(self.nV))

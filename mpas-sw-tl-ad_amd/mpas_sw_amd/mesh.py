@@ -450,3 +450,57 @@ def get_mesh(level: int) -> Mesh:
         m = icosahedral_mesh(level)
         _CACHE[level] = m
     return m
+
+
+def union_mesh(meshes):
+    """Pack several meshes into one index space (a disjoint union).
+
+    Instance k's cells, edges and vertices follow those of instances < k;
+    its connectivity is shifted by the same offsets, so one launch of any
+    stage kernel advances every instance.  Returns (mesh, offsets) with
+    offsets[k] = (cell0, edge0, vertex0) of instance k.
+    """
+    offs = []
+    c0 = e0 = v0 = 0
+    for m in meshes:
+        offs.append((c0, e0, v0))
+        c0 += m.nCells
+        e0 += m.nEdges
+        v0 += m.nVertices
+    kw = {"level": -1, "nCells": c0, "nEdges": e0, "nVertices": v0}
+
+    def cat(name):
+        return np.concatenate([getattr(m, name) for m in meshes], axis=-1)
+
+    def shifted(name, which):
+        parts = []
+        for m, (oc, oe, ov) in zip(meshes, offs):
+            a = getattr(m, name).astype(np.int64)
+            off = {"c": oc, "e": oe, "v": ov}[which]
+            parts.append(np.where(a > 0, a + off, 0).astype(np.int32))
+        return np.concatenate(parts, axis=-1)
+
+    for f in dataclasses.fields(Mesh):
+        n = f.name
+        if n in kw or n in ("maxEdges", "maxEdges2", "vertexDegree"):
+            continue
+        if n in ("cellsOnEdge", "cellsOnCell", "cellsOnVertex"):
+            kw[n] = shifted(n, "c")
+        elif n in ("edgesOnCell", "edgesOnEdge", "edgesOnVertex"):
+            kw[n] = shifted(n, "e")
+        elif n in ("verticesOnEdge", "verticesOnCell"):
+            kw[n] = shifted(n, "v")
+        else:
+            kw[n] = cat(n)
+    return Mesh(**kw), offs
+
+
+def union_sfc_permutation(meshes, offsets):
+    """Per-instance space-filling-curve order, instances kept contiguous."""
+    pcs, pes, pvs = [], [], []
+    for m, (oc, oe, ov) in zip(meshes, offsets):
+        pc, pe, pv = sfc_permutation(m)
+        pcs.append(pc + oc)
+        pes.append(pe + oe)
+        pvs.append(pv + ov)
+    return np.concatenate(pcs), np.concatenate(pes), np.concatenate(pvs)

@@ -207,3 +207,25 @@ def test_fp32_members_track_fp64():
         out[prec] = (tl, m.to_reference(ht, "c"), m.to_reference(ut, "e"))
     for a, b in zip(out["fp32"], out["fp64"]):
         assert rel_l2(a, b) < 1e-5
+
+
+def test_packed_uneven_batch_equals_separate_runs():
+    """Instances of different sizes packed in one index space step exactly as
+    they do alone (one launch per stage for all of them)."""
+    print("packed instances L2 tc2 + L3 tc6 + L2 tc6", flush=True)
+    from mpas_sw_amd.core import SWModel
+    from mpas_sw_amd.ensemble import packed_instances
+    specs = [(2, 2), (3, 6), (2, 6)]
+    dt = 900.0
+    mesh, phys, offs, perm = packed_instances(specs)
+    m = SWModel(mesh, phys, dt, test_case=2, perm=perm, precision="fp64")
+    h = m.to_internal(phys["h"], "c", member=False)
+    u = m.to_internal(phys["u"], "e", member=False)
+    m.base(h, u, 5)
+    H, U = m.to_reference(h, "c"), m.to_reference(u, "e")
+    for (level, tc), (oc, oe, ov) in zip(specs, offs):
+        mm, pp = case(level, tc)
+        o = make_oracle(level, tc, dt)
+        o.rk4(5)
+        np.testing.assert_array_equal(H[oc:oc + mm.nCells], o["h1"])
+        np.testing.assert_array_equal(U[oe:oe + mm.nEdges], o["u1"])
