@@ -39,6 +39,8 @@ struct DMesh {
     const int* eoe;       // [E][12]
     const double* woe;    // [E][12]
     const int* eoeT;      // [E][12] slot of this edge in eoe[eoe[e][j]]
+    const double* woeT;   // [E][12] woe[eoe[e][j]][eoeT[e][j]]: this edge's weight in
+                          //         the list of its j-th neighbour (0 when padded)
     const double* dc;     // [E]
     const double* dv;     // [E]
     // vertices
@@ -110,7 +112,7 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
 template <typename T>
 void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, const RKCoeffs& k,
                const BaseStage& b, T* gh, T* gu, T* pe_ad, T* he_ad, T* ua_p, T* pc_ad, T* ha_p,
-               T* hv_ad, T* circ_ad, T* lamh, T* lamu, T* acch, T* accu, hipStream_t s);
+               T* ke_ad, T* hv_ad, T* circ_ad, T* lamh, T* lamu, T* acch, T* accu, hipStream_t s);
 // g = w[3] * lam  (seed of the reverse sweep)
 template <typename T>
 void adj_seed(int n, T* g, const T* lam, double w3, hipStream_t s);

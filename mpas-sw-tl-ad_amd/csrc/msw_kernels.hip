@@ -33,62 +33,82 @@ constexpr int kE2 = kMaxEdges2;
 constexpr int kC6 = kMaxEdges;
 
 template <bool WU>
-__device__ __forceinline__ int entity_of(long t, int lb)
+__device__ __forceinline__ int entity_of(unsigned t, int lb)
 {
     int e = static_cast<int>(t >> lb);
     if constexpr (WU) e = __builtin_amdgcn_readfirstlane(e);
     return e;
 }
 
-// logical block index: XCD x (= blockIdx % 8) owns the contiguous range
-// [x * gridDim/8, (x+1) * gridDim/8); gridDim is a multiple of 8
-__device__ __forceinline__ long xcd_block()
+// Blocks are dealt round-robin to the 8 XCDs (block i runs on XCD i % 8).
+// The launch grid is 8 * per blocks and XCD x takes the logical blocks
+// [x * per, (x + 1) * per), one contiguous stretch of the curve order.
+// Every thread handles one item (one member of one entity); all index
+// arithmetic is 32-bit (items < 2^31 is checked when a context is made).
+__device__ __forceinline__ unsigned xcd_block()
 {
-    const long per = static_cast<long>(gridDim.x >> 3);
-    return static_cast<long>(blockIdx.x & 7u) * per + static_cast<long>(blockIdx.x >> 3);
+    return (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
 }
 
+// blocks for `items` threads, rounded up to a multiple of 8
+inline int grid_items(long items)
+{
+    const long per = (items + 8L * kBlock - 1) / (8L * kBlock);
+    return static_cast<int>(8 * (per > 0 ? per : 1));
+}
+
+// grid for the fused edge + cell kernels: each XCD takes one eighth of the
+// edges and the matching eighth of the cells
+inline int grid_edge_cell(int nE, int nC, int lb)
+{
+    const long pe = ((static_cast<long>(nE) << lb) + 8L * kBlock - 1) / (8L * kBlock);
+    const long pc = ((static_cast<long>(nC) << lb) + 8L * kBlock - 1) / (8L * kBlock);
+    return static_cast<int>(8 * (pe + pc));
+}
+
+// grid-stride form for the elementwise kernels (any grid)
 inline int grid_for(long threads)
 {
     long blocks = (threads + kBlock - 1) / kBlock;
-    const long cap = 1L << 20;   // block-stride beyond ~268M threads
+    const long cap = 1L << 20;
     if (blocks > cap) blocks = cap;
-    blocks = (blocks + 7) & ~7L;
-    if (blocks < 8) blocks = 8;
+    if (blocks < 1) blocks = 1;
     return static_cast<int>(blocks);
 }
 
-inline long nblocks(long threads) { return (threads + kBlock - 1) / kBlock; }
+// one item per thread over `count` entities
+#define ENTITY_ITEM(count)                                                     \
+    const unsigned total_ = static_cast<unsigned>(count) << lb;                \
+    const unsigned t = xcd_block() * kBlock + threadIdx.x;                     \
+    if (t >= total_) return;
 
-// one entity range
-#define ENTITY_LOOP(count)                                                               \
-    const long total_ = static_cast<long>(count) << lb;                                 \
-    const long nb_ = (total_ + kBlock - 1) / kBlock;                                     \
-    for (long L_ = xcd_block(); L_ < nb_; L_ += gridDim.x)                               \
-        for (long t = L_ * kBlock + threadIdx.x; t < total_; t = total_)
-
-// edges and cells interleaved in proportion along the curve, so one XCD
-// updates the edges and the cells of the same region
-#define EDGE_CELL_LOOP                                                                  \
-    const long tE_ = static_cast<long>(m.nE) << lb, tC_ = static_cast<long>(m.nC) << lb; \
-    const long nbE_ = (tE_ + kBlock - 1) / kBlock, nbC_ = (tC_ + kBlock - 1) / kBlock;   \
-    const long N_ = nbE_ + nbC_;                                                         \
-    for (long L_ = xcd_block(); L_ < N_; L_ += gridDim.x)
-
-#define EDGE_CELL_SPLIT(is_edge, t)                                                     \
-    const long e0_ = (L_ * nbE_) / N_, e1_ = ((L_ + 1) * nbE_) / N_;                    \
-    const bool is_edge = e1_ > e0_;                                                      \
-    const long t = (is_edge ? e0_ : L_ - e0_) * kBlock + threadIdx.x;                   \
-    if (t >= (is_edge ? tE_ : tC_)) continue;
+// one item per thread over the edges (is_edge) or the cells
+#define EDGE_CELL_ITEM(is_edge, t)                                             \
+    const unsigned tE_ = static_cast<unsigned>(m.nE) << lb;                    \
+    const unsigned tC_ = static_cast<unsigned>(m.nC) << lb;                    \
+    const unsigned pE_ = (tE_ + 8u * kBlock - 1) / (8u * kBlock);              \
+    const unsigned pC_ = (tC_ + 8u * kBlock - 1) / (8u * kBlock);              \
+    const unsigned x_ = blockIdx.x & 7u, j_ = blockIdx.x >> 3;                 \
+    const bool is_edge = j_ < pE_;                                             \
+    const unsigned t = (is_edge ? x_ * pE_ + j_ : x_ * pC_ + (j_ - pE_)) * kBlock + threadIdx.x; \
+    if (t >= (is_edge ? tE_ : tC_)) return;
 
 #define GRID_LOOP(total)                                                               \
     for (long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x; t < (total); \
          t += static_cast<long>(gridDim.x) * kBlock)
 
+// row `row` of a members-innermost field: the row base is wave-uniform
+// whenever the row index is, so the load is one SGPR base + lane offset
 template <typename T>
-__device__ __forceinline__ T ld(const T* p, long row, int B, int b)
+__device__ __forceinline__ T* rowp(T* p, int row, int B)
 {
-    return p[row * B + b];
+    return p + static_cast<size_t>(static_cast<unsigned>(row)) * static_cast<unsigned>(B);
+}
+
+template <typename T>
+__device__ __forceinline__ T ld(const T* p, int row, int B, int b)
+{
+    return rowp(p, row, B)[b];
 }
 
 // ---------------------------------------------------------------------------
@@ -101,7 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_vertex(DMesh m, int lb, const T*
                                                        T* __restrict__ hv, T* __restrict__ vort)
 {
     const int B = 1 << lb;
-    ENTITY_LOOP(m.nV)
+    ENTITY_ITEM(m.nV)
     {
         const int iv = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
@@ -120,10 +140,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd_vertex(DMesh m, int lb, const T*
 #pragma unroll
         for (int i = 0; i < 3; ++i) hh = hh + hc[i] * T(m.kite[3 * iv + i]);
         hh = hh / at;
-        const long o = static_cast<long>(iv) * B + b;
-        pvv[o] = (T(m.fV[iv]) + vo) / hh;
-        if (hv) hv[o] = hh;
-        if (vort) vort[o] = vo;
+        rowp(pvv, iv, B)[b] = (T(m.fV[iv]) + vo) / hh;
+        if (hv) rowp(hv, iv, B)[b] = hh;
+        if (vort) rowp(vort, iv, B)[b] = vo;
     }
 }
 
@@ -134,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_cell(DMesh m, int lb, const T* _
                                                      T* __restrict__ pc)
 {
     const int B = 1 << lb;
-    ENTITY_LOOP(m.nC)
+    ENTITY_ITEM(m.nC)
     {
         const int ic = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
@@ -155,9 +174,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_cell(DMesh m, int lb, const T* _
         T p = T(0);
 #pragma unroll
         for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
-        const long o = static_cast<long>(ic) * B + b;
-        ke[o] = k;
-        pc[o] = p;
+        rowp(ke, ic, B)[b] = k;
+        rowp(pc, ic, B)[b] = p;
     }
 }
 
@@ -175,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_edge(DMesh m, DConfig c, int lb,
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    ENTITY_LOOP(m.nE)
+    ENTITY_ITEM(m.nE)
     {
         const int e = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
@@ -199,20 +217,22 @@ __global__ __launch_bounds__(kBlock) void k_fwd_edge(DMesh m, DConfig c, int lb,
         T p = T(0.5) * p1 + T(0.5) * p2;
         p = p - apvm * vv * dt * gt;
         p = p - apvm * ue * dt * gn;
-        const long o = static_cast<long>(e) * B + b;
-        he[o] = T(0.5) * (h1 + h2);
-        pe[o] = p;
-        if (vout) vout[o] = vv;
-        if (gpt_out) gpt_out[o] = gt;
-        if (gpn_out) gpn_out[o] = gn;
+        rowp(he, e, B)[b] = T(0.5) * (h1 + h2);
+        rowp(pe, e, B)[b] = p;
+        if (vout) rowp(vout, e, B)[b] = vv;
+        if (gpt_out) rowp(gpt_out, e, B)[b] = gt;
+        if (gpn_out) rowp(gpn_out, e, B)[b] = gn;
     }
 }
 
 // RK4 bookkeeping of one stage for one value (src/mpas_sw_module.f90:590-611)
 template <typename T>
 __device__ __forceinline__ void rk_update(int stage, T w, T sw, T tend, T* x, T* acc, T* pnext,
-                                          long o, bool frozen)
+                                          int row, int B, int o, bool frozen)
 {
+    x = rowp(x, row, B);
+    acc = rowp(acc, row, B);
+    pnext = rowp(pnext, row, B);
     if (stage == 0) {
         const T xv = x[o];
         acc[o] = xv + w * tend;
@@ -237,9 +257,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
                                                      T* acch, T* accu, T* ph, T* pu)
 {
     const int B = 1 << lb;
-    EDGE_CELL_LOOP
+    EDGE_CELL_ITEM(is_edge, t)
     {
-        EDGE_CELL_SPLIT(is_edge, t)
         const int b = static_cast<int>(t & (B - 1));
         if (is_edge) {
             const int e = entity_of<WU>(t, lb);
@@ -265,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
             }
             const T tend = q - (k2 - k1 + T(kGravity) * (h2 + T(m.hs[c2]) - h1 - T(m.hs[c1]))) /
                                    T(m.dc[e]);
-            rk_update(stage, w, sw, tend, xu, accu, pu, static_cast<long>(e) * B + b,
+            rk_update(stage, w, sw, tend, xu, accu, pu, e, B, b,
                       c.frozen_u != 0);
         } else {
             const int ic = entity_of<WU>(t, lb);
@@ -282,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
                 th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
             }
             th = th / T(m.areaC[ic]);
-            rk_update(stage, w, sw, th, xh, acch, ph, static_cast<long>(ic) * B + b, false);
+            rk_update(stage, w, sw, th, xh, acch, ph, ic, B, b, false);
         }
     }
 }
@@ -300,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_vertex(DMesh m, int lb, const dou
                                                       T* __restrict__ pvv_tl)
 {
     const int B = 1 << lb;
-    ENTITY_LOOP(m.nV)
+    ENTITY_ITEM(m.nV)
     {
         const int iv = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
@@ -320,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_vertex(DMesh m, int lb, const dou
         for (int i = 0; i < 3; ++i) ht = ht + hc[i] * T(m.kite[3 * iv + i]);
         ht = ht / at;
         const T hv = T(hvb[iv]);
-        pvv_tl[static_cast<long>(iv) * B + b] =
+        rowp(pvv_tl, iv, B)[b] =
             vt / hv - ht * (T(m.fV[iv]) + T(vortb[iv])) / (hv * hv);
     }
 }
@@ -333,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_cell(DMesh m, int lb, const doubl
                                                     T* __restrict__ ke_tl, T* __restrict__ pc_tl)
 {
     const int B = 1 << lb;
-    ENTITY_LOOP(m.nC)
+    ENTITY_ITEM(m.nC)
     {
         const int ic = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
@@ -354,9 +373,8 @@ __global__ __launch_bounds__(kBlock) void k_tl_cell(DMesh m, int lb, const doubl
         T p = T(0);
 #pragma unroll
         for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
-        const long o = static_cast<long>(ic) * B + b;
-        ke_tl[o] = k;
-        pc_tl[o] = p;
+        rowp(ke_tl, ic, B)[b] = k;
+        rowp(pc_tl, ic, B)[b] = p;
     }
 }
 
@@ -371,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_edge(DMesh m, DConfig c, int lb, 
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    ENTITY_LOOP(m.nE)
+    ENTITY_ITEM(m.nE)
     {
         const int e = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
@@ -395,9 +413,8 @@ __global__ __launch_bounds__(kBlock) void k_tl_edge(DMesh m, DConfig c, int lb, 
         T p = T(0.5) * p1 + T(0.5) * p2;
         p = p - apvm * vt * dt * T(bs.gpt[e]) - apvm * T(bs.v[e]) * dt * gtt;
         p = p - apvm * ue * dt * T(bs.gpn[e]) - apvm * T(bs.u[e]) * dt * gnt;
-        const long o = static_cast<long>(e) * B + b;
-        pe_tl[o] = p;
-        he_tl[o] = T(0.5) * (h1 + h2);
+        rowp(pe_tl, e, B)[b] = p;
+        rowp(he_tl, e, B)[b] = T(0.5) * (h1 + h2);
     }
 }
 
@@ -413,9 +430,8 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
                                                     T* acch, T* accu, T* ph, T* pu)
 {
     const int B = 1 << lb;
-    EDGE_CELL_LOOP
+    EDGE_CELL_ITEM(is_edge, t)
     {
-        EDGE_CELL_SPLIT(is_edge, t)
         const int b = static_cast<int>(t & (B - 1));
         if (is_edge) {
             const int e = entity_of<WU>(t, lb);
@@ -446,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
                 qt = qt + wgt * ux[j] * wpv * heb + wgt * ubo * wpvt * heb + wgt * ubo * wpv * hx[j];
             }
             const T tend = qt - (k2 - k1 + T(kGravity) * (h2 - h1)) / T(m.dc[e]);
-            rk_update(stage, w, sw, tend, xu, accu, pu, static_cast<long>(e) * B + b,
+            rk_update(stage, w, sw, tend, xu, accu, pu, e, B, b,
                       c.frozen_u != 0);
         } else {
             const int ic = entity_of<WU>(t, lb);
@@ -464,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
                 th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
             }
             th = th / T(m.areaC[ic]);
-            rk_update(stage, w, sw, th, xh, acch, ph, static_cast<long>(ic) * B + b, false);
+            rk_update(stage, w, sw, th, xh, acch, ph, ic, B, b, false);
         }
     }
 }
@@ -475,8 +491,9 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
 // reference becomes a sum at the receiving entity, so no atomics are needed
 // and results are deterministic.
 
-// tend_u/q adjoint (:689-717), tend_h adjoint (:720-736), ke -> u (:306-318),
-// upwinding in the normal direction (:193-200)
+// tend_u/q adjoint (:689-717), tend_h adjoint (:720-736), upwinding in the
+// normal direction (:193-200).  The kinetic-energy part of tend_u's
+// adjoint is a per-cell sum and is formed in k_adj_cell1.
 template <typename T, bool WU>
 __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, DConfig c, int lb, BaseStage bs,
                                                       const T* __restrict__ gh,
@@ -487,25 +504,21 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, DConfig c, int lb
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    ENTITY_LOOP(m.nE)
+    ENTITY_ITEM(m.nE)
     {
         const int e = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
         const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
-        int ix[kE2], i1[kC6], i2[kC6];
-        T gx[kE2], g1[kC6], g2[kC6];
+        int ix[kE2];
+        T gx[kE2];
 #pragma unroll
         for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
 #pragma unroll
-        for (int i = 0; i < kC6; ++i) { i1[i] = m.eoc[kC6 * c1 + i]; i2[i] = m.eoc[kC6 * c2 + i]; }
-#pragma unroll
         for (int j = 0; j < kE2; ++j) gx[j] = ld(gu, ix[j], B, b);
-#pragma unroll
-        for (int i = 0; i < kC6; ++i) { g1[i] = ld(gu, i1[i], B, b); g2[i] = ld(gu, i2[i], B, b); }
         const T gh1 = ld(gh, c1, B, b), gh2 = ld(gh, c2, B, b);
         const T ge = ld(gu, e, B, b);
         const T a1 = T(m.areaC[c1]), a2 = T(m.areaC[c2]);
-        const T dv = T(m.dv[e]), dc = T(m.dc[e]);
+        const T dv = T(m.dv[e]);
         const T ub = T(bs.u[e]), heb = T(bs.he[e]), peb = T(bs.pe[e]);
         const T flux_ad = gh2 / a2 - gh1 / a1;
         T pad = T(0);
@@ -514,47 +527,37 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, DConfig c, int lb
 #pragma unroll
         for (int j = 0; j < kE2; ++j) {
             const int eo = ix[j];
-            const int jt = m.eoeT[kE2 * e + j];
             const T wgt = T(m.woe[kE2 * e + j]);
-            const T wt = T(m.woe[kE2 * eo + jt]);
+            const T wt = T(m.woeT[kE2 * e + j]);
             const T wpv = T(0.5) * (peb + T(bs.pe[eo]));
             // this edge's own q: pv_edge_ad(e) += 0.5*w*u(eo)*he(eo)*q_ad(e)
             pad = pad + T(0.5) * (wgt * T(bs.u[eo]) * T(bs.he[eo]) * ge);
-            // edge eo's q, in which this edge is neighbour jt
+            // edge eo's q, in which this edge is neighbour eoeT[e][j]
             had = had + wt * ub * wpv * gx[j];
             uad = uad + wt * wpv * heb * gx[j];
             pad = pad + T(0.5) * (wt * ub * heb * gx[j]);
         }
-        // ke_ad of both cells from tend_u's kinetic-energy gradient
-        T k1 = T(0), k2 = T(0);
-#pragma unroll
-        for (int i = 0; i < kC6; ++i) {
-            k1 = k1 + T(m.sgnC[kC6 * c1 + i]) * g1[i] / T(m.dc[i1[i]]);
-            k2 = k2 + T(m.sgnC[kC6 * c2 + i]) * g2[i] / T(m.dc[i2[i]]);
-        }
-        const T kc = T(0.25) * dc * dv * T(2.0) * ub;
-        uad = uad + kc * (k1 / a1) + kc * (k2 / a2);
         uad = uad - apvm * dt * T(bs.gpn[e]) * pad;
-        const long o = static_cast<long>(e) * B + b;
-        pe_ad[o] = pad;
-        he_ad[o] = had;
-        ua_p[o] = uad;
+        rowp(pe_ad, e, B)[b] = pad;
+        rowp(he_ad, e, B)[b] = had;
+        rowp(ua_p, e, B)[b] = uad;
     }
 }
 
 // gradPVn -> pv_cell (:202-213), tend_u's gravity term -> h (:696-697),
-// h_edge -> h (:358-369)
+// h_edge -> h (:358-369), tend_u's kinetic-energy gradient -> ke (:693-695)
 template <typename T, bool WU>
 __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb, BaseStage bs,
                                                       const T* __restrict__ gu,
                                                       const T* __restrict__ pe_ad,
                                                       const T* __restrict__ he_ad,
                                                       T* __restrict__ pc_ad,
-                                                      T* __restrict__ ha_p)
+                                                      T* __restrict__ ha_p,
+                                                      T* __restrict__ ke_ad)
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    ENTITY_LOOP(m.nC)
+    ENTITY_ITEM(m.nC)
     {
         const int ic = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
@@ -568,7 +571,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
             pa[i] = ld(pe_ad, ie[i], B, b);
             ha[i] = ld(he_ad, ie[i], B, b);
         }
-        T pcad = T(0), had = T(0);
+        T pcad = T(0), had = T(0), kad = T(0);
 #pragma unroll
         for (int i = 0; i < kC6; ++i) {
             const int e = ie[i];
@@ -578,10 +581,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
             const T gpn_ad = -apvm * T(bs.u[e]) * dt * pa[i];
             pcad = pcad - s * gpn_ad / dc;
             had = had + s * (T(kGravity) / dc * ge[i]) + mask * (T(0.5) * ha[i]);
+            kad = kad + s * ge[i] / dc;
         }
-        const long o = static_cast<long>(ic) * B + b;
-        pc_ad[o] = pcad;
-        ha_p[o] = had;
+        rowp(pc_ad, ic, B)[b] = pcad;
+        rowp(ha_p, ic, B)[b] = had;
+        rowp(ke_ad, ic, B)[b] = kad / T(m.areaC[ic]);
     }
 }
 
@@ -597,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    ENTITY_LOOP(m.nV)
+    ENTITY_ITEM(m.nV)
     {
         const int iv = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
@@ -621,17 +625,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
         T hva = -(T(m.fV[iv]) + T(bs.vort[iv])) / (hv * hv) * pa;
         hva = hva / at;
         const T vora = pa / hv;
-        const long o = static_cast<long>(iv) * B + b;
-        hv_ad[o] = hva;
-        circ_ad[o] = vora / at;
+        rowp(hv_ad, iv, B)[b] = hva;
+        rowp(circ_ad, iv, B)[b] = vora / at;
     }
 }
 
 // Epilogue of an adjoint stage for one value
 template <typename T>
 __device__ __forceinline__ void adj_update(int stage, bool first, T w, T sw, T pa, T* lam, T* acc,
-                                           T* g, long o)
+                                           T* g, int row, int B, int o)
 {
+    lam = rowp(lam, row, B);
+    acc = rowp(acc, row, B);
+    g = rowp(g, row, B);
     if (stage > 0) {
         acc[o] = first ? pa : acc[o] + pa;
         g[o] = w * lam[o] + sw * pa;
@@ -650,14 +656,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
                                                       const T* __restrict__ ua_p,
                                                       const T* __restrict__ ha_p,
                                                       const T* __restrict__ hv_ad,
-                                                      const T* __restrict__ circ_ad, T* lamh,
+                                                      const T* __restrict__ circ_ad,
+                                                      const T* __restrict__ ke_ad, T* lamh,
                                                       T* lamu, T* acch, T* accu, T* gh, T* gu)
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
-    EDGE_CELL_LOOP
+    EDGE_CELL_ITEM(is_edge, t)
     {
-        EDGE_CELL_SPLIT(is_edge, t)
         const int b = static_cast<int>(t & (B - 1));
         if (is_edge) {
             const int e = entity_of<WU>(t, lb);
@@ -667,17 +673,20 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
             for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
 #pragma unroll
             for (int j = 0; j < kE2; ++j) px[j] = ld(pe_ad, ix[j], B, b);
+            const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
             const T cv2 = ld(circ_ad, m.voe[2 * e + 1], B, b), cv1 = ld(circ_ad, m.voe[2 * e], B, b);
+            const T k1 = ld(ke_ad, c1, B, b), k2 = ld(ke_ad, c2, B, b);
             T ua = ld(ua_p, e, B, b);
 #pragma unroll
             for (int j = 0; j < kE2; ++j) {
-                const int eo = ix[j];
-                const int jt = m.eoeT[kE2 * e + j];
-                const T vad = -apvm * dt * T(bs.gpt[eo]) * px[j];
-                ua = ua + T(m.woe[kE2 * eo + jt]) * vad;
+                const T vad = -apvm * dt * T(bs.gpt[ix[j]]) * px[j];
+                ua = ua + T(m.woeT[kE2 * e + j]) * vad;
             }
             ua = ua + T(m.dc[e]) * (cv2 - cv1);
-            adj_update(stage, first, w, sw, ua, lamu, accu, gu, static_cast<long>(e) * B + b);
+            // ke(c) = sum 0.25*dc*dv*u^2 / area over the cell's edges (:245-256)
+            const T kc = T(0.25) * T(m.dc[e]) * T(m.dv[e]) * T(2.0) * T(bs.u[e]);
+            ua = ua + kc * k1 + kc * k2;
+            adj_update(stage, first, w, sw, ua, lamu, accu, gu, e, B, b);
         } else {
             const int ic = entity_of<WU>(t, lb);
             int iv[kC6];
@@ -689,7 +698,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
             T ha = ld(ha_p, ic, B, b);
 #pragma unroll
             for (int k = 0; k < kC6; ++k) ha = ha + T(m.kiteCR[kC6 * ic + k]) * hx[k];
-            adj_update(stage, first, w, sw, ha, lamh, acch, gh, static_cast<long>(ic) * B + b);
+            adj_update(stage, first, w, sw, ha, lamh, acch, gh, ic, B, b);
         }
     }
 }
@@ -729,12 +738,11 @@ __global__ __launch_bounds__(kBlock) void k_reconstruct(int nC, int lb, const in
         }
         const T clat = T(trig[4 * ic]), slat = T(trig[4 * ic + 1]);
         const T clon = T(trig[4 * ic + 2]), slon = T(trig[4 * ic + 3]);
-        const long o = static_cast<long>(ic) * B + b;
-        out[o] = X;
-        out[plane + o] = Y;
-        out[2 * plane + o] = Z;
-        out[3 * plane + o] = -X * slon + Y * clon;
-        out[4 * plane + o] = -(X * clon + Y * slon) * slat + Z * clat;
+        rowp(out, ic, B)[b] = X;
+        rowp(out + plane, ic, B)[b] = Y;
+        rowp(out + 2 * plane, ic, B)[b] = Z;
+        rowp(out + 3 * plane, ic, B)[b] = -X * slon + Y * clon;
+        rowp(out + 4 * plane, ic, B)[b] = -(X * clon + Y * slon) * slat + Z * clat;
     }
 }
 
@@ -836,9 +844,9 @@ RKCoeffs rk_coeffs(double dt)
     return k;
 }
 
-#define LAUNCH_WU(kernel, T, lb, threads, stream, ...)                                       \
+#define LAUNCH_WU(kernel, T, lb, grid, stream, ...)                                          \
     do {                                                                                     \
-        const int _g = grid_for(threads);                                                    \
+        const int _g = (grid);                                                               \
         if ((lb) >= 6)                                                                       \
             hipLaunchKernelGGL((kernel<T, true>), dim3(_g), dim3(kBlock), 0, stream, __VA_ARGS__); \
         else                                                                                 \
@@ -849,9 +857,9 @@ template <typename T>
 void fwd_diag(const DMesh& m, const DConfig& c, int lb, const T* h, const T* u, T* pvv, T* hv,
               T* vort, T* ke, T* pc, T* he, T* pe, T* v, T* gpt, T* gpn, hipStream_t s)
 {
-    LAUNCH_WU(k_fwd_vertex, T, lb, static_cast<long>(m.nV) << lb, s, m, lb, h, u, pvv, hv, vort);
-    LAUNCH_WU(k_fwd_cell, T, lb, static_cast<long>(m.nC) << lb, s, m, lb, u, pvv, ke, pc);
-    LAUNCH_WU(k_fwd_edge, T, lb, static_cast<long>(m.nE) << lb, s, m, c, lb, h, u, pvv, pc, he, pe, v,
+    LAUNCH_WU(k_fwd_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, lb, h, u, pvv, hv, vort);
+    LAUNCH_WU(k_fwd_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, u, pvv, ke, pc);
+    LAUNCH_WU(k_fwd_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, h, u, pvv, pc, he, pe, v,
               gpt, gpn);
 }
 
@@ -860,8 +868,7 @@ void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, con
                         const T* h, const T* u, const T* he, const T* pe, const T* ke, T* xh,
                         T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s)
 {
-    const long threads = (static_cast<long>(m.nE) + m.nC) << lb;
-    LAUNCH_WU(k_fwd_tend, T, lb, threads, s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), h, u,
+    LAUNCH_WU(k_fwd_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), h, u,
               he, pe, ke, xh, xu, acch, accu, ph, pu);
 }
 
@@ -870,33 +877,31 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
               const BaseStage& b, const T* h_tl, const T* u_tl, T* pvv_tl, T* ke_tl, T* pc_tl,
               T* pe_tl, T* he_tl, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s)
 {
-    LAUNCH_WU(k_tl_vertex, T, lb, static_cast<long>(m.nV) << lb, s, m, lb, b.hv, b.vort, h_tl, u_tl,
+    LAUNCH_WU(k_tl_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, lb, b.hv, b.vort, h_tl, u_tl,
               pvv_tl);
-    LAUNCH_WU(k_tl_cell, T, lb, static_cast<long>(m.nC) << lb, s, m, lb, b.u, u_tl, pvv_tl, ke_tl,
+    LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, b.u, u_tl, pvv_tl, ke_tl,
               pc_tl);
-    LAUNCH_WU(k_tl_edge, T, lb, static_cast<long>(m.nE) << lb, s, m, c, lb, b, h_tl, u_tl, pvv_tl,
+    LAUNCH_WU(k_tl_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, b, h_tl, u_tl, pvv_tl,
               pc_tl, pe_tl, he_tl);
-    const long threads = (static_cast<long>(m.nE) + m.nC) << lb;
-    LAUNCH_WU(k_tl_tend, T, lb, threads, s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), b, h_tl,
+    LAUNCH_WU(k_tl_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), b, h_tl,
               u_tl, pe_tl, he_tl, ke_tl, xh, xu, acch, accu, ph, pu);
 }
 
 template <typename T>
 void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, const RKCoeffs& k,
                const BaseStage& b, T* gh, T* gu, T* pe_ad, T* he_ad, T* ua_p, T* pc_ad, T* ha_p,
-               T* hv_ad, T* circ_ad, T* lamh, T* lamu, T* acch, T* accu, hipStream_t s)
+               T* ke_ad, T* hv_ad, T* circ_ad, T* lamh, T* lamu, T* acch, T* accu, hipStream_t s)
 {
-    LAUNCH_WU(k_adj_edge1, T, lb, static_cast<long>(m.nE) << lb, s, m, c, lb, b, gh, gu, pe_ad, he_ad,
+    LAUNCH_WU(k_adj_edge1, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, b, gh, gu, pe_ad, he_ad,
               ua_p);
-    LAUNCH_WU(k_adj_cell1, T, lb, static_cast<long>(m.nC) << lb, s, m, c, lb, b, gu, pe_ad, he_ad,
-              pc_ad, ha_p);
-    LAUNCH_WU(k_adj_vertex, T, lb, static_cast<long>(m.nV) << lb, s, m, c, lb, b, pe_ad, pc_ad, hv_ad,
+    LAUNCH_WU(k_adj_cell1, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, c, lb, b, gu, pe_ad, he_ad,
+              pc_ad, ha_p, ke_ad);
+    LAUNCH_WU(k_adj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, c, lb, b, pe_ad, pc_ad, hv_ad,
               circ_ad);
     const double w = stage > 0 ? k.w[stage - 1] : 0.0;
     const double sw = stage > 0 ? k.sw[stage - 1] : 0.0;
-    const long threads = (static_cast<long>(m.nE) + m.nC) << lb;
-    LAUNCH_WU(k_adj_final, T, lb, threads, s, m, c, lb, stage, first, T(w), T(sw), b, pe_ad, ua_p,
-              ha_p, hv_ad, circ_ad, lamh, lamu, acch, accu, gh, gu);
+    LAUNCH_WU(k_adj_final, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, first, T(w), T(sw), b, pe_ad, ua_p,
+              ha_p, hv_ad, circ_ad, ke_ad, lamh, lamu, acch, accu, gh, gu);
 }
 
 template <typename T>
@@ -910,7 +915,7 @@ template <typename T>
 void reconstruct(int nC, int lb, const int* nEoC, const int* eoc, const double* coeffs,
                  const double* trig, const T* u, T* out5, hipStream_t s)
 {
-    LAUNCH_WU(k_reconstruct, T, lb, static_cast<long>(nC) << lb, s, nC, lb, nEoC, eoc, coeffs, trig,
+    LAUNCH_WU(k_reconstruct, T, lb, grid_for(static_cast<long>(nC) << lb), s, nC, lb, nEoC, eoc, coeffs, trig,
               u, out5);
 }
 
@@ -977,7 +982,7 @@ void nonfinite_flags(long n, int lb, const T* x, int* flags, hipStream_t s)
                               T*, T*, T*, hipStream_t);                                             \
     template void adj_stage<T>(const DMesh&, const DConfig&, int, int, bool, const RKCoeffs&,       \
                                const BaseStage&, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, \
-                               hipStream_t);                                                        \
+                               T*, hipStream_t);                                                        \
     template void adj_seed<T>(int, T*, const T*, double, hipStream_t);                              \
     template void reconstruct<T>(int, int, const int*, const int*, const double*, const double*,    \
                                  const T*, T*, hipStream_t);                                        \

@@ -83,6 +83,7 @@ struct msw_ctx {
     void* mv2 = nullptr;              // [V]
     void* mc1 = nullptr;              // [C]
     void* mc2 = nullptr;              // [C]
+    void* mc3 = nullptr;              // [C]
     void* me1 = nullptr;              // [E]
     void* me2 = nullptr;              // [E]
     void* me3 = nullptr;              // [E]
@@ -298,6 +299,11 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
         dc[k] = d->dcEdge[o];
         dv[k] = d->dvEdge[o];
     }
+    std::vector<double> woeT(static_cast<size_t>(E) * kMaxEdges2, 0.0);
+    for (size_t k = 0; k < static_cast<size_t>(E); ++k)
+        for (int j = 0; j < kMaxEdges2; ++j)
+            woeT[kMaxEdges2 * k + j] =
+                woe[kMaxEdges2 * static_cast<size_t>(eoe[kMaxEdges2 * k + j]) + eoeT[kMaxEdges2 * k + j]];
     // ---- vertices
     std::vector<std::vector<int>> vertEdges(V);
     for (int e = 0; e < E; ++e) {
@@ -329,7 +335,7 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
     UP(m.nEoC, nEoC); UP(m.eoc, eoc); UP(m.sgnC, sgnC); UP(m.eocR, eocR); UP(m.sgnCR, sgnCR);
     UP(m.vocR, vocR); UP(m.kiteCR, kiteCR); UP(m.areaC, areaC); UP(m.hs, hs);
     UP(m.coe, coe); UP(m.voe, voe); UP(m.nEoE, nEoE); UP(m.eoe, eoe); UP(m.woe, woe);
-    UP(m.eoeT, eoeT); UP(m.dc, dc); UP(m.dv, dv);
+    UP(m.eoeT, eoeT); UP(m.woeT, woeT); UP(m.dc, dc); UP(m.dv, dv);
     UP(m.eovR, eovR); UP(m.sgnV, sgnV); UP(m.cov, cov); UP(m.kite, kite); UP(m.areaT, areaT);
     UP(m.fV, fV);
     UP(c->coeffs, coeffs); UP(c->trig, trig);
@@ -361,12 +367,12 @@ int alloc_buffers(msw_ctx* c)
     if ((rc = c->alloc(&c->dextra, V + 2 * C))) return rc;
     const size_t es = c->esz();
     char* mem = nullptr;
-    const size_t total = (3 * (C + E) + 2 * V + 2 * C + 3 * E) * B * es;
+    const size_t total = (3 * (C + E) + 2 * V + 3 * C + 3 * E) * B * es;
     if ((rc = c->alloc(&mem, total))) return rc;
     size_t off = 0;
     auto take = [&](void** p, size_t rows) { *p = mem + off; off += rows * B * es; };
     take(&c->macc, C + E); take(&c->mpA, C + E); take(&c->mpB, C + E);
-    take(&c->mv1, V); take(&c->mv2, V); take(&c->mc1, C); take(&c->mc2, C);
+    take(&c->mv1, V); take(&c->mv2, V); take(&c->mc1, C); take(&c->mc2, C); take(&c->mc3, C);
     take(&c->me1, E); take(&c->me2, E); take(&c->me3, E);
     c->dot_nblk = dot_blocks(std::max<long>(C, E));
     if ((rc = c->alloc(&c->dpartial, static_cast<size_t>(c->dot_nblk) * B + 2 * B))) return rc;
@@ -486,7 +492,8 @@ void enqueue_adj_step(msw_ctx* c, T* lamh, T* lamu, bool want_end)
     for (int s = 3; s >= 0; --s)
         adj_stage<T>(m, c->dcfg, c->lb, s, s == 3, k, c->st[s], g, g + C * B,
                      static_cast<T*>(c->me1), static_cast<T*>(c->me2), static_cast<T*>(c->me3),
-                     static_cast<T*>(c->mc1), static_cast<T*>(c->mc2), static_cast<T*>(c->mv1),
+                     static_cast<T*>(c->mc1), static_cast<T*>(c->mc2), static_cast<T*>(c->mc3),
+                     static_cast<T*>(c->mv1),
                      static_cast<T*>(c->mv2), lamh, lamu, acc, acc + C * B, c->stream);
 }
 
@@ -571,6 +578,11 @@ int msw_create(const msw_mesh_desc* mesh, const msw_config* cfg, int prec, int b
     c->nC = mesh->nCells;
     c->nE = mesh->nEdges;
     c->nV = mesh->nVertices;
+    // kernels index (entity, member) items with 32-bit arithmetic
+    const long max_rows = std::max<long>({static_cast<long>(c->nC), static_cast<long>(c->nE),
+                                          static_cast<long>(c->nV)});
+    if ((max_rows << c->lb) >= (1L << 31))
+        return fail(MSW_EINVAL, "mesh entities times batch must stay below 2^31");
     c->stream = static_cast<hipStream_t>(hip_stream);
     c->cfg = *cfg;
     c->dcfg.dt = cfg->dt;
