@@ -91,8 +91,14 @@ typedef struct msw_ctx msw_ctx;
 int msw_create(const msw_mesh_desc *mesh, const msw_config *cfg, int prec, int batch,
                void *hip_stream, msw_ctx **out);
 int msw_destroy(msw_ctx *ctx);
-/* Update dt/apvm/visc between calls (same mesh). */
+/* Update dt/apvm/visc/drag/wind between calls (same mesh). */
 int msw_set_config(msw_ctx *ctx, const msw_config *cfg);
+/* Replaces: the module array u_src (src/mpas_sw_module.f90:78,223-224), the
+ * wind-stress source of sw_compute_tend (src/mpas_sw_dyncore.f90:619-625).
+ * u_src: device FP64 [nEdges] in the internal edge order, copied on the
+ * creation stream; NULL resets it to zero (its value after var_allocation).
+ * Read only when config wind_stress is set. */
+int msw_set_forcing(msw_ctx *ctx, const double *u_src);
 /* Use a captured hipGraph per step for the step functions (default off). */
 int msw_set_graphs(msw_ctx *ctx, int enable);
 

@@ -13,6 +13,8 @@ constexpr int kMaxEdges = 6;
 constexpr int kMaxEdges2 = 12;
 constexpr int kVertexDegree = 3;
 constexpr double kGravity = 9.80616;   // src/mpas_sw_dyncore.f90:414
+constexpr double kRhoRef = 1000.0;     // src/mpas_sw_dyncore.f90:458
+constexpr double kDragCoef = 1.0e-3;   // bottom drag, src/mpas_sw_dyncore.f90:633
 
 // Device-resident mesh tables, entities in the library's internal order
 // (space-filling curve), 0-based.  "R" tables list neighbours in the order
@@ -57,6 +59,22 @@ struct DConfig {
     double dt;
     double apvm;
     int frozen_u;   // test case 1: wind field held fixed (src/mpas_sw_module.f90:596-598)
+    double visc2;   // config_h_mom_eddy_visc2 (del2 on u when > 0)
+    double visc4;   // config_h_mom_eddy_visc4 (del4 on u when > 0)
+    int wind;       // config_wind_stress
+    int drag;       // config_bottom_drag
+    const double* usrc;   // [E] wind-stress source u_src (internal order), FP64
+};
+
+// Fields the del2/del4 terms of tend_u read (src/mpas_sw_dyncore.f90:508-617):
+// divergence [C] and vorticity [V] of the stage state, and for del4 the
+// divergence [C] and vorticity [V] of nabla^2 u.  Null when not needed.
+template <typename T>
+struct Dissip {
+    const T* div;
+    const T* vort;
+    const T* dsd;
+    const T* dsv;
 };
 
 // Per-stage base (nonlinear) fields kept for the tangent-linear and adjoint
@@ -97,13 +115,31 @@ void fwd_diag(const DMesh& m, const DConfig& c, int lb, const T* h, const T* u,
 template <typename T>
 void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
                         const T* h, const T* u, const T* he, const T* pe, const T* ke,
-                        T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s);
+                        const Dissip<T>& dp, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
+                        hipStream_t s);
+
+// Scratch of the del2/del4 terms for one field set (members or base):
+// div [C], dsu [E], dsd [C], dsv [V]; vort [V] comes from the vertex pass.
+template <typename T>
+struct DissipScratch {
+    T* div;
+    T* dsu;
+    T* dsd;
+    T* dsv;
+};
+// divergence of u and, for del4, the nabla^2 u fields; returns what the
+// tendency kernels read (all null when no viscosity is on)
+template <typename T>
+Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, const T* vort,
+                        const DissipScratch<T>& ds, hipStream_t s);
 
 // Tangent-linear stage on top of base fields `b` of the same stage.
+// vort_tl [V] and ds are used only when a viscosity is on.
 template <typename T>
 void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
               const BaseStage& b, const T* h_tl, const T* u_tl,
-              T* pvv_tl, T* ke_tl, T* pc_tl, T* pe_tl, T* he_tl,
+              T* pvv_tl, T* ke_tl, T* pc_tl, T* pe_tl, T* he_tl, T* vort_tl,
+              const DissipScratch<T>& ds,
               T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s);
 
 // Adjoint of one RK4 stage: pa = F'(p_s)^T g, then the epilogue
@@ -112,7 +148,8 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
 template <typename T>
 void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, const RKCoeffs& k,
                const BaseStage& b, T* gh, T* gu, T* pe_ad, T* he_ad, T* ua_p, T* pc_ad, T* ha_p,
-               T* ke_ad, T* hv_ad, T* circ_ad, T* lamh, T* lamu, T* acch, T* accu, hipStream_t s);
+               T* ke_ad, T* hv_ad, T* circ_ad, const DissipScratch<T>& ds, T* lamh, T* lamu,
+               T* acch, T* accu, hipStream_t s);
 // g = w[3] * lam  (seed of the reverse sweep)
 template <typename T>
 void adj_seed(int n, T* g, const T* lam, double w3, hipStream_t s);

@@ -246,6 +246,125 @@ __device__ __forceinline__ void rk_update(int stage, T w, T sw, T tend, T* x, T*
     }
 }
 
+// (f(c2) - f(c1)) / dc - (g(v2) - g(v1)) / dv at edge e: the del2 operator
+// of src/mpas_sw_dyncore.f90:518-519 (and of del4 at :547-548, :597-600)
+template <typename T>
+__device__ __forceinline__ T grad_div_curl(const T* f, const T* g, int c1, int c2, int v1, int v2,
+                                           int B, int b, T dc, T dv)
+{
+    return (ld(f, c2, B, b) - ld(f, c1, B, b)) / dc - (ld(g, v2, B, b) - ld(g, v1, B, b)) / dv;
+}
+
+// Dissipation and forcing added to tend_u in the reference's order
+// (src/mpas_sw_dyncore.f90:508-637): + del2, - del4, + wind stress, - drag.
+// The mesh scaling factors are 1 (config_h_ScaleWithMesh cannot be set
+// through the reference's Python interface), so msd * visc * x == visc * x.
+template <typename T>
+__device__ __forceinline__ T tend_forcing(const DMesh& m, const DConfig& c, const Dissip<T>& dp,
+                                          int e, int c1, int c2, int B, int b, T tend, T ue, T hee,
+                                          T ke1, T ke2)
+{
+    const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
+    const T dc = T(m.dc[e]), dv = T(m.dv[e]);
+    if (c.visc2 > 0.0) tend = tend + T(c.visc2) * grad_div_curl(dp.div, dp.vort, c1, c2, v1, v2, B, b, dc, dv);
+    if (c.visc4 > 0.0) tend = tend - T(c.visc4) * grad_div_curl(dp.dsd, dp.dsv, c1, c2, v1, v2, B, b, dc, dv);
+    if (c.wind) tend = tend + T(c.usrc[e]) / T(kRhoRef) / hee;
+    if (c.drag) {
+        const T kee = T(0.5) * (ke1 + ke2);
+        tend = tend - T(kDragCoef) * ue * sqrt(T(2.0) * kee) / hee;
+    }
+    return tend;
+}
+
+// divergence (src/mpas_sw_dyncore.f90:221-242); the TLM applies the same
+// operator to u_tl (src/mpas_sw_dyncore_tlm.f90)
+template <typename T, bool WU>
+__global__ __launch_bounds__(kBlock) void k_div(DMesh m, int lb, const T* __restrict__ u,
+                                                T* __restrict__ div)
+{
+    const int B = 1 << lb;
+    ENTITY_ITEM(m.nC)
+    {
+        const int ic = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        int ie[kC6];
+        T ue[kC6];
+#pragma unroll
+        for (int k = 0; k < kC6; ++k) ie[k] = m.eocR[kC6 * ic + k];
+#pragma unroll
+        for (int k = 0; k < kC6; ++k) ue[k] = ld(u, ie[k], B, b);
+        T d = T(0);
+#pragma unroll
+        for (int k = 0; k < kC6; ++k) d = d + T(m.sgnCR[kC6 * ic + k]) * (ue[k] * T(m.dv[ie[k]]));
+        const T r = T(1.0) / T(m.areaC[ic]);
+        rowp(div, ic, B)[b] = d * r;
+    }
+}
+
+// nabla^2 u on edges (src/mpas_sw_dyncore.f90:538-550)
+template <typename T, bool WU>
+__global__ __launch_bounds__(kBlock) void k_del4_edge(DMesh m, int lb, const T* __restrict__ div,
+                                                      const T* __restrict__ vort,
+                                                      T* __restrict__ dsu)
+{
+    const int B = 1 << lb;
+    ENTITY_ITEM(m.nE)
+    {
+        const int e = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        rowp(dsu, e, B)[b] = grad_div_curl(div, vort, m.coe[2 * e], m.coe[2 * e + 1], m.voe[2 * e],
+                                           m.voe[2 * e + 1], B, b, T(m.dc[e]), T(m.dv[e]));
+    }
+}
+
+// vorticity of nabla^2 u (:552-570)
+template <typename T, bool WU>
+__global__ __launch_bounds__(kBlock) void k_del4_vertex(DMesh m, int lb, const T* __restrict__ dsu,
+                                                        T* __restrict__ dsv)
+{
+    const int B = 1 << lb;
+    ENTITY_ITEM(m.nV)
+    {
+        const int iv = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        int ie[3];
+        T x[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ie[i] = m.eovR[3 * iv + i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) x[i] = ld(dsu, ie[i], B, b);
+        T sc = T(0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sc = sc + T(m.sgnV[3 * iv + i]) * (T(m.dc[ie[i]]) * x[i]);
+        const T r = T(1.0) / T(m.areaT[iv]);
+        rowp(dsv, iv, B)[b] = sc * r;
+    }
+}
+
+// divergence of nabla^2 u (:572-589)
+template <typename T, bool WU>
+__global__ __launch_bounds__(kBlock) void k_del4_cell(DMesh m, int lb, const T* __restrict__ dsu,
+                                                      T* __restrict__ dsd)
+{
+    const int B = 1 << lb;
+    ENTITY_ITEM(m.nC)
+    {
+        const int ic = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        int ie[kC6];
+        T x[kC6];
+#pragma unroll
+        for (int k = 0; k < kC6; ++k) ie[k] = m.eocR[kC6 * ic + k];
+#pragma unroll
+        for (int k = 0; k < kC6; ++k) x[k] = ld(dsu, ie[k], B, b);
+        T d = T(0);
+#pragma unroll
+        for (int k = 0; k < kC6; ++k) d = d + T(m.sgnCR[kC6 * ic + k]) * (x[k] * T(m.dv[ie[k]]));
+        const T r = T(1.0) / T(m.areaC[ic]);
+        rowp(dsd, ic, B)[b] = d * r;
+    }
+}
+
 // tend_u (:485-505) on edges and tend_h (:468-482) on cells, then RK update
 template <typename T, bool WU>
 __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb, int stage,
@@ -253,8 +372,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
                                                      const T* __restrict__ u,
                                                      const T* __restrict__ he,
                                                      const T* __restrict__ pe,
-                                                     const T* __restrict__ ke, T* xh, T* xu,
-                                                     T* acch, T* accu, T* ph, T* pu)
+                                                     const T* __restrict__ ke, Dissip<T> dp,
+                                                     T* xh, T* xu, T* acch, T* accu, T* ph, T* pu)
 {
     const int B = 1 << lb;
     EDGE_CELL_ITEM(is_edge, t)
@@ -282,8 +401,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
                 const T workpv = T(0.5) * (pee + px[j]);
                 q = q + T(m.woe[kE2 * e + j]) * ux[j] * workpv * hx[j];
             }
-            const T tend = q - (k2 - k1 + T(kGravity) * (h2 + T(m.hs[c2]) - h1 - T(m.hs[c1]))) /
-                                   T(m.dc[e]);
+            T tend = q - (k2 - k1 + T(kGravity) * (h2 + T(m.hs[c2]) - h1 - T(m.hs[c1]))) /
+                             T(m.dc[e]);
+            if (c.visc2 > 0.0 || c.visc4 > 0.0 || c.wind || c.drag)
+                tend = tend_forcing(m, c, dp, e, c1, c2, B, b, tend, ld(u, e, B, b), ld(he, e, B, b),
+                                    k1, k2);
             rk_update(stage, w, sw, tend, xu, accu, pu, e, B, b,
                       c.frozen_u != 0);
         } else {
@@ -316,7 +438,8 @@ __global__ __launch_bounds__(kBlock) void k_tl_vertex(DMesh m, int lb, const dou
                                                       const double* __restrict__ vortb,
                                                       const T* __restrict__ h_tl,
                                                       const T* __restrict__ u_tl,
-                                                      T* __restrict__ pvv_tl)
+                                                      T* __restrict__ pvv_tl,
+                                                      T* __restrict__ vort_tl)
 {
     const int B = 1 << lb;
     ENTITY_ITEM(m.nV)
@@ -341,6 +464,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_vertex(DMesh m, int lb, const dou
         const T hv = T(hvb[iv]);
         rowp(pvv_tl, iv, B)[b] =
             vt / hv - ht * (T(m.fV[iv]) + T(vortb[iv])) / (hv * hv);
+        if (vort_tl) rowp(vort_tl, iv, B)[b] = vt;
     }
 }
 
@@ -418,6 +542,30 @@ __global__ __launch_bounds__(kBlock) void k_tl_edge(DMesh m, DConfig c, int lb, 
     }
 }
 
+// Tangent-linear dissipation and forcing (src/mpas_sw_dyncore_tlm.f90:596-768):
+// + del2, - del4 of the TL divergence/vorticity, TL wind stress and drag
+template <typename T>
+__device__ __forceinline__ T tend_forcing_tl(const DMesh& m, const DConfig& c, const Dissip<T>& dp,
+                                             const BaseStage& bs, int e, int c1, int c2, int B,
+                                             int b, T tend, T ut, T het, T ket1, T ket2)
+{
+    const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
+    const T dc = T(m.dc[e]), dv = T(m.dv[e]);
+    if (c.visc2 > 0.0) tend = tend + T(c.visc2) * grad_div_curl(dp.div, dp.vort, c1, c2, v1, v2, B, b, dc, dv);
+    if (c.visc4 > 0.0) tend = tend - T(c.visc4) * grad_div_curl(dp.dsd, dp.dsv, c1, c2, v1, v2, B, b, dc, dv);
+    const T he = T(bs.he[e]);
+    if (c.wind) tend = tend - het * T(c.usrc[e]) / T(kRhoRef) / (he * he);
+    if (c.drag) {
+        const T ke_edge_tl = T(0.5) * (ket1 + ket2);
+        const T ke_edge = T(0.5) * (T(bs.ke[c1]) + T(bs.ke[c2]));
+        const T ub = T(bs.u[e]);
+        tend = tend - T(kDragCoef) * ut * sqrt(T(2.0) * ke_edge) / he -
+               T(0.5) * T(kDragCoef) * ub * (T(2.0) * ke_edge_tl) / sqrt(T(2.0) * ke_edge) / he +
+               T(kDragCoef) * ub * sqrt(T(2.0) * ke_edge) * het / (he * he);
+    }
+    return tend;
+}
+
 // tend_u_tl (:564-595) and tend_h_tl (:541-561), then RK update
 template <typename T, bool WU>
 __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, int stage, T w,
@@ -426,8 +574,8 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
                                                     const T* __restrict__ u_tl,
                                                     const T* __restrict__ pe_tl,
                                                     const T* __restrict__ he_tl,
-                                                    const T* __restrict__ ke_tl, T* xh, T* xu,
-                                                    T* acch, T* accu, T* ph, T* pu)
+                                                    const T* __restrict__ ke_tl, Dissip<T> dp,
+                                                    T* xh, T* xu, T* acch, T* accu, T* ph, T* pu)
 {
     const int B = 1 << lb;
     EDGE_CELL_ITEM(is_edge, t)
@@ -461,7 +609,10 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
                 const T ubo = T(bs.u[eo]);
                 qt = qt + wgt * ux[j] * wpv * heb + wgt * ubo * wpvt * heb + wgt * ubo * wpv * hx[j];
             }
-            const T tend = qt - (k2 - k1 + T(kGravity) * (h2 - h1)) / T(m.dc[e]);
+            T tend = qt - (k2 - k1 + T(kGravity) * (h2 - h1)) / T(m.dc[e]);
+            if (c.visc2 > 0.0 || c.visc4 > 0.0 || c.wind || c.drag)
+                tend = tend_forcing_tl(m, c, dp, bs, e, c1, c2, B, b, tend, ld(u_tl, e, B, b),
+                                       ld(he_tl, e, B, b), k1, k2);
             rk_update(stage, w, sw, tend, xu, accu, pu, e, B, b,
                       c.frozen_u != 0);
         } else {
@@ -538,6 +689,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, DConfig c, int lb
             pad = pad + T(0.5) * (wt * ub * heb * gx[j]);
         }
         uad = uad - apvm * dt * T(bs.gpn[e]) * pad;
+        if (c.drag) {   // src/mpas_sw_dyncore_adj.f90:549-565
+            const T kee = T(0.5) * (T(bs.ke[c1]) + T(bs.ke[c2]));
+            const T sq = sqrt(T(2.0) * kee);
+            had = had + T(kDragCoef) * ub * sq / (heb * heb) * ge;
+            uad = uad - T(kDragCoef) * sq / heb * ge;
+        }
+        if (c.wind)     // :567-572
+            had = had - T(c.usrc[e]) / T(kRhoRef) / (heb * heb) * ge;
         rowp(pe_ad, e, B)[b] = pad;
         rowp(he_ad, e, B)[b] = had;
         rowp(ua_p, e, B)[b] = uad;
@@ -553,7 +712,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
                                                       const T* __restrict__ he_ad,
                                                       T* __restrict__ pc_ad,
                                                       T* __restrict__ ha_p,
-                                                      T* __restrict__ ke_ad)
+                                                      T* __restrict__ ke_ad,
+                                                      const T* __restrict__ dsu_ad,
+                                                      T* __restrict__ dva)
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
@@ -583,6 +744,30 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
             had = had + s * (T(kGravity) / dc * ge[i]) + mask * (T(0.5) * ha[i]);
             kad = kad + s * ge[i] / dc;
         }
+        if (c.drag) {   // ke_edge -> ke of both cells (src/mpas_sw_dyncore_adj.f90:555-562)
+#pragma unroll
+            for (int i = 0; i < kC6; ++i) {
+                const int e = ie[i];
+                const T mask = T(fabs(m.sgnC[kC6 * ic + i]));
+                const T kee = T(0.5) * (T(bs.ke[m.coe[2 * e]]) + T(bs.ke[m.coe[2 * e + 1]]));
+                const T kea = -T(0.5) * T(kDragCoef) * T(bs.u[e]) * T(2.0) / sqrt(T(2.0) * kee) /
+                              T(bs.he[e]) * ge[i];
+                kad = kad + mask * (T(0.5) * kea);
+            }
+        }
+        if (c.visc2 > 0.0 || c.visc4 > 0.0) {
+            // divergence adjoint of the del2/del4 terms (:579-584, :659-662,
+            // :681-682), returned scaled by 1/area for the u adjoint
+            T da = T(0);
+#pragma unroll
+            for (int i = 0; i < kC6; ++i) {
+                T x = T(0);
+                if (c.visc2 > 0.0) x = T(c.visc2) * ge[i];
+                if (c.visc4 > 0.0) x = x + ld(dsu_ad, ie[i], B, b);
+                da = da - T(m.sgnC[kC6 * ic + i]) * x / T(m.dc[ie[i]]);
+            }
+            rowp(dva, ic, B)[b] = da * (T(1.0) / T(m.areaC[ic]));
+        }
         rowp(pc_ad, ic, B)[b] = pcad;
         rowp(ha_p, ic, B)[b] = had;
         rowp(ke_ad, ic, B)[b] = kad / T(m.areaC[ic]);
@@ -597,7 +782,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
                                                        const T* __restrict__ pe_ad,
                                                        const T* __restrict__ pc_ad,
                                                        T* __restrict__ hv_ad,
-                                                       T* __restrict__ circ_ad)
+                                                       T* __restrict__ circ_ad,
+                                                       const T* __restrict__ gu,
+                                                       const T* __restrict__ dsu_ad)
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
@@ -624,9 +811,89 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
         const T at = T(m.areaT[iv]);
         T hva = -(T(m.fV[iv]) + T(bs.vort[iv])) / (hv * hv) * pa;
         hva = hva / at;
-        const T vora = pa / hv;
+        T vora = pa / hv;
+        if (c.visc2 > 0.0 || c.visc4 > 0.0) {
+            // vorticity adjoint of the del2/del4 terms (:664-665, :679-680)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                T x = T(0);
+                if (c.visc2 > 0.0) x = T(c.visc2) * ld(gu, ie[i], B, b);
+                if (c.visc4 > 0.0) x = x + ld(dsu_ad, ie[i], B, b);
+                vora = vora - T(m.sgnV[3 * iv + i]) * x / T(m.dv[ie[i]]);
+            }
+        }
         rowp(hv_ad, iv, B)[b] = hva;
         rowp(circ_ad, iv, B)[b] = vora / at;
+    }
+}
+
+// del4 adjoint, first half (src/mpas_sw_dyncore_adj.f90:588-616): from
+// tend_u_ad, the adjoints of the vorticity (times 1/areaTriangle: the
+// circulation adjoint) and of the divergence (times 1/areaCell) of nabla^2 u
+template <typename T, bool WU>
+__global__ __launch_bounds__(kBlock) void k_adj_del4_vertex(DMesh m, DConfig c, int lb,
+                                                            const T* __restrict__ gu,
+                                                            T* __restrict__ dsc_ad)
+{
+    const int B = 1 << lb;
+    ENTITY_ITEM(m.nV)
+    {
+        const int iv = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        int ie[3];
+        T g[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ie[i] = m.eovR[3 * iv + i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) g[i] = ld(gu, ie[i], B, b);
+        T a = T(0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) a = a + T(m.sgnV[3 * iv + i]) * (T(c.visc4) * g[i]) / T(m.dv[ie[i]]);
+        rowp(dsc_ad, iv, B)[b] = a * (T(1.0) / T(m.areaT[iv]));
+    }
+}
+
+template <typename T, bool WU>
+__global__ __launch_bounds__(kBlock) void k_adj_del4_cell(DMesh m, DConfig c, int lb,
+                                                          const T* __restrict__ gu,
+                                                          T* __restrict__ dsd_ad)
+{
+    const int B = 1 << lb;
+    ENTITY_ITEM(m.nC)
+    {
+        const int ic = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        int ie[kC6];
+        T g[kC6];
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) ie[i] = m.eoc[kC6 * ic + i];
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) g[i] = ld(gu, ie[i], B, b);
+        T a = T(0);
+#pragma unroll
+        for (int i = 0; i < kC6; ++i) a = a + T(m.sgnC[kC6 * ic + i]) * (T(c.visc4) * g[i]) / T(m.dc[ie[i]]);
+        rowp(dsd_ad, ic, B)[b] = a * (T(1.0) / T(m.areaC[ic]));
+    }
+}
+
+// del4 adjoint, second half (:618-652): nabla^2 u adjoint on edges
+template <typename T, bool WU>
+__global__ __launch_bounds__(kBlock) void k_adj_del4_edge(DMesh m, int lb,
+                                                          const T* __restrict__ dsd_ad,
+                                                          const T* __restrict__ dsc_ad,
+                                                          T* __restrict__ dsu_ad)
+{
+    const int B = 1 << lb;
+    ENTITY_ITEM(m.nE)
+    {
+        const int e = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+        const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
+        const T dv = T(m.dv[e]), dc = T(m.dc[e]);
+        T a = -dv * ld(dsd_ad, c2, B, b) + dv * ld(dsd_ad, c1, B, b);
+        a = a + dc * ld(dsc_ad, v2, B, b) - dc * ld(dsc_ad, v1, B, b);
+        rowp(dsu_ad, e, B)[b] = a;
     }
 }
 
@@ -657,7 +924,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
                                                       const T* __restrict__ ha_p,
                                                       const T* __restrict__ hv_ad,
                                                       const T* __restrict__ circ_ad,
-                                                      const T* __restrict__ ke_ad, T* lamh,
+                                                      const T* __restrict__ ke_ad,
+                                                      const T* __restrict__ dva, T* lamh,
                                                       T* lamu, T* acch, T* accu, T* gh, T* gu)
 {
     const int B = 1 << lb;
@@ -686,6 +954,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
             // ke(c) = sum 0.25*dc*dv*u^2 / area over the cell's edges (:245-256)
             const T kc = T(0.25) * T(m.dc[e]) * T(m.dv[e]) * T(2.0) * T(bs.u[e]);
             ua = ua + kc * k1 + kc * k2;
+            // divergence(c) = sum(+-u dv) / area (:221-242)
+            if (c.visc2 > 0.0 || c.visc4 > 0.0)
+                ua = ua + T(m.dv[e]) * (ld(dva, c1, B, b) - ld(dva, c2, B, b));
             adj_update(stage, first, w, sw, ua, lamu, accu, gu, e, B, b);
         } else {
             const int ic = entity_of<WU>(t, lb);
@@ -865,43 +1136,82 @@ void fwd_diag(const DMesh& m, const DConfig& c, int lb, const T* h, const T* u, 
 
 template <typename T>
 void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
-                        const T* h, const T* u, const T* he, const T* pe, const T* ke, T* xh,
-                        T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s)
+                        const T* h, const T* u, const T* he, const T* pe, const T* ke,
+                        const Dissip<T>& dp, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
+                        hipStream_t s)
 {
-    LAUNCH_WU(k_fwd_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), h, u,
-              he, pe, ke, xh, xu, acch, accu, ph, pu);
+    LAUNCH_WU(k_fwd_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, T(k.w[stage]),
+              T(k.sw[stage]), h, u, he, pe, ke, dp, xh, xu, acch, accu, ph, pu);
+}
+
+template <typename T>
+Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, const T* vort,
+                        const DissipScratch<T>& ds, hipStream_t s)
+{
+    Dissip<T> dp{nullptr, nullptr, nullptr, nullptr};
+    if (!(c.visc2 > 0.0 || c.visc4 > 0.0)) return dp;
+    LAUNCH_WU(k_div, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, u, ds.div);
+    dp.div = ds.div;
+    dp.vort = vort;
+    if (c.visc4 > 0.0) {
+        LAUNCH_WU(k_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, lb, ds.div,
+                  vort, ds.dsu);
+        LAUNCH_WU(k_del4_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, lb, ds.dsu,
+                  ds.dsv);
+        LAUNCH_WU(k_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, ds.dsu,
+                  ds.dsd);
+        dp.dsd = ds.dsd;
+        dp.dsv = ds.dsv;
+    }
+    return dp;
 }
 
 template <typename T>
 void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
               const BaseStage& b, const T* h_tl, const T* u_tl, T* pvv_tl, T* ke_tl, T* pc_tl,
-              T* pe_tl, T* he_tl, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s)
+              T* pe_tl, T* he_tl, T* vort_tl, const DissipScratch<T>& ds, T* xh, T* xu, T* acch,
+              T* accu, T* ph, T* pu, hipStream_t s)
 {
-    LAUNCH_WU(k_tl_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, lb, b.hv, b.vort, h_tl, u_tl,
-              pvv_tl);
-    LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, b.u, u_tl, pvv_tl, ke_tl,
-              pc_tl);
-    LAUNCH_WU(k_tl_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, b, h_tl, u_tl, pvv_tl,
-              pc_tl, pe_tl, he_tl);
-    LAUNCH_WU(k_tl_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), b, h_tl,
-              u_tl, pe_tl, he_tl, ke_tl, xh, xu, acch, accu, ph, pu);
+    const bool visc = c.visc2 > 0.0 || c.visc4 > 0.0;
+    LAUNCH_WU(k_tl_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, lb, b.hv, b.vort,
+              h_tl, u_tl, pvv_tl, visc ? vort_tl : nullptr);
+    LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, b.u, u_tl,
+              pvv_tl, ke_tl, pc_tl);
+    LAUNCH_WU(k_tl_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, b, h_tl,
+              u_tl, pvv_tl, pc_tl, pe_tl, he_tl);
+    const Dissip<T> dp = dissip_fields<T>(m, c, lb, u_tl, vort_tl, ds, s);
+    LAUNCH_WU(k_tl_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, T(k.w[stage]),
+              T(k.sw[stage]), b, h_tl, u_tl, pe_tl, he_tl, ke_tl, dp, xh, xu, acch, accu, ph, pu);
 }
 
 template <typename T>
 void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, const RKCoeffs& k,
                const BaseStage& b, T* gh, T* gu, T* pe_ad, T* he_ad, T* ua_p, T* pc_ad, T* ha_p,
-               T* ke_ad, T* hv_ad, T* circ_ad, T* lamh, T* lamu, T* acch, T* accu, hipStream_t s)
+               T* ke_ad, T* hv_ad, T* circ_ad, const DissipScratch<T>& ds, T* lamh, T* lamu,
+               T* acch, T* accu, hipStream_t s)
 {
-    LAUNCH_WU(k_adj_edge1, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, b, gh, gu, pe_ad, he_ad,
-              ua_p);
-    LAUNCH_WU(k_adj_cell1, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, c, lb, b, gu, pe_ad, he_ad,
-              pc_ad, ha_p, ke_ad);
-    LAUNCH_WU(k_adj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, c, lb, b, pe_ad, pc_ad, hv_ad,
-              circ_ad);
+    // ds.dsv: circulation adjoint of nabla^2 u [V]; ds.dsd: its divergence
+    // adjoint [C]; ds.dsu: nabla^2 u adjoint [E]; ds.div: divergence
+    // adjoint / area [C]
+    if (c.visc4 > 0.0) {
+        LAUNCH_WU(k_adj_del4_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, c, lb,
+                  gu, ds.dsv);
+        LAUNCH_WU(k_adj_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, c, lb, gu,
+                  ds.dsd);
+        LAUNCH_WU(k_adj_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, lb,
+                  ds.dsd, ds.dsv, ds.dsu);
+    }
+    LAUNCH_WU(k_adj_edge1, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, b, gh, gu,
+              pe_ad, he_ad, ua_p);
+    LAUNCH_WU(k_adj_cell1, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, c, lb, b, gu,
+              pe_ad, he_ad, pc_ad, ha_p, ke_ad, ds.dsu, ds.div);
+    LAUNCH_WU(k_adj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, c, lb, b, pe_ad,
+              pc_ad, hv_ad, circ_ad, gu, ds.dsu);
     const double w = stage > 0 ? k.w[stage - 1] : 0.0;
     const double sw = stage > 0 ? k.sw[stage - 1] : 0.0;
-    LAUNCH_WU(k_adj_final, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, first, T(w), T(sw), b, pe_ad, ua_p,
-              ha_p, hv_ad, circ_ad, ke_ad, lamh, lamu, acch, accu, gh, gu);
+    LAUNCH_WU(k_adj_final, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, first, T(w),
+              T(sw), b, pe_ad, ua_p, ha_p, hv_ad, circ_ad, ke_ad, ds.div, lamh, lamu, acch, accu, gh,
+              gu);
 }
 
 template <typename T>
@@ -975,14 +1285,16 @@ void nonfinite_flags(long n, int lb, const T* x, int* flags, hipStream_t s)
     template void fwd_diag<T>(const DMesh&, const DConfig&, int, const T*, const T*, T*, T*, T*, T*, \
                               T*, T*, T*, T*, T*, T*, hipStream_t);                                 \
     template void fwd_tend_update_he<T>(const DMesh&, const DConfig&, int, int, const RKCoeffs&,    \
-                                        const T*, const T*, const T*, const T*, const T*, T*, T*,   \
-                                        T*, T*, T*, T*, hipStream_t);                               \
+                                        const T*, const T*, const T*, const T*, const T*,           \
+                                        const Dissip<T>&, T*, T*, T*, T*, T*, T*, hipStream_t);     \
+    template Dissip<T> dissip_fields<T>(const DMesh&, const DConfig&, int, const T*, const T*,       \
+                                        const DissipScratch<T>&, hipStream_t);                      \
     template void tl_stage<T>(const DMesh&, const DConfig&, int, int, const RKCoeffs&,              \
-                              const BaseStage&, const T*, const T*, T*, T*, T*, T*, T*, T*, T*, T*, \
-                              T*, T*, T*, hipStream_t);                                             \
+                              const BaseStage&, const T*, const T*, T*, T*, T*, T*, T*, T*,         \
+                              const DissipScratch<T>&, T*, T*, T*, T*, T*, T*, hipStream_t);        \
     template void adj_stage<T>(const DMesh&, const DConfig&, int, int, bool, const RKCoeffs&,       \
-                               const BaseStage&, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, \
-                               T*, hipStream_t);                                                        \
+                               const BaseStage&, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*,            \
+                               const DissipScratch<T>&, T*, T*, T*, T*, hipStream_t);               \
     template void adj_seed<T>(int, T*, const T*, double, hipStream_t);                              \
     template void reconstruct<T>(int, int, const int*, const int*, const double*, const double*,    \
                                  const T*, T*, hipStream_t);                                        \

@@ -84,6 +84,17 @@ struct msw_ctx {
     void* mc1 = nullptr;              // [C]
     void* mc2 = nullptr;              // [C]
     void* mc3 = nullptr;              // [C]
+    // del2/del4 scratch, allocated when a viscosity is first switched on
+    void* xdiv = nullptr;             // [C]  member precision
+    void* xdsu = nullptr;             // [E]
+    void* xdsd = nullptr;             // [C]
+    void* xdsv = nullptr;             // [V]
+    void* xvort = nullptr;            // [V]
+    double* bdiv = nullptr;           // [C]  base (B = 1, FP64)
+    double* bdsu = nullptr;           // [E]
+    double* bdsd = nullptr;           // [C]
+    double* bdsv = nullptr;           // [V]
+    double* usrc = nullptr;           // [E]  wind-stress source, internal order
     void* me1 = nullptr;              // [E]
     void* me2 = nullptr;              // [E]
     void* me3 = nullptr;              // [E]
@@ -379,6 +390,55 @@ int alloc_buffers(msw_ctx* c)
     return MSW_OK;
 }
 
+// Scratch of the del2/del4 terms, made when a viscosity is first on.
+int ensure_dissip(msw_ctx* c)
+{
+    if (!(c->cfg.visc2 > 0.0 || c->cfg.visc4 > 0.0) || c->xdiv) return MSW_OK;
+    const size_t C = c->nC, E = c->nE, V = c->nV, B = c->B;
+    int rc;
+    char* mem = nullptr;
+    if ((rc = c->alloc(&mem, (2 * C + E + 2 * V) * B * c->esz()))) return rc;
+    size_t off = 0;
+    auto take = [&](void** p, size_t rows) { *p = mem + off; off += rows * B * c->esz(); };
+    take(&c->xdiv, C); take(&c->xdsu, E); take(&c->xdsd, C); take(&c->xdsv, V); take(&c->xvort, V);
+    double* bm = nullptr;
+    if ((rc = c->alloc(&bm, 2 * C + E + V))) return rc;
+    c->bdiv = bm; c->bdsu = bm + C; c->bdsd = bm + C + E; c->bdsv = bm + 2 * C + E;
+    return MSW_OK;
+}
+
+void set_dcfg(msw_ctx* c)
+{
+    const msw_config& g = c->cfg;
+    c->dcfg.dt = g.dt;
+    c->dcfg.apvm = g.apvm_upwinding;
+    c->dcfg.frozen_u = g.test_case == 1 ? 1 : 0;
+    c->dcfg.visc2 = g.visc2;
+    c->dcfg.visc4 = g.visc4;
+    c->dcfg.wind = g.wind_stress ? 1 : 0;
+    c->dcfg.drag = g.bottom_drag ? 1 : 0;
+    c->dcfg.usrc = c->usrc;
+}
+
+template <typename T>
+DissipScratch<T> member_ds(msw_ctx* c)
+{
+    return {static_cast<T*>(c->xdiv), static_cast<T*>(c->xdsu), static_cast<T*>(c->xdsd),
+            static_cast<T*>(c->xdsv)};
+}
+
+DissipScratch<double> base_ds(msw_ctx* c) { return {c->bdiv, c->bdsu, c->bdsd, c->bdsv}; }
+
+// base (B = 1) tendency + RK update of one stage, with the dissipation terms
+void base_tend(msw_ctx* c, const BaseStage& b, int s, const RKCoeffs& k, double* xh, double* xu,
+               double* nh)
+{
+    const Dissip<double> dp = dissip_fields<double>(c->dm, c->dcfg, 0, b.u, b.vort, base_ds(c), c->stream);
+    const size_t C = c->nC;
+    fwd_tend_update_he<double>(c->dm, c->dcfg, 0, s, k, b.h, b.u, b.he, b.pe, b.ke, dp, xh, xu,
+                               c->bacc, c->bacc + C, nh, nh + C, c->stream);
+}
+
 // ---------------------------------------------------------------------------
 // drivers
 
@@ -398,12 +458,14 @@ void enqueue_fwd_step(msw_ctx* c, T* h, T* u)
     T* pe = static_cast<T*>(c->me2);
     const T* rh = h;
     const T* ru = u;
+    T* vort = static_cast<T*>(c->xvort);   // null unless a viscosity is on
     for (int s = 0; s < 4; ++s) {
-        fwd_diag<T>(m, c->dcfg, c->lb, rh, ru, pvv, nullptr, nullptr, ke, pc, he, pe, nullptr,
+        fwd_diag<T>(m, c->dcfg, c->lb, rh, ru, pvv, nullptr, vort, ke, pc, he, pe, nullptr,
                     nullptr, nullptr, c->stream);
+        const Dissip<T> dp = dissip_fields<T>(m, c->dcfg, c->lb, ru, vort, member_ds<T>(c), c->stream);
         T* nb = (s % 2 == 0) ? pA : pB;
-        fwd_tend_update_he<T>(m, c->dcfg, c->lb, s, k, rh, ru, he, pe, ke, h, u, acc, acc + C * B, nb,
-                              nb + C * B, c->stream);
+        fwd_tend_update_he<T>(m, c->dcfg, c->lb, s, k, rh, ru, he, pe, ke, dp, h, u, acc,
+                              acc + C * B, nb, nb + C * B, c->stream);
         rh = nb;
         ru = nb + C * B;
     }
@@ -423,9 +485,9 @@ void enqueue_base_step(msw_ctx* c, double* h, double* u)
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
                          b.gpt, b.gpn, c->stream);
         double* nh = s < 3 ? c->st[s + 1].h : c->bend;
-        fwd_tend_update_he<double>(m, c->dcfg, 0, s, k, b.h, b.u, b.he, b.pe, b.ke, h, u, c->bacc,
-                                   c->bacc + C, nh, nh + C, c->stream);
+        base_tend(c, b, s, k, h, u, nh);
     }
+    (void)C;
 }
 
 template <typename T>
@@ -451,11 +513,11 @@ void enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
                          b.gpt, b.gpn, c->stream);
         T* nb = (s % 2 == 0) ? pA : pB;
-        tl_stage<T>(m, c->dcfg, c->lb, s, k, b, th, tu, pvv_tl, ke_tl, pc_tl, pe_tl, he_tl, htl, utl,
-                    acc, acc + C * B, nb, nb + C * B, c->stream);
+        tl_stage<T>(m, c->dcfg, c->lb, s, k, b, th, tu, pvv_tl, ke_tl, pc_tl, pe_tl, he_tl,
+                    static_cast<T*>(c->xvort), member_ds<T>(c), htl, utl, acc, acc + C * B, nb,
+                    nb + C * B, c->stream);
         double* nh = s < 3 ? c->st[s + 1].h : c->bend;
-        fwd_tend_update_he<double>(m, c->dcfg, 0, s, k, b.h, b.u, b.he, b.pe, b.ke, h, u, c->bacc,
-                                   c->bacc + C, nh, nh + C, c->stream);
+        base_tend(c, b, s, k, h, u, nh);
         th = nb;
         tu = nb + C * B;
     }
@@ -481,8 +543,7 @@ void enqueue_adj_step(msw_ctx* c, T* lamh, T* lamu, bool want_end)
             double* nh = s < 3 ? c->st[s + 1].h : c->bend;
             double* wxh = s < 3 ? xh : c->bend;
             double* wxu = s < 3 ? xu : c->bend + C;
-            fwd_tend_update_he<double>(m, c->dcfg, 0, s, k, b.h, b.u, b.he, b.pe, b.ke, wxh, wxu,
-                                       c->bacc, c->bacc + C, nh, nh + C, c->stream);
+            base_tend(c, b, s, k, wxh, wxu, nh);
         }
     }
     T* g = static_cast<T*>(c->mpA);
@@ -494,16 +555,14 @@ void enqueue_adj_step(msw_ctx* c, T* lamh, T* lamu, bool want_end)
                      static_cast<T*>(c->me1), static_cast<T*>(c->me2), static_cast<T*>(c->me3),
                      static_cast<T*>(c->mc1), static_cast<T*>(c->mc2), static_cast<T*>(c->mc3),
                      static_cast<T*>(c->mv1),
-                     static_cast<T*>(c->mv2), lamh, lamu, acc, acc + C * B, c->stream);
+                     static_cast<T*>(c->mv2), member_ds<T>(c), lamh, lamu, acc, acc + C * B,
+                     c->stream);
 }
 
 int check_cfg(msw_ctx* c)
 {
     const msw_config& g = c->cfg;
-    if (g.visc2 > 0.0 || g.visc4 > 0.0)
-        return fail(MSW_EUNSUPPORTED, "del2/del4 viscosity is not implemented on the GPU path");
-    if (g.wind_stress || g.bottom_drag)
-        return fail(MSW_EUNSUPPORTED, "wind stress / bottom drag are not implemented on the GPU path");
+    if (g.visc2 < 0.0 || g.visc4 < 0.0) return fail(MSW_EINVAL, "negative viscosity");
     if (!(g.dt > 0.0)) return fail(MSW_EINVAL, "dt must be positive");
     return MSW_OK;
 }
@@ -585,12 +644,13 @@ int msw_create(const msw_mesh_desc* mesh, const msw_config* cfg, int prec, int b
         return fail(MSW_EINVAL, "mesh entities times batch must stay below 2^31");
     c->stream = static_cast<hipStream_t>(hip_stream);
     c->cfg = *cfg;
-    c->dcfg.dt = cfg->dt;
-    c->dcfg.apvm = cfg->apvm_upwinding;
-    c->dcfg.frozen_u = cfg->test_case == 1 ? 1 : 0;
     int rc = build_mesh(c.get(), mesh);
     if (rc) return rc;
     if ((rc = alloc_buffers(c.get()))) return rc;
+    if ((rc = c->alloc(&c->usrc, c->nE))) return rc;
+    HIP_TRY(hipMemset(c->usrc, 0, static_cast<size_t>(c->nE) * sizeof(double)));
+    if ((rc = ensure_dissip(c.get()))) return rc;
+    set_dcfg(c.get());
     HIP_TRY(hipDeviceSynchronize());
     *out = c.release();
     return MSW_OK;
@@ -607,12 +667,24 @@ int msw_destroy(msw_ctx* ctx)
 int msw_set_config(msw_ctx* ctx, const msw_config* cfg)
 {
     if (!ctx || !cfg) return fail(MSW_EINVAL, "NULL argument");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->cfg = *cfg;
-    ctx->dcfg.dt = cfg->dt;
-    ctx->dcfg.apvm = cfg->apvm_upwinding;
-    ctx->dcfg.frozen_u = cfg->test_case == 1 ? 1 : 0;
+    int rc = ensure_dissip(ctx);
+    if (rc) return rc;
+    set_dcfg(ctx);
     for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     ctx->graphs.clear();
+    return MSW_OK;
+}
+
+int msw_set_forcing(msw_ctx* ctx, const double* u_src)
+{
+    if (!ctx) return fail(MSW_EINVAL, "NULL ctx");
+    if (u_src)
+        HIP_TRY(hipMemcpyAsync(ctx->usrc, u_src, static_cast<size_t>(ctx->nE) * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    else
+        HIP_TRY(hipMemsetAsync(ctx->usrc, 0, static_cast<size_t>(ctx->nE) * sizeof(double), ctx->stream));
     return MSW_OK;
 }
 

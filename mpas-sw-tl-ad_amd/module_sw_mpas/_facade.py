@@ -277,14 +277,20 @@ class _Module:
     def _model(self):
         """The GPU model for the current mesh and configuration."""
         v = self._vars
-        if v["config_h_mom_eddy_visc2"] > 0 or v["config_h_mom_eddy_visc4"] > 0:
-            raise NotImplementedError("del2/del4 viscosity is not available on the GPU path yet")
-        if v["config_wind_stress"] or v["config_bottom_drag"]:
-            raise NotImplementedError("wind stress / bottom drag are not available on the GPU path yet")
-        if v["config_thickness_adv_order"] in (3, 4) and np.any(self._arrays["deriv_two"] != 0):
+        A = self._arrays
+        visc = v["config_h_mom_eddy_visc2"] > 0 or v["config_h_mom_eddy_visc4"] > 0
+        if visc and (np.any(A["meshscalingdel2"] != 1.0) or np.any(A["meshscalingdel4"] != 1.0)):
+            # the GPU kernels take meshScalingDel2/4 == 1, the value
+            # compute_mesh_scaling gives unless config_h_ScaleWithMesh is set
+            raise NotImplementedError("viscosity with mesh scaling (config_h_scalewithmesh) on the GPU path")
+        if v["config_thickness_adv_order"] in (3, 4) and np.any(A["deriv_two"] != 0):
             raise NotImplementedError("thickness advection orders 3/4 with a non-zero deriv_two")
-        key = (v["config_dt"], v["config_apvm_upwinding"], v["config_test_case"])
+        key = (v["config_dt"], v["config_apvm_upwinding"], v["config_test_case"],
+               v["config_h_mom_eddy_visc2"], v["config_h_mom_eddy_visc4"],
+               bool(v["config_wind_stress"]), bool(v["config_bottom_drag"]))
         gpu = self._gpu
+        if gpu is not None and gpu["key"][2] != key[2]:
+            gpu = None      # another test case: other h_s/fVertex, frozen u
         if gpu is None:
             from mpas_sw_amd.core import SWModel
             A = self._arrays
@@ -293,12 +299,22 @@ class _Module:
                     "fVertex": A["fvertex"], "h_s": A["h_s"], "lonCell": A["loncell"],
                     "coeffs_reconstruct": A["coeffs_reconstruct"]}
             m = SWModel(self._mesh(), phys, v["config_dt"], apvm=v["config_apvm_upwinding"],
-                        test_case=v["config_test_case"])
+                        test_case=v["config_test_case"], visc2=v["config_h_mom_eddy_visc2"],
+                        visc4=v["config_h_mom_eddy_visc4"],
+                        wind_stress=bool(v["config_wind_stress"]),
+                        bottom_drag=bool(v["config_bottom_drag"]))
             gpu = {"model": m, "key": key}
             object.__setattr__(self, "_gpu", gpu)
         elif gpu["key"] != key:
-            gpu["model"].set_config(dt=v["config_dt"], apvm=v["config_apvm_upwinding"])
+            gpu["model"].set_config(dt=v["config_dt"], apvm=v["config_apvm_upwinding"],
+                                    visc2=v["config_h_mom_eddy_visc2"],
+                                    visc4=v["config_h_mom_eddy_visc4"],
+                                    wind_stress=bool(v["config_wind_stress"]),
+                                    bottom_drag=bool(v["config_bottom_drag"]))
             gpu["key"] = key
+        if v["config_wind_stress"]:
+            # the reference reads the module array u_src at every tendency call
+            gpu["model"].set_forcing(A["u_src"][0])
         return gpu["model"]
 
     def _up(self, m, arr, kind):

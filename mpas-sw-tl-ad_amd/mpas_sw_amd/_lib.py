@@ -72,7 +72,7 @@ class Config(C.Structure):
 
 # every symbol include/mpas_sw_amd.h declares
 EXPORTS = (
-    "msw_create", "msw_destroy", "msw_set_config", "msw_set_graphs", "msw_fwd_steps",
+    "msw_create", "msw_destroy", "msw_set_config", "msw_set_forcing", "msw_set_graphs", "msw_fwd_steps",
     "msw_base_steps", "msw_tlm_steps", "msw_adj_step", "msw_adj_steps", "msw_diagnostics",
     "msw_diag_names", "msw_diag_kinds", "msw_reconstruct", "msw_dot", "msw_nonfinite",
     "msw_permute", "msw_device_bytes", "msw_abi_version", "msw_last_error",
@@ -104,6 +104,7 @@ def load():
         "msw_create": ([C.POINTER(MeshDesc), C.POINTER(Config), i32, i32, vp, C.POINTER(vp)], i32),
         "msw_destroy": ([vp], i32),
         "msw_set_config": ([vp, C.POINTER(Config)], i32),
+        "msw_set_forcing": ([vp, vp], i32),
         "msw_set_graphs": ([vp, i32], i32),
         "msw_fwd_steps": ([vp, vp, vp, i32], i32),
         "msw_base_steps": ([vp, vp, vp, i32, vp, vp], i32),

@@ -45,13 +45,15 @@ class SWModel:
     sfc : reorder entities along a 3-D Hilbert curve (default) or keep the
         reference order.
     perm : explicit (cell, edge, vertex) permutations, new -> old (overrides sfc).
+    u_src : wind-stress source on edges (reference order), read when
+        wind_stress is set; zero by default as after var_allocation.
     """
 
     def __init__(self, mesh: Mesh, phys: dict, dt: float, *, apvm: float = 0.5, test_case: int = 2,
                  visc2: float = 0.0, visc4: float = 0.0, thickness_adv_order: int = 2,
                  monotonic: bool = False, wind_stress: bool = False, bottom_drag: bool = False,
                  batch: int = 1, precision: str = "fp64", sfc: bool = True, device=None,
-                 graphs: bool = False, perm=None):
+                 graphs: bool = False, perm=None, u_src=None):
         if precision not in _DT:
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.lib = _lib.load()
@@ -117,6 +119,8 @@ class SWModel:
         if graphs:
             self.set_graphs(True)
         self._inv = {k: np.argsort(v).astype(np.int64) for k, v in self.perm.items()}
+        if u_src is not None:
+            self.set_forcing(u_src)
 
     # ------------------------------------------------------------------ utils
     def close(self):
@@ -133,12 +137,30 @@ class SWModel:
     def set_graphs(self, on: bool):
         _lib.check(self.lib.msw_set_graphs(self._h, int(bool(on))))
 
-    def set_config(self, dt=None, apvm=None):
+    def set_config(self, dt=None, apvm=None, visc2=None, visc4=None, wind_stress=None,
+                   bottom_drag=None):
         if dt is not None:
             self.cfg.dt = dt
         if apvm is not None:
             self.cfg.apvm_upwinding = apvm
+        if visc2 is not None:
+            self.cfg.visc2 = visc2
+        if visc4 is not None:
+            self.cfg.visc4 = visc4
+        if wind_stress is not None:
+            self.cfg.wind_stress = int(bool(wind_stress))
+        if bottom_drag is not None:
+            self.cfg.bottom_drag = int(bool(bottom_drag))
         _lib.check(self.lib.msw_set_config(self._h, C.byref(self.cfg)))
+
+    def set_forcing(self, u_src):
+        """Wind-stress source u_src [E] (reference order); None resets it to 0."""
+        if u_src is None:
+            _lib.check(self.lib.msw_set_forcing(self._h, None))
+            return
+        t = self.to_internal(np.asarray(u_src, np.float64).reshape(-1), "e", member=False)
+        _lib.check(self.lib.msw_set_forcing(self._h, C.c_void_p(t.data_ptr())))
+        self.stream.synchronize()
 
     @property
     def device_bytes(self) -> int:

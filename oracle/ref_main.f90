@@ -19,7 +19,7 @@ program msw_ref
   real(kind=RKIND) :: cfg(8)
   integer :: has_state, has_tl, has_ad
   real(kind=RKIND), allocatable :: st_h(:), st_u(:), tl_h(:), tl_u(:), ad_h(:), ad_u(:)
-  real(kind=RKIND), allocatable :: traj_h(:,:), traj_u(:,:)
+  real(kind=RKIND), allocatable :: traj_h(:,:), traj_u(:,:), usrc_in(:)
   integer(8) :: t0, t1, rate
   integer, parameter :: uin = 21, uout = 22
 
@@ -75,6 +75,12 @@ program msw_ref
   if (has_state /= 0) read(uin) st_h, st_u
   if (has_tl /= 0) read(uin) tl_h, tl_u
   if (has_ad /= 0) read(uin) ad_h, ad_u
+  ! cfg(5) /= 0: a wind-stress source field u_src follows (module array,
+  ! src/mpas_sw_module.f90:78,223-224; zero unless the caller sets it)
+  if (cfg(5) /= 0.0) then
+     allocate(usrc_in(nEdges))
+     read(uin) usrc_in
+  end if
   close(uin)
 
   ! ---- initial conditions, as initial_conditions does
@@ -94,6 +100,7 @@ program msw_ref
      h(1,1,:) = st_h
      u(1,1,:) = st_u
   end if
+  if (allocated(usrc_in)) u_src(1,:) = usrc_in
   call var_allocation_tlm()
   call var_allocation_adj()
   call sw_mpas_init_block()

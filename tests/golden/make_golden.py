@@ -6,7 +6,7 @@ was made from (mesh digest, configuration, perturbations) and the reference's
 outputs.  The mesh itself is regenerated by mpas_sw_amd.mesh; its SHA-256 is
 stored so a test can tell when the generator changed.
 
-usage: python tests/golden/make_golden.py
+usage: python tests/golden/make_golden.py [fixture names]   (default: all)
 """
 
 import os
@@ -37,8 +37,42 @@ def save(name, mesh, cfg, **arrays):
     print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
 
 
-def main():
+def wind_source(mesh):
+    """A smooth zonal-ish stress field (N/m^2) on edges for the wind-stress
+    branch; the reference leaves u_src at zero unless the caller sets it."""
+    return 0.1 * np.cos(mesh.latEdge) * np.cos(mesh.angleEdge) + 0.02 * np.sin(3.0 * mesh.lonEdge)
+
+
+# del2/del4 viscosity, bottom drag and wind stress all switched on
+DISSIPATION = dict(visc2=2.0e5, visc4=1.0e15, bottom_drag=True, wind_stress=True)
+
+
+def dissipation_case():
+    """Level 3, TC5 (mountain), dt 900 s, 12 steps of forward, TLM and
+    adjoint with every dissipation/forcing branch of sw_compute_tend on."""
+    m3 = get_mesh(3)
+    tc, dt, n = 5, 900.0, 12
+    usrc = wind_source(m3)
+    kw = dict(test_case=tc, dt=dt, u_src=usrc, **DISSIPATION)
+    dh, du = perturb(m3, SEED + 3)
+    ic = ref.run(m3, "ic", **kw)
+    f = ref.run(m3, "fwd", nsteps=n, **kw)
+    t = ref.run(m3, "tlm", nsteps=n, tl=(dh, du), **kw)
+    a = ref.run(m3, "adj", nsteps=n, ad=(dh, du), **kw)
+    gh, gu = perturb(m3, SEED + 4, 1e-3, 1e-2)
+    tad = ref.run(m3, "tendadj", ad=(gh, gu), **kw)
+    cfg = dict(test_case=tc, dt=dt, nsteps=n, **DISSIPATION)
+    save("tc5_l3_dissipation", m3, cfg, h0=ic["h"], u0=ic["u"], u_src=usrc, dh=dh, du=du,
+         h_fwd=f["h"], u_fwd=f["u"], h=t["h"], u=t["u"], h_tl=t["h_tl"], u_tl=t["u_tl"],
+         h_ad=a["h_ad"], u_ad=a["u_ad"], gh=gh, gu=gu, tad_h_ad=tad["h_ad"], tad_u_ad=tad["u_ad"])
+
+
+def main(names):
     assert ref.available(), "build the reference first: make -C oracle ref"
+    if names:
+        for n in names:
+            {"tc5_l3_dissipation": dissipation_case}[n]()
+        return
 
     # 1. Williamson TC2, level 5, dt = 300 s, 288 steps (one day)
     m5 = get_mesh(5)
@@ -88,6 +122,9 @@ def main():
             arrays["kiteAreasOnVertex"] = ic["kiteAreasOnVertex"]
     save("l3_initial_conditions", m3, dict(), **arrays)
 
+    # 5. dissipation and forcing branches
+    dissipation_case()
+
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

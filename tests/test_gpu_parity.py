@@ -229,3 +229,107 @@ def test_packed_uneven_batch_equals_separate_runs():
         o.rk4(5)
         np.testing.assert_array_equal(H[oc:oc + mm.nCells], o["h1"])
         np.testing.assert_array_equal(U[oe:oe + mm.nEdges], o["u1"])
+
+
+# del2 + del4 viscosity, bottom drag and wind stress, as in the
+# tc5_l3_dissipation golden case that pins the oracle to the reference
+DISSIP = dict(visc2=2.0e5, visc4=1.0e15, wind_stress=True, bottom_drag=True)
+
+
+def _usrc(mesh):
+    return 0.1 * np.cos(mesh.latEdge) * np.cos(mesh.angleEdge) + 0.02 * np.sin(3.0 * mesh.lonEdge)
+
+
+def _dissip_oracle(dt):
+    from oracle.oracle import Oracle
+    from helpers import oracle_phys
+    mesh, phys = case(LEVEL, 5)
+    o = Oracle(mesh, oracle_phys(phys), dt, test_case=5, u_src=_usrc(mesh), **DISSIP)
+    o["h1"][:] = phys["h"]
+    o["u1"][:] = phys["u"]
+    o.init_block()
+    return o
+
+
+def _dissip_model(dt, **kw):
+    mesh, phys = case(LEVEL, 5)
+    return model(LEVEL, 5, dt, u_src=_usrc(mesh), **DISSIP, **kw)
+
+
+def test_dissipation_forward_and_tlm_bitwise():
+    print("dissipation fwd/tlm L3 tc5", flush=True)
+    dt, n = 900.0, 6
+    mesh, phys = case(LEVEL, 5)
+    o = _dissip_oracle(dt)
+    o.rk4(n)
+    m = _dissip_model(dt)
+    h = m.to_internal(phys["h"], "c")
+    u = m.to_internal(phys["u"], "e")
+    m.fwd(h, u, n)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(m.to_reference(h, "c"), o["h1"])
+    np.testing.assert_array_equal(m.to_reference(u, "e"), o["u1"])
+    dh, du = perturbations(mesh, 1)
+    o = _dissip_oracle(dt)
+    o["h_tl1"][:] = dh[:, 0]
+    o["u_tl1"][:] = du[:, 0]
+    o.rk4_tlm(n)
+    h = m.to_internal(phys["h"], "c", member=False)
+    u = m.to_internal(phys["u"], "e", member=False)
+    ht, ut = m.to_internal(dh[:, 0], "c"), m.to_internal(du[:, 0], "e")
+    m.tlm(h, u, ht, ut, n)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(m.to_reference(h, "c"), o["h1"])
+    np.testing.assert_array_equal(m.to_reference(ht, "c"), o["h_tl1"])
+    np.testing.assert_array_equal(m.to_reference(ut, "e"), o["u_tl1"])
+
+
+def test_dissipation_adjoint_matches_oracle():
+    print("dissipation adj L3 tc5", flush=True)
+    dt, n = 900.0, 4
+    mesh, phys = case(LEVEL, 5)
+    dh, du = perturbations(mesh, 1)
+    o = _dissip_oracle(dt)
+    traj = []
+    for _ in range(n):
+        traj.append((o["h1"].copy(), o["u1"].copy()))
+        o.rk4()
+    o["h_ad1"][:] = dh[:, 0]
+    o["u_ad1"][:] = du[:, 0]
+    for s in range(n - 1, -1, -1):
+        o["h1"][:] = traj[s][0]
+        o["u1"][:] = traj[s][1]
+        o.rk4_adj()
+    m = _dissip_model(dt)
+    h = m.to_internal(phys["h"], "c", member=False)
+    u = m.to_internal(phys["u"], "e", member=False)
+    th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+    tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+    m.base(h, u, n, th, tu)
+    ha, ua = m.to_internal(dh[:, 0], "c"), m.to_internal(du[:, 0], "e")
+    m.adj(th, tu, ha, ua, n)
+    torch.cuda.synchronize()
+    assert rel_l2(m.to_reference(ha, "c"), o["h_ad1"]) < 1e-11
+    assert rel_l2(m.to_reference(ua, "e"), o["u_ad1"]) < 1e-11
+
+
+def test_dissipation_dot_product_batched():
+    """<L x, y> = <x, L^T y> with every dissipation/forcing term on, 4 members."""
+    print("dissipation dot-product L3 tc5 B=4", flush=True)
+    dt, B = 900.0, 4
+    mesh, phys = case(LEVEL, 5)
+    xh, xu = perturbations(mesh, B, seed=3)
+    yh, yu = perturbations(mesh, B, seed=300)
+    m = _dissip_model(dt, batch=B)
+    h0 = m.to_internal(phys["h"], "c", member=False)
+    u0 = m.to_internal(phys["u"], "e", member=False)
+    Lxh, Lxu = m.to_internal(xh, "c"), m.to_internal(xu, "e")
+    h, u = h0.clone(), u0.clone()
+    m.tlm(h, u, Lxh, Lxu, 1)
+    LTyh, LTyu = m.to_internal(yh, "c"), m.to_internal(yu, "e")
+    yh_d, yu_d = LTyh.clone(), LTyu.clone()
+    m.adj_step(h0, u0, LTyh, LTyu)
+    xh_d, xu_d = m.to_internal(xh, "c"), m.to_internal(xu, "e")
+    lhs = m.dot(Lxh, Lxu, yh_d, yu_d).cpu().numpy()
+    rhs = m.dot(xh_d, xu_d, LTyh, LTyu).cpu().numpy()
+    assert np.all(np.abs(lhs - rhs) <= 1e-12 * np.abs(lhs))

@@ -80,3 +80,50 @@ def test_reconstruction_bitwise():
     o = make_oracle(3, 2, 600.0)
     np.testing.assert_array_equal(o["uReconstructZonal"], g["tc2_uReconstructZonal"])
     np.testing.assert_array_equal(o["uReconstructMeridional"], g["tc2_uReconstructMeridional"])
+
+
+def test_dissipation_and_forcing_bitwise():
+    """del2 + del4 viscosity, bottom drag and wind stress (all on): forward,
+    TLM and adjoint windows and the adjoint of one tendency evaluation."""
+    from oracle.oracle import Oracle
+    g, meta = golden("tc5_l3_dissipation")
+    n, dt = meta["nsteps"], meta["dt"]
+    mesh, phys = case(3, 5)
+    kw = dict(test_case=5, visc2=meta["visc2"], visc4=meta["visc4"],
+              wind_stress=meta["wind_stress"], bottom_drag=meta["bottom_drag"], u_src=g["u_src"])
+
+    def fresh():
+        o = Oracle(mesh, oracle_phys(phys), dt, **kw)
+        o["h1"][:] = g["h0"]
+        o["u1"][:] = g["u0"]
+        o.init_block()
+        return o
+
+    o = fresh()
+    o.rk4(n)
+    np.testing.assert_array_equal(o["h1"], g["h_fwd"])
+    np.testing.assert_array_equal(o["u1"], g["u_fwd"])
+    o = fresh()
+    o["h_tl1"][:] = g["dh"]
+    o["u_tl1"][:] = g["du"]
+    o.rk4_tlm(n)
+    np.testing.assert_array_equal(o["h_tl1"], g["h_tl"])
+    np.testing.assert_array_equal(o["u_tl1"], g["u_tl"])
+    np.testing.assert_array_equal(o["h1"], g["h"])
+    o = fresh()
+    traj = []
+    for _ in range(n):
+        traj.append((o["h1"].copy(), o["u1"].copy()))
+        o.rk4()
+    o["h_ad1"][:] = g["dh"]
+    o["u_ad1"][:] = g["du"]
+    for s in range(n - 1, -1, -1):
+        o["h1"][:] = traj[s][0]
+        o["u1"][:] = traj[s][1]
+        o.rk4_adj()
+    np.testing.assert_array_equal(o["h_ad1"], g["h_ad"])
+    np.testing.assert_array_equal(o["u_ad1"], g["u_ad"])
+    o = fresh()
+    ha, ua = o.tend_full_adj(g["h0"], g["u0"], g["gh"], g["gu"])
+    np.testing.assert_array_equal(ha, g["tad_h_ad"])
+    np.testing.assert_array_equal(ua, g["tad_u_ad"])
