@@ -13,9 +13,12 @@ steps in reverse (each recomputing its four nonlinear stage states from the
 stored step-start state).  value = C * B * K / T with T the wall time of both
 sweeps, inputs resident in HBM.
 
-Also reported: the plain unbatched FP64 path (B = 1, same kernels) and its
-ratio, the FP32-member variant, the algorithmic-bytes roofline, and the
-reference Fortran timed on this host's CPU (one core) as cpu_baseline.
+The TL tendency is formed from the fused TL thickness flux
+(MSW_OPT_TL_FUSED_FLUX; same TLM to rounding).  Also reported: the same
+batch in the reference's term order (bit-identical TLM), the plain unbatched
+FP64 path (B = 1, same kernels, reference order) and its ratio, the
+FP32-member variant, the algorithmic-bytes roofline, and the reference
+Fortran timed on this host's CPU (one core) as cpu_baseline.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -77,10 +80,11 @@ def run_window(m, bufs, h0, u0, ht0, ut0, nsteps):
     return wall, ev[0].elapsed_time(ev[1]) / 1e3, ev[1].elapsed_time(ev[2]) / 1e3
 
 
-def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True):
+def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True, fused=True):
     import torch
     from mpas_sw_amd.core import SWModel
-    m = SWModel(mesh, phys, DT, test_case=TC, batch=batch, precision=precision, graphs=graphs)
+    m = SWModel(mesh, phys, DT, test_case=TC, batch=batch, precision=precision, graphs=graphs,
+                tl_fused_flux=fused)
     h0 = m.to_internal(phys["h"], "c", member=False)
     u0 = m.to_internal(phys["u"], "e", member=False)
     sel = slice(0, batch)
@@ -165,9 +169,13 @@ def main():
     g = not args.no_graphs
     res = gpu_rate(mesh, phys, dh, du, BATCH, "fp64", args.steps, args.warmup, g)
     log(f"batched fp64: {res}")
+    exact = None
+    if not args.no_plain:
+        exact = gpu_rate(mesh, phys, dh, du, BATCH, "fp64", args.steps, args.warmup, g, fused=False)
+        log(f"batched fp64, reference term order: {exact}")
     plain = None
     if not args.no_plain:
-        plain = gpu_rate(mesh, phys, dh, du, 1, "fp64", args.steps, args.warmup, g)
+        plain = gpu_rate(mesh, phys, dh, du, 1, "fp64", args.steps, args.warmup, g, fused=False)
         log(f"plain fp64: {plain}")
     f32 = None
     if not args.no_fp32:
@@ -210,9 +218,12 @@ def main():
         "data": "synthetic (generated level-7 mesh, Williamson TC6, seeded perturbations)",
         "config": {"workload": "batched TLM+ADJ RK4, level-7 icosahedral Voronoi mesh, TC6, dt 75 s",
                    "cells": mesh.nCells, "edges": mesh.nEdges, "vertices": mesh.nVertices,
-                   "batch": BATCH, "dt": DT, "window_steps": K},
+                   "batch": BATCH, "dt": DT, "window_steps": K,
+                   "tl_tendency": "fused TL flux (MSW_OPT_TL_FUSED_FLUX)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "fp64_reference_term_order": None if exact is None else {
+            "value": exact["rate"], "ms_per_step": 1e3 * exact["seconds"] / K},
         "plain_fp64": None if plain is None else {"value": plain["rate"], "ms_per_step": 1e3 * plain["seconds"] / K},
         "batched_vs_plain": None if plain is None else res["rate"] / plain["rate"],
         "fp32": None if f32 is None else {

@@ -59,6 +59,8 @@ struct DConfig {
     double dt;
     double apvm;
     int frozen_u;   // test case 1: wind field held fixed (src/mpas_sw_module.f90:596-598)
+    int tl_fused;   // TL tendency from the fused TL thickness flux (fewer gathers,
+                    // sums regrouped); 0 keeps the reference's term order
     double visc2;   // config_h_mom_eddy_visc2 (del2 on u when > 0)
     double visc4;   // config_h_mom_eddy_visc4 (del4 on u when > 0)
     int wind;       // config_wind_stress
@@ -92,6 +94,22 @@ struct BaseStage {
     double* pvv;    // [V]  pv_vertex
     double* hv;     // [V]  h_vertex
     double* vort;   // [V]  relative vorticity
+    double* rec;    // [E][4] pv_edge, u, h_edge, gradPVt packed (32-byte
+                    //        aligned) for gathers over edgesOnEdge
+};
+
+// Coefficients of a TL or adjoint stage operator, formed once per stage
+// from the base fields (B = 1, FP64) and shared by every member.  Each
+// member kernel then reads contiguous coefficient rows instead of gathering
+// base values through the connectivity.  The TL stage uses e48 and c32 with
+// its own layout (k_tl_coef); TL and adjoint steps never overlap.
+struct AdjCoef {
+    double* e48;   // adjoint [E][24]: woeT*workpv, woeT*(-apvm*dt*gradPVt(eo)) per
+                   //   edgesOnEdge slot; the TL stage uses [E][48]
+    double* e8;    // [E][16]: u*dv, dv*he, sum_j 0.5*w*u*he, apvm*dt*gradPVn, drag/wind
+                   //   h_edge and u terms, ke coefficient, u, he, 0.5*u*he, 0...
+    double* v8;    // [V][8]:  kite/areaCell[3], pv_edge terms[3], h_vertex, circulation
+    double* c32;   // [C][32]: pv_cell[6], gravity[6], ke[6], h_edge[6], 1/areaCell, 0...
 };
 
 // RK4 coefficients of src/mpas_sw_module.f90:567-575.
@@ -111,7 +129,7 @@ RKCoeffs rk_coeffs(double dt);
 template <typename T>
 void fwd_diag(const DMesh& m, const DConfig& c, int lb, const T* h, const T* u,
               T* pvv, T* hv, T* vort, T* ke, T* pc, T* he, T* pe, T* v, T* gpt, T* gpn,
-              hipStream_t s);
+              double* rec, hipStream_t s);
 template <typename T>
 void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
                         const T* h, const T* u, const T* he, const T* pe, const T* ke,
@@ -139,7 +157,7 @@ template <typename T>
 void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
               const BaseStage& b, const T* h_tl, const T* u_tl,
               T* pvv_tl, T* ke_tl, T* pc_tl, T* pe_tl, T* he_tl, T* vort_tl,
-              const DissipScratch<T>& ds,
+              const DissipScratch<T>& ds, const AdjCoef& q,
               T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s);
 
 // Adjoint of one RK4 stage: pa = F'(p_s)^T g, then the epilogue
@@ -148,8 +166,8 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
 template <typename T>
 void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, const RKCoeffs& k,
                const BaseStage& b, T* gh, T* gu, T* pe_ad, T* he_ad, T* ua_p, T* pc_ad, T* ha_p,
-               T* ke_ad, T* hv_ad, T* circ_ad, const DissipScratch<T>& ds, T* lamh, T* lamu,
-               T* acch, T* accu, hipStream_t s);
+               T* ke_ad, T* hv_ad, T* circ_ad, const DissipScratch<T>& ds, const AdjCoef& q,
+               T* lamh, T* lamu, T* acch, T* accu, hipStream_t s);
 // g = w[3] * lam  (seed of the reverse sweep)
 template <typename T>
 void adj_seed(int n, T* g, const T* lam, double w3, hipStream_t s);

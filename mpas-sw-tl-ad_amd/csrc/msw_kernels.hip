@@ -93,6 +93,11 @@ inline int grid_for(long threads)
     const unsigned t = (is_edge ? x_ * pE_ + j_ : x_ * pC_ + (j_ - pE_)) * kBlock + threadIdx.x; \
     if (t >= (is_edge ? tE_ : tC_)) return;
 
+// one flat item per thread (no member axis), blocks dealt XCD-aware
+#define FLAT_ITEM(count)                                                       \
+    const unsigned t = xcd_block() * kBlock + threadIdx.x;                     \
+    if (t >= static_cast<unsigned>(count)) return;
+
 #define GRID_LOOP(total)                                                               \
     for (long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x; t < (total); \
          t += static_cast<long>(gridDim.x) * kBlock)
@@ -189,7 +194,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_edge(DMesh m, DConfig c, int lb,
                                                      const T* __restrict__ pc, T* __restrict__ he,
                                                      T* __restrict__ pe, T* __restrict__ vout,
                                                      T* __restrict__ gpt_out,
-                                                     T* __restrict__ gpn_out)
+                                                     T* __restrict__ gpn_out,
+                                                     double* __restrict__ rec)
 {
     const int B = 1 << lb;
     const T apvm = T(c.apvm), dt = T(c.dt);
@@ -222,6 +228,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_edge(DMesh m, DConfig c, int lb,
         if (vout) rowp(vout, e, B)[b] = vv;
         if (gpt_out) rowp(gpt_out, e, B)[b] = gt;
         if (gpn_out) rowp(gpn_out, e, B)[b] = gn;
+        if (rec) {   // base path (B = 1): one record per edge for neighbour gathers
+            double2* r = reinterpret_cast<double2*>(rec + 4L * e);
+            r[0] = make_double2(static_cast<double>(p), static_cast<double>(ue));
+            r[1] = make_double2(static_cast<double>(T(0.5) * (h1 + h2)), static_cast<double>(gt));
+        }
     }
 }
 
@@ -470,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_vertex(DMesh m, int lb, const dou
 
 // :301-316, :401-421
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_tl_cell(DMesh m, int lb, const double* __restrict__ ub,
+__global__ __launch_bounds__(kBlock) void k_tl_cell(DMesh m, int lb, const double* __restrict__ c32,
                                                     const T* __restrict__ u_tl,
                                                     const T* __restrict__ pvv_tl,
                                                     T* __restrict__ ke_tl, T* __restrict__ pc_tl)
@@ -486,12 +497,14 @@ __global__ __launch_bounds__(kBlock) void k_tl_cell(DMesh m, int lb, const doubl
         for (int i = 0; i < kC6; ++i) { ie[i] = m.eoc[kC6 * ic + i]; iv[i] = m.vocR[kC6 * ic + i]; }
 #pragma unroll
         for (int i = 0; i < kC6; ++i) { ue[i] = ld(u_tl, ie[i], B, b); pv[i] = ld(pvv_tl, iv[i], B, b); }
+        const double* r = c32 + 32L * ic;
         const T area = T(m.areaC[ic]);
         T k = T(0);
 #pragma unroll
         for (int i = 0; i < kC6; ++i) {
+            // 0.25*dc*dv*(2*u*u_tl) on real edges
             const T mask = T(fabs(m.sgnC[kC6 * ic + i]));
-            k = k + mask * (T(0.25) * T(m.dc[ie[i]]) * T(m.dv[ie[i]]) * (T(2.0) * T(ub[ie[i]]) * ue[i]));
+            k = k + mask * (T(r[24 + i]) * (T(r[12 + i]) * ue[i]));
         }
         k = k / area;
         T p = T(0);
@@ -538,7 +551,60 @@ __global__ __launch_bounds__(kBlock) void k_tl_edge(DMesh m, DConfig c, int lb, 
         p = p - apvm * vt * dt * T(bs.gpt[e]) - apvm * T(bs.v[e]) * dt * gtt;
         p = p - apvm * ue * dt * T(bs.gpn[e]) - apvm * T(bs.u[e]) * dt * gnt;
         rowp(pe_tl, e, B)[b] = p;
-        rowp(he_tl, e, B)[b] = T(0.5) * (h1 + h2);
+        const T het = T(0.5) * (h1 + h2);
+        // fused form: the TL thickness flux u_tl*h_edge + u*h_edge_tl (per dvEdge)
+        rowp(he_tl, e, B)[b] = c.tl_fused ? ue * T(bs.he[e]) + T(bs.u[e]) * het : het;
+    }
+}
+
+// Base-state coefficients of the TL stage (B = 1, FP64), in the products
+// the reference forms, so the member kernels read contiguous rows and keep
+// the reference's rounding:
+//   e48[e] = 0.5*(pv_edge(e)+pv_edge(eo)), h_edge(eo), w*u(eo), (w*u(eo))*workpv
+//            for the 12 edgesOnEdge slots (src/mpas_sw_dyncore_tlm.f90:564-595)
+//   c32[c] = h_edge, u*dvEdge, dvEdge over the reference-ordered edges of the
+//            cell (:541-561); 2*u and 0.25*dcEdge*dvEdge over its edge slots (:301-316)
+__global__ __launch_bounds__(kBlock) void k_tl_coef(DMesh m, BaseStage bs, int fused,
+                                                    double* __restrict__ e48,
+                                                    double* __restrict__ c32)
+{
+    // 16 lanes per edge (slot j = lane), then 8 lanes per cell (slot k)
+    const unsigned ne = 16u * static_cast<unsigned>(m.nE);
+    FLAT_ITEM(ne + 8u * static_cast<unsigned>(m.nC))
+    {
+        if (t < ne) {
+            const int e = static_cast<int>(t >> 4), j = static_cast<int>(t & 15);
+            if (j >= kE2) return;
+            const int eo = m.eoe[kE2 * e + j];
+            const double4 ro = *reinterpret_cast<const double4*>(bs.rec + 4L * eo);  // pe u he gpt
+            const double wpv = 0.5 * (bs.pe[e] + ro.x);
+            const double k2 = m.woe[kE2 * e + j] * ro.y;
+            double* r = e48 + 48L * e;
+            if (fused) {
+                // q_tl = sum_j w*workpv*F_tl(eo) + 0.5*w*u(eo)*he(eo)*(pv_tl(e)+pv_tl(eo))
+                r[j] = m.woe[kE2 * e + j] * wpv;
+                r[12 + j] = 0.5 * (k2 * ro.z);
+            } else {
+                r[j] = wpv;
+                r[12 + j] = ro.z;
+                r[24 + j] = k2;
+                r[36 + j] = k2 * wpv;
+            }
+        } else {
+            const unsigned u = t - ne;
+            const int ic = static_cast<int>(u >> 3), k = static_cast<int>(u & 7);
+            double* r = c32 + 32L * ic;
+            if (k < kC6) {
+                const int e = m.eocR[kC6 * ic + k], es = m.eoc[kC6 * ic + k];
+                r[k] = bs.he[e];
+                r[6 + k] = bs.u[e] * m.dv[e];
+                r[12 + k] = 2.0 * bs.u[es];
+                r[18 + k] = m.dv[e];
+                r[24 + k] = 0.25 * m.dc[es] * m.dv[es];
+            } else {
+                r[24 + k] = 0.0;   // slots 30, 31
+            }
+        }
     }
 }
 
@@ -569,7 +635,8 @@ __device__ __forceinline__ T tend_forcing_tl(const DMesh& m, const DConfig& c, c
 // tend_u_tl (:564-595) and tend_h_tl (:541-561), then RK update
 template <typename T, bool WU>
 __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, int stage, T w,
-                                                    T sw, BaseStage bs,
+                                                    T sw, BaseStage bs, const double* __restrict__ e48,
+                                                    const double* __restrict__ c32,
                                                     const T* __restrict__ h_tl,
                                                     const T* __restrict__ u_tl,
                                                     const T* __restrict__ pe_tl,
@@ -585,50 +652,69 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
             const int e = entity_of<WU>(t, lb);
             const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
             int ix[kE2];
-            T ux[kE2], px[kE2], hx[kE2];
 #pragma unroll
             for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
-#pragma unroll
-            for (int j = 0; j < kE2; ++j) {
-                ux[j] = ld(u_tl, ix[j], B, b);
-                px[j] = ld(pe_tl, ix[j], B, b);
-                hx[j] = ld(he_tl, ix[j], B, b);
-            }
             const T pte = ld(pe_tl, e, B, b);
             const T k1 = ld(ke_tl, c1, B, b), k2 = ld(ke_tl, c2, B, b);
             const T h1 = ld(h_tl, c1, B, b), h2 = ld(h_tl, c2, B, b);
-            const T pbe = T(bs.pe[e]);
+            const double* r = e48 + 48L * e;
             T qt = T(0);
+            if (c.tl_fused) {
+                // he_tl holds the TL flux F_tl = u_tl*he + u*he_tl
+                T fx[kE2], px[kE2];
 #pragma unroll
-            for (int j = 0; j < kE2; ++j) {
-                const int eo = ix[j];
-                const T wgt = T(m.woe[kE2 * e + j]);
-                const T wpvt = T(0.5) * (pte + px[j]);
-                const T wpv = T(0.5) * (pbe + T(bs.pe[eo]));
-                const T heb = T(bs.he[eo]);
-                const T ubo = T(bs.u[eo]);
-                qt = qt + wgt * ux[j] * wpv * heb + wgt * ubo * wpvt * heb + wgt * ubo * wpv * hx[j];
+                for (int j = 0; j < kE2; ++j) {
+                    fx[j] = ld(he_tl, ix[j], B, b);
+                    px[j] = ld(pe_tl, ix[j], B, b);
+                }
+#pragma unroll
+                for (int j = 0; j < kE2; ++j) qt = qt + T(r[j]) * fx[j] + T(r[12 + j]) * (pte + px[j]);
+            } else {
+                T ux[kE2], px[kE2], hx[kE2];
+#pragma unroll
+                for (int j = 0; j < kE2; ++j) {
+                    ux[j] = ld(u_tl, ix[j], B, b);
+                    px[j] = ld(pe_tl, ix[j], B, b);
+                    hx[j] = ld(he_tl, ix[j], B, b);
+                }
+#pragma unroll
+                for (int j = 0; j < kE2; ++j) {
+                    // w*u_tl*workpv*he + w*u*workpv_tl*he + w*u*workpv*h_edge_tl
+                    const T wgt = T(m.woe[kE2 * e + j]);
+                    const T wpv = T(r[j]), heb = T(r[12 + j]);
+                    const T wpvt = T(0.5) * (pte + px[j]);
+                    qt = qt + wgt * ux[j] * wpv * heb + T(r[24 + j]) * wpvt * heb + T(r[36 + j]) * hx[j];
+                }
             }
             T tend = qt - (k2 - k1 + T(kGravity) * (h2 - h1)) / T(m.dc[e]);
             if (c.visc2 > 0.0 || c.visc4 > 0.0 || c.wind || c.drag)
                 tend = tend_forcing_tl(m, c, dp, bs, e, c1, c2, B, b, tend, ld(u_tl, e, B, b),
-                                       ld(he_tl, e, B, b), k1, k2);
+                                       c.tl_fused ? T(0.5) * (h1 + h2) : ld(he_tl, e, B, b), k1, k2);
             rk_update(stage, w, sw, tend, xu, accu, pu, e, B, b,
                       c.frozen_u != 0);
         } else {
             const int ic = entity_of<WU>(t, lb);
             int ie[kC6];
-            T ue[kC6], hh[kC6];
 #pragma unroll
             for (int k = 0; k < kC6; ++k) ie[k] = m.eocR[kC6 * ic + k];
-#pragma unroll
-            for (int k = 0; k < kC6; ++k) { ue[k] = ld(u_tl, ie[k], B, b); hh[k] = ld(he_tl, ie[k], B, b); }
+            const double* r = c32 + 32L * ic;
             T th = T(0);
+            if (c.tl_fused) {
+                T fx[kC6];
 #pragma unroll
-            for (int k = 0; k < kC6; ++k) {
-                const int e = ie[k];
-                const T flux = ue[k] * T(m.dv[e]) * T(bs.he[e]) + T(bs.u[e]) * T(m.dv[e]) * hh[k];
-                th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
+                for (int k = 0; k < kC6; ++k) fx[k] = ld(he_tl, ie[k], B, b);
+#pragma unroll
+                for (int k = 0; k < kC6; ++k) th = th + T(-m.sgnCR[kC6 * ic + k]) * (T(r[18 + k]) * fx[k]);
+            } else {
+                T ue[kC6], hh[kC6];
+#pragma unroll
+                for (int k = 0; k < kC6; ++k) { ue[k] = ld(u_tl, ie[k], B, b); hh[k] = ld(he_tl, ie[k], B, b); }
+#pragma unroll
+                for (int k = 0; k < kC6; ++k) {
+                    // u_tl*dv*h_edge + u*dv*h_edge_tl
+                    const T flux = ue[k] * T(r[18 + k]) * T(r[k]) + T(r[6 + k]) * hh[k];
+                    th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
+                }
             }
             th = th / T(m.areaC[ic]);
             rk_update(stage, w, sw, th, xh, acch, ph, ic, B, b, false);
@@ -642,11 +728,121 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
 // reference becomes a sum at the receiving entity, so no atomics are needed
 // and results are deterministic.
 
-// tend_u/q adjoint (:689-717), tend_h adjoint (:720-736), upwinding in the
-// normal direction (:193-200).  The kinetic-energy part of tend_u's
-// adjoint is a per-cell sum and is formed in k_adj_cell1.
+// ---- adjoint stage coefficients (B = 1, one thread per entity)
+
+// edges: tend_u/q adjoint (src/mpas_sw_dyncore_adj.f90:689-717), tend_h
+// adjoint (:720-736), normal upwinding (:193-200), drag and wind (:549-572),
+// v -> u through the transposed edgesOnEdge (:296-304), ke -> u (:306-318)
+__global__ __launch_bounds__(kBlock) void k_adj_coef_edge(DMesh m, DConfig c, BaseStage bs,
+                                                          double* __restrict__ e24,
+                                                          double* __restrict__ e16)
+{
+    // 16 lanes per edge (all of them active through the reduction below):
+    // lanes 0-11 fill the edgesOnEdge slots, lanes 12-15 the per-edge values
+    const unsigned t = xcd_block() * kBlock + threadIdx.x;
+    const unsigned total = 16u * static_cast<unsigned>(m.nE);
+    if (t >= total) return;   // whole 16-lane groups only, so the shuffles stay inside live lanes
+    const int e = static_cast<int>(t >> 4), j = static_cast<int>(t & 15);
+    double term = 0.0;   // 0.5*w*u(eo)*he(eo), summed over the slots
+    if (j < kE2) {
+        const int eo = m.eoe[kE2 * e + j];
+        const double4 ro = *reinterpret_cast<const double4*>(bs.rec + 4L * eo);  // pe u he gpt
+        const double wt = m.woeT[kE2 * e + j];
+        double* r = e24 + 24L * e;
+        r[j] = wt * (0.5 * (bs.pe[e] + ro.x));
+        r[12 + j] = wt * (-c.apvm * c.dt * ro.w);
+        term = 0.5 * (m.woe[kE2 * e + j] * ro.y * ro.z);
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) term += __shfl_xor(term, off, 16);
+    if (j >= kE2) {
+        const double ub = bs.u[e], heb = bs.he[e], dv = m.dv[e];
+        double* q = e16 + 16L * e;
+        const int k = j - kE2;
+        if (k == 0) {
+            q[0] = ub * dv;
+            q[1] = dv * heb;
+            q[7] = ub;
+            q[8] = heb;
+        } else if (k == 1) {
+            q[2] = term;
+            q[3] = c.apvm * c.dt * bs.gpn[e];
+            q[9] = 0.5 * (ub * heb);
+        } else if (k == 2) {
+            double hge = 0.0, uge = 0.0;
+            if (c.drag) {
+                const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+                const double sq = sqrt(2.0 * (0.5 * (bs.ke[c1] + bs.ke[c2])));
+                hge = kDragCoef * ub * sq / (heb * heb);
+                uge = -kDragCoef * sq / heb;
+            }
+            if (c.wind) hge = hge - c.usrc[e] / kRhoRef / (heb * heb);
+            q[4] = hge;
+            q[5] = uge;
+        } else {
+            q[6] = 0.25 * m.dc[e] * dv * 2.0 * ub;
+            for (int i = 10; i < 16; ++i) q[i] = 0.0;
+        }
+    }
+}
+
+// vertices: pv_cell -> pv_vertex (:215-229), pv_edge and gradPVt ->
+// pv_vertex (:235-262), pv_vertex -> h_vertex, vorticity (:268-284)
+__global__ __launch_bounds__(kBlock) void k_adj_coef_vertex(DMesh m, DConfig c, BaseStage bs,
+                                                            double* __restrict__ v8)
+{
+    FLAT_ITEM(8u * static_cast<unsigned>(m.nV))
+    {
+        const int iv = static_cast<int>(t / 8), k = static_cast<int>(t % 8);
+        double v;
+        if (k < 3) {
+            v = m.kite[3 * iv + k] / m.areaC[m.cov[3 * iv + k]];
+        } else if (k < 6) {
+            const int e = m.eovR[3 * iv + k - 3];
+            v = 0.5 + m.sgnV[3 * iv + k - 3] * (-c.apvm * bs.v[e] * c.dt) / m.dv[e];
+        } else {
+            const double hv = bs.hv[iv], at = m.areaT[iv];
+            v = k == 6 ? -(m.fV[iv] + bs.vort[iv]) / (hv * hv) / at : 1.0 / hv / at;
+        }
+        v8[t] = v;
+    }
+}
+
+// cells: gradPVn -> pv_cell (:202-213), gravity -> h (:696-697), ke
+// gradient and drag -> ke (:555-562, :693-695), h_edge -> h (:358-369)
+__global__ __launch_bounds__(kBlock) void k_adj_coef_cell(DMesh m, DConfig c, BaseStage bs,
+                                                          double* __restrict__ c32)
+{
+    // 8 lanes per cell: lanes 0-5 fill the edge slots, lane 6 the area
+    FLAT_ITEM(8u * static_cast<unsigned>(m.nC))
+    {
+        const int ic = static_cast<int>(t >> 3), i = static_cast<int>(t & 7);
+        double* q = c32 + 32L * ic;
+        if (i < kC6) {
+            const int e = m.eoc[kC6 * ic + i];
+            const double s = m.sgnC[kC6 * ic + i], mask = fabs(s), dc = m.dc[e];
+            q[i] = s * c.apvm * bs.u[e] * c.dt / dc;
+            q[6 + i] = s * (kGravity / dc);
+            double kk = s / dc;
+            if (c.drag) {
+                const double kee = 0.5 * (bs.ke[m.coe[2 * e]] + bs.ke[m.coe[2 * e + 1]]);
+                kk = kk + mask * 0.5 * (-0.5 * kDragCoef * bs.u[e] * 2.0 / sqrt(2.0 * kee) / bs.he[e]);
+            }
+            q[12 + i] = kk;
+            q[18 + i] = mask * 0.5;
+        } else if (i == 6) {
+            q[24] = 1.0 / m.areaC[ic];
+        } else {
+            for (int k = 25; k < 32; ++k) q[k] = 0.0;
+        }
+    }
+}
+
+// ---- adjoint member kernels: pa = F'(p_s)^T g as gathers
+
+// edges: pv_edge, h_edge and the partial u adjoints
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, DConfig c, int lb, BaseStage bs,
+__global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q,
                                                       const T* __restrict__ gh,
                                                       const T* __restrict__ gu,
                                                       T* __restrict__ pe_ad,
@@ -654,7 +850,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, DConfig c, int lb
                                                       T* __restrict__ ua_p)
 {
     const int B = 1 << lb;
-    const T apvm = T(c.apvm), dt = T(c.dt);
     ENTITY_ITEM(m.nE)
     {
         const int e = entity_of<WU>(t, lb);
@@ -668,45 +863,31 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, DConfig c, int lb
         for (int j = 0; j < kE2; ++j) gx[j] = ld(gu, ix[j], B, b);
         const T gh1 = ld(gh, c1, B, b), gh2 = ld(gh, c2, B, b);
         const T ge = ld(gu, e, B, b);
-        const T a1 = T(m.areaC[c1]), a2 = T(m.areaC[c2]);
-        const T dv = T(m.dv[e]);
-        const T ub = T(bs.u[e]), heb = T(bs.he[e]), peb = T(bs.pe[e]);
-        const T flux_ad = gh2 / a2 - gh1 / a1;
-        T pad = T(0);
-        T had = ub * dv * flux_ad;
-        T uad = dv * heb * flux_ad;
+        const double* r = q.e48 + 24L * e;
+        const double* s = q.e8 + 16L * e;
+        const T flux_ad = gh2 * T(q.c32[32L * c2 + 24]) - gh1 * T(q.c32[32L * c1 + 24]);
+        // edge eo's q contains this edge as a neighbour: with wt the weight of
+        // this edge in eo's list, d q(eo) / d u(e) = wt*workpv*h_edge(e) and
+        // / d h_edge(e) = wt*workpv*u(e), / d pv_edge(e) = 0.5*wt*u*h_edge
+        T s1 = T(0), s2 = T(0);
 #pragma unroll
         for (int j = 0; j < kE2; ++j) {
-            const int eo = ix[j];
-            const T wgt = T(m.woe[kE2 * e + j]);
-            const T wt = T(m.woeT[kE2 * e + j]);
-            const T wpv = T(0.5) * (peb + T(bs.pe[eo]));
-            // this edge's own q: pv_edge_ad(e) += 0.5*w*u(eo)*he(eo)*q_ad(e)
-            pad = pad + T(0.5) * (wgt * T(bs.u[eo]) * T(bs.he[eo]) * ge);
-            // edge eo's q, in which this edge is neighbour eoeT[e][j]
-            had = had + wt * ub * wpv * gx[j];
-            uad = uad + wt * wpv * heb * gx[j];
-            pad = pad + T(0.5) * (wt * ub * heb * gx[j]);
+            s1 = s1 + T(r[j]) * gx[j];
+            s2 = s2 + T(m.woeT[kE2 * e + j]) * gx[j];
         }
-        uad = uad - apvm * dt * T(bs.gpn[e]) * pad;
-        if (c.drag) {   // src/mpas_sw_dyncore_adj.f90:549-565
-            const T kee = T(0.5) * (T(bs.ke[c1]) + T(bs.ke[c2]));
-            const T sq = sqrt(T(2.0) * kee);
-            had = had + T(kDragCoef) * ub * sq / (heb * heb) * ge;
-            uad = uad - T(kDragCoef) * sq / heb * ge;
-        }
-        if (c.wind)     // :567-572
-            had = had - T(c.usrc[e]) / T(kRhoRef) / (heb * heb) * ge;
+        T had = T(s[0]) * flux_ad + T(s[4]) * ge + T(s[7]) * s1;
+        T uad = T(s[1]) * flux_ad + T(s[5]) * ge + T(s[8]) * s1;
+        const T pad = T(s[2]) * ge + T(s[9]) * s2;
+        uad = uad - T(s[3]) * pad;
         rowp(pe_ad, e, B)[b] = pad;
         rowp(he_ad, e, B)[b] = had;
         rowp(ua_p, e, B)[b] = uad;
     }
 }
 
-// gradPVn -> pv_cell (:202-213), tend_u's gravity term -> h (:696-697),
-// h_edge -> h (:358-369), tend_u's kinetic-energy gradient -> ke (:693-695)
+// cells: pv_cell, h and ke adjoints (+ divergence adjoint of del2/del4)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb, BaseStage bs,
+__global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb, AdjCoef q,
                                                       const T* __restrict__ gu,
                                                       const T* __restrict__ pe_ad,
                                                       const T* __restrict__ he_ad,
@@ -717,7 +898,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
                                                       T* __restrict__ dva)
 {
     const int B = 1 << lb;
-    const T apvm = T(c.apvm), dt = T(c.dt);
     ENTITY_ITEM(m.nC)
     {
         const int ic = entity_of<WU>(t, lb);
@@ -732,28 +912,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
             pa[i] = ld(pe_ad, ie[i], B, b);
             ha[i] = ld(he_ad, ie[i], B, b);
         }
+        const double* r = q.c32 + 32L * ic;
         T pcad = T(0), had = T(0), kad = T(0);
 #pragma unroll
         for (int i = 0; i < kC6; ++i) {
-            const int e = ie[i];
-            const T s = T(m.sgnC[kC6 * ic + i]);
-            const T mask = T(fabs(m.sgnC[kC6 * ic + i]));
-            const T dc = T(m.dc[e]);
-            const T gpn_ad = -apvm * T(bs.u[e]) * dt * pa[i];
-            pcad = pcad - s * gpn_ad / dc;
-            had = had + s * (T(kGravity) / dc * ge[i]) + mask * (T(0.5) * ha[i]);
-            kad = kad + s * ge[i] / dc;
-        }
-        if (c.drag) {   // ke_edge -> ke of both cells (src/mpas_sw_dyncore_adj.f90:555-562)
-#pragma unroll
-            for (int i = 0; i < kC6; ++i) {
-                const int e = ie[i];
-                const T mask = T(fabs(m.sgnC[kC6 * ic + i]));
-                const T kee = T(0.5) * (T(bs.ke[m.coe[2 * e]]) + T(bs.ke[m.coe[2 * e + 1]]));
-                const T kea = -T(0.5) * T(kDragCoef) * T(bs.u[e]) * T(2.0) / sqrt(T(2.0) * kee) /
-                              T(bs.he[e]) * ge[i];
-                kad = kad + mask * (T(0.5) * kea);
-            }
+            pcad = pcad + T(r[i]) * pa[i];
+            had = had + T(r[6 + i]) * ge[i] + T(r[18 + i]) * ha[i];
+            kad = kad + T(r[12 + i]) * ge[i];
         }
         if (c.visc2 > 0.0 || c.visc4 > 0.0) {
             // divergence adjoint of the del2/del4 terms (:579-584, :659-662,
@@ -766,19 +931,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
                 if (c.visc4 > 0.0) x = x + ld(dsu_ad, ie[i], B, b);
                 da = da - T(m.sgnC[kC6 * ic + i]) * x / T(m.dc[ie[i]]);
             }
-            rowp(dva, ic, B)[b] = da * (T(1.0) / T(m.areaC[ic]));
+            rowp(dva, ic, B)[b] = da * T(r[24]);
         }
         rowp(pc_ad, ic, B)[b] = pcad;
         rowp(ha_p, ic, B)[b] = had;
-        rowp(ke_ad, ic, B)[b] = kad / T(m.areaC[ic]);
+        rowp(ke_ad, ic, B)[b] = kad * T(r[24]);
     }
 }
 
-// pv_cell -> pv_vertex (:215-229), pv_edge -> pv_vertex (:244-252),
-// gradPVt -> pv_vertex (:235-262), pv_vertex -> h_vertex, vorticity
-// (:268-284), vorticity -> circulation (:343-349)
+// vertices: h_vertex and circulation adjoints (+ vorticity adjoint of
+// del2/del4, :664-665, :679-680)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int lb, BaseStage bs,
+__global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int lb, AdjCoef q,
                                                        const T* __restrict__ pe_ad,
                                                        const T* __restrict__ pc_ad,
                                                        T* __restrict__ hv_ad,
@@ -787,7 +951,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
                                                        const T* __restrict__ dsu_ad)
 {
     const int B = 1 << lb;
-    const T apvm = T(c.apvm), dt = T(c.dt);
     ENTITY_ITEM(m.nV)
     {
         const int iv = entity_of<WU>(t, lb);
@@ -798,32 +961,24 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
         for (int i = 0; i < 3; ++i) { ic[i] = m.cov[3 * iv + i]; ie[i] = m.eovR[3 * iv + i]; }
 #pragma unroll
         for (int i = 0; i < 3; ++i) { pc[i] = ld(pc_ad, ic[i], B, b); pe[i] = ld(pe_ad, ie[i], B, b); }
+        const double* r = q.v8 + 8L * iv;
         T pa = T(0);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) pa = pa + T(m.kite[3 * iv + i]) / T(m.areaC[ic[i]]) * pc[i];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const int e = ie[i];
-            const T gpt_ad = -apvm * T(bs.v[e]) * dt * pe[i];
-            pa = pa + T(0.5) * pe[i] + T(m.sgnV[3 * iv + i]) * gpt_ad / T(m.dv[e]);
-        }
-        const T hv = T(bs.hv[iv]);
-        const T at = T(m.areaT[iv]);
-        T hva = -(T(m.fV[iv]) + T(bs.vort[iv])) / (hv * hv) * pa;
-        hva = hva / at;
-        T vora = pa / hv;
+        for (int i = 0; i < 3; ++i) pa = pa + T(r[i]) * pc[i] + T(r[3 + i]) * pe[i];
+        T circ = pa * T(r[7]);
         if (c.visc2 > 0.0 || c.visc4 > 0.0) {
-            // vorticity adjoint of the del2/del4 terms (:664-665, :679-680)
+            T vv = T(0);
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 T x = T(0);
                 if (c.visc2 > 0.0) x = T(c.visc2) * ld(gu, ie[i], B, b);
                 if (c.visc4 > 0.0) x = x + ld(dsu_ad, ie[i], B, b);
-                vora = vora - T(m.sgnV[3 * iv + i]) * x / T(m.dv[ie[i]]);
+                vv = vv - T(m.sgnV[3 * iv + i]) * x / T(m.dv[ie[i]]);
             }
+            circ = circ + vv / T(m.areaT[iv]);
         }
-        rowp(hv_ad, iv, B)[b] = hva;
-        rowp(circ_ad, iv, B)[b] = vora / at;
+        rowp(hv_ad, iv, B)[b] = T(r[6]) * pa;
+        rowp(circ_ad, iv, B)[b] = circ;
     }
 }
 
@@ -918,7 +1073,7 @@ __device__ __forceinline__ void adj_update(int stage, bool first, T w, T sw, T p
 // adjoint bookkeeping (src/mpas_sw_module.f90:912-980)
 template <typename T, bool WU>
 __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb, int stage,
-                                                      bool first, T w, T sw, BaseStage bs,
+                                                      bool first, T w, T sw, AdjCoef q,
                                                       const T* __restrict__ pe_ad,
                                                       const T* __restrict__ ua_p,
                                                       const T* __restrict__ ha_p,
@@ -929,7 +1084,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
                                                       T* lamu, T* acch, T* accu, T* gh, T* gu)
 {
     const int B = 1 << lb;
-    const T apvm = T(c.apvm), dt = T(c.dt);
     EDGE_CELL_ITEM(is_edge, t)
     {
         const int b = static_cast<int>(t & (B - 1));
@@ -944,15 +1098,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
             const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
             const T cv2 = ld(circ_ad, m.voe[2 * e + 1], B, b), cv1 = ld(circ_ad, m.voe[2 * e], B, b);
             const T k1 = ld(ke_ad, c1, B, b), k2 = ld(ke_ad, c2, B, b);
+            const double* r = q.e48 + 24L * e + 12;
             T ua = ld(ua_p, e, B, b);
 #pragma unroll
-            for (int j = 0; j < kE2; ++j) {
-                const T vad = -apvm * dt * T(bs.gpt[ix[j]]) * px[j];
-                ua = ua + T(m.woeT[kE2 * e + j]) * vad;
-            }
+            for (int j = 0; j < kE2; ++j) ua = ua + T(r[j]) * px[j];
             ua = ua + T(m.dc[e]) * (cv2 - cv1);
-            // ke(c) = sum 0.25*dc*dv*u^2 / area over the cell's edges (:245-256)
-            const T kc = T(0.25) * T(m.dc[e]) * T(m.dv[e]) * T(2.0) * T(bs.u[e]);
+            const T kc = T(q.e8[16L * e + 6]);
             ua = ua + kc * k1 + kc * k2;
             // divergence(c) = sum(+-u dv) / area (:221-242)
             if (c.visc2 > 0.0 || c.visc4 > 0.0)
@@ -1126,12 +1277,12 @@ RKCoeffs rk_coeffs(double dt)
 
 template <typename T>
 void fwd_diag(const DMesh& m, const DConfig& c, int lb, const T* h, const T* u, T* pvv, T* hv,
-              T* vort, T* ke, T* pc, T* he, T* pe, T* v, T* gpt, T* gpn, hipStream_t s)
+              T* vort, T* ke, T* pc, T* he, T* pe, T* v, T* gpt, T* gpn, double* rec, hipStream_t s)
 {
     LAUNCH_WU(k_fwd_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, lb, h, u, pvv, hv, vort);
     LAUNCH_WU(k_fwd_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, u, pvv, ke, pc);
     LAUNCH_WU(k_fwd_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, h, u, pvv, pc, he, pe, v,
-              gpt, gpn);
+              gpt, gpn, rec);
 }
 
 template <typename T>
@@ -1169,27 +1320,36 @@ Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, co
 template <typename T>
 void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
               const BaseStage& b, const T* h_tl, const T* u_tl, T* pvv_tl, T* ke_tl, T* pc_tl,
-              T* pe_tl, T* he_tl, T* vort_tl, const DissipScratch<T>& ds, T* xh, T* xu, T* acch,
-              T* accu, T* ph, T* pu, hipStream_t s)
+              T* pe_tl, T* he_tl, T* vort_tl, const DissipScratch<T>& ds, const AdjCoef& q, T* xh,
+              T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s)
 {
     const bool visc = c.visc2 > 0.0 || c.visc4 > 0.0;
     LAUNCH_WU(k_tl_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, lb, b.hv, b.vort,
               h_tl, u_tl, pvv_tl, visc ? vort_tl : nullptr);
-    LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, b.u, u_tl,
+    hipLaunchKernelGGL(k_tl_coef, dim3(grid_items(16L * m.nE + 8L * m.nC)), dim3(kBlock), 0, s, m,
+                       b, c.tl_fused, q.e48, q.c32);
+    LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, q.c32, u_tl,
               pvv_tl, ke_tl, pc_tl);
     LAUNCH_WU(k_tl_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, b, h_tl,
               u_tl, pvv_tl, pc_tl, pe_tl, he_tl);
     const Dissip<T> dp = dissip_fields<T>(m, c, lb, u_tl, vort_tl, ds, s);
     LAUNCH_WU(k_tl_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, T(k.w[stage]),
-              T(k.sw[stage]), b, h_tl, u_tl, pe_tl, he_tl, ke_tl, dp, xh, xu, acch, accu, ph, pu);
+              T(k.sw[stage]), b, q.e48, q.c32, h_tl, u_tl, pe_tl, he_tl, ke_tl, dp, xh, xu, acch,
+              accu, ph, pu);
 }
 
 template <typename T>
 void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, const RKCoeffs& k,
                const BaseStage& b, T* gh, T* gu, T* pe_ad, T* he_ad, T* ua_p, T* pc_ad, T* ha_p,
-               T* ke_ad, T* hv_ad, T* circ_ad, const DissipScratch<T>& ds, T* lamh, T* lamu,
-               T* acch, T* accu, hipStream_t s)
+               T* ke_ad, T* hv_ad, T* circ_ad, const DissipScratch<T>& ds, const AdjCoef& q,
+               T* lamh, T* lamu, T* acch, T* accu, hipStream_t s)
 {
+    hipLaunchKernelGGL(k_adj_coef_edge, dim3(grid_items(16L * m.nE)), dim3(kBlock), 0, s, m, c, b,
+                       q.e48, q.e8);
+    hipLaunchKernelGGL(k_adj_coef_vertex, dim3(grid_items(8L * m.nV)), dim3(kBlock), 0, s, m, c, b,
+                       q.v8);
+    hipLaunchKernelGGL(k_adj_coef_cell, dim3(grid_items(8L * m.nC)), dim3(kBlock), 0, s, m, c, b,
+                       q.c32);
     // ds.dsv: circulation adjoint of nabla^2 u [V]; ds.dsd: its divergence
     // adjoint [C]; ds.dsu: nabla^2 u adjoint [E]; ds.div: divergence
     // adjoint / area [C]
@@ -1201,16 +1361,16 @@ void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, 
         LAUNCH_WU(k_adj_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, lb,
                   ds.dsd, ds.dsv, ds.dsu);
     }
-    LAUNCH_WU(k_adj_edge1, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, b, gh, gu,
+    LAUNCH_WU(k_adj_edge1, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, lb, q, gh, gu,
               pe_ad, he_ad, ua_p);
-    LAUNCH_WU(k_adj_cell1, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, c, lb, b, gu,
+    LAUNCH_WU(k_adj_cell1, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, c, lb, q, gu,
               pe_ad, he_ad, pc_ad, ha_p, ke_ad, ds.dsu, ds.div);
-    LAUNCH_WU(k_adj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, c, lb, b, pe_ad,
+    LAUNCH_WU(k_adj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, c, lb, q, pe_ad,
               pc_ad, hv_ad, circ_ad, gu, ds.dsu);
     const double w = stage > 0 ? k.w[stage - 1] : 0.0;
     const double sw = stage > 0 ? k.sw[stage - 1] : 0.0;
     LAUNCH_WU(k_adj_final, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, first, T(w),
-              T(sw), b, pe_ad, ua_p, ha_p, hv_ad, circ_ad, ke_ad, ds.div, lamh, lamu, acch, accu, gh,
+              T(sw), q, pe_ad, ua_p, ha_p, hv_ad, circ_ad, ke_ad, ds.div, lamh, lamu, acch, accu, gh,
               gu);
 }
 
@@ -1283,7 +1443,7 @@ void nonfinite_flags(long n, int lb, const T* x, int* flags, hipStream_t s)
 
 #define INST(T)                                                                                     \
     template void fwd_diag<T>(const DMesh&, const DConfig&, int, const T*, const T*, T*, T*, T*, T*, \
-                              T*, T*, T*, T*, T*, T*, hipStream_t);                                 \
+                              T*, T*, T*, T*, T*, T*, double*, hipStream_t);                        \
     template void fwd_tend_update_he<T>(const DMesh&, const DConfig&, int, int, const RKCoeffs&,    \
                                         const T*, const T*, const T*, const T*, const T*,           \
                                         const Dissip<T>&, T*, T*, T*, T*, T*, T*, hipStream_t);     \
@@ -1291,10 +1451,12 @@ void nonfinite_flags(long n, int lb, const T* x, int* flags, hipStream_t s)
                                         const DissipScratch<T>&, hipStream_t);                      \
     template void tl_stage<T>(const DMesh&, const DConfig&, int, int, const RKCoeffs&,              \
                               const BaseStage&, const T*, const T*, T*, T*, T*, T*, T*, T*,         \
-                              const DissipScratch<T>&, T*, T*, T*, T*, T*, T*, hipStream_t);        \
+                              const DissipScratch<T>&, const AdjCoef&, T*, T*, T*, T*, T*, T*,      \
+                              hipStream_t);                                                         \
     template void adj_stage<T>(const DMesh&, const DConfig&, int, int, bool, const RKCoeffs&,       \
                                const BaseStage&, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*,            \
-                               const DissipScratch<T>&, T*, T*, T*, T*, hipStream_t);               \
+                               const DissipScratch<T>&, const AdjCoef&, T*, T*, T*, T*,             \
+                               hipStream_t);                                                        \
     template void adj_seed<T>(int, T*, const T*, double, hipStream_t);                              \
     template void reconstruct<T>(int, int, const int*, const int*, const double*, const double*,    \
                                  const T*, T*, hipStream_t);                                        \

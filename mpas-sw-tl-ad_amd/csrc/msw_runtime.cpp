@@ -95,6 +95,8 @@ struct msw_ctx {
     double* bdsd = nullptr;           // [C]
     double* bdsv = nullptr;           // [V]
     double* usrc = nullptr;           // [E]  wind-stress source, internal order
+    AdjCoef adjq{};                   // adjoint stage coefficients (B = 1)
+    bool tl_fused = false;            // MSW_OPT_TL_FUSED_FLUX
     void* me1 = nullptr;              // [E]
     void* me2 = nullptr;              // [E]
     void* me3 = nullptr;              // [E]
@@ -371,6 +373,7 @@ int alloc_buffers(msw_ctx* c)
         b.he = eb; b.pe = eb + E; b.v = eb + 2 * E; b.gpt = eb + 3 * E; b.gpn = eb + 4 * E;
         b.ke = eb + 5 * E; b.pc = b.ke + C;
         b.pvv = b.pc + C; b.hv = b.pvv + V; b.vort = b.hv + V;
+        if ((rc = c->alloc(&b.rec, 4 * E))) return rc;
     }
     if ((rc = c->alloc(&c->bacc, C + E))) return rc;
     if ((rc = c->alloc(&c->bx, C + E))) return rc;
@@ -385,6 +388,12 @@ int alloc_buffers(msw_ctx* c)
     take(&c->macc, C + E); take(&c->mpA, C + E); take(&c->mpB, C + E);
     take(&c->mv1, V); take(&c->mv2, V); take(&c->mc1, C); take(&c->mc2, C); take(&c->mc3, C);
     take(&c->me1, E); take(&c->me2, E); take(&c->me3, E);
+    double* q = nullptr;
+    if ((rc = c->alloc(&q, 64 * E + 8 * V + 32 * C))) return rc;
+    c->adjq.e48 = q;
+    c->adjq.e8 = q + 48 * E;
+    c->adjq.v8 = q + 64 * E;
+    c->adjq.c32 = q + 64 * E + 8 * V;
     c->dot_nblk = dot_blocks(std::max<long>(C, E));
     if ((rc = c->alloc(&c->dpartial, static_cast<size_t>(c->dot_nblk) * B + 2 * B))) return rc;
     return MSW_OK;
@@ -418,6 +427,7 @@ void set_dcfg(msw_ctx* c)
     c->dcfg.wind = g.wind_stress ? 1 : 0;
     c->dcfg.drag = g.bottom_drag ? 1 : 0;
     c->dcfg.usrc = c->usrc;
+    c->dcfg.tl_fused = c->tl_fused ? 1 : 0;
 }
 
 template <typename T>
@@ -461,7 +471,7 @@ void enqueue_fwd_step(msw_ctx* c, T* h, T* u)
     T* vort = static_cast<T*>(c->xvort);   // null unless a viscosity is on
     for (int s = 0; s < 4; ++s) {
         fwd_diag<T>(m, c->dcfg, c->lb, rh, ru, pvv, nullptr, vort, ke, pc, he, pe, nullptr,
-                    nullptr, nullptr, c->stream);
+                    nullptr, nullptr, nullptr, c->stream);
         const Dissip<T> dp = dissip_fields<T>(m, c->dcfg, c->lb, ru, vort, member_ds<T>(c), c->stream);
         T* nb = (s % 2 == 0) ? pA : pB;
         fwd_tend_update_he<T>(m, c->dcfg, c->lb, s, k, rh, ru, he, pe, ke, dp, h, u, acc,
@@ -483,7 +493,7 @@ void enqueue_base_step(msw_ctx* c, double* h, double* u)
     for (int s = 0; s < 4; ++s) {
         BaseStage& b = c->st[s];
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
-                         b.gpt, b.gpn, c->stream);
+                         b.gpt, b.gpn, b.rec, c->stream);
         double* nh = s < 3 ? c->st[s + 1].h : c->bend;
         base_tend(c, b, s, k, h, u, nh);
     }
@@ -511,11 +521,11 @@ void enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
     for (int s = 0; s < 4; ++s) {
         BaseStage& b = c->st[s];
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
-                         b.gpt, b.gpn, c->stream);
+                         b.gpt, b.gpn, b.rec, c->stream);
         T* nb = (s % 2 == 0) ? pA : pB;
         tl_stage<T>(m, c->dcfg, c->lb, s, k, b, th, tu, pvv_tl, ke_tl, pc_tl, pe_tl, he_tl,
-                    static_cast<T*>(c->xvort), member_ds<T>(c), htl, utl, acc, acc + C * B, nb,
-                    nb + C * B, c->stream);
+                    static_cast<T*>(c->xvort), member_ds<T>(c), c->adjq, htl, utl, acc, acc + C * B,
+                    nb, nb + C * B, c->stream);
         double* nh = s < 3 ? c->st[s + 1].h : c->bend;
         base_tend(c, b, s, k, h, u, nh);
         th = nb;
@@ -537,7 +547,7 @@ void enqueue_adj_step(msw_ctx* c, T* lamh, T* lamu, bool want_end)
     for (int s = 0; s < 4; ++s) {
         BaseStage& b = c->st[s];
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
-                         b.gpt, b.gpn, c->stream);
+                         b.gpt, b.gpn, b.rec, c->stream);
         if (s < 3 || want_end) {
             // stage 3 writes x_new into its "x" argument: give it the end buffer
             double* nh = s < 3 ? c->st[s + 1].h : c->bend;
@@ -555,8 +565,8 @@ void enqueue_adj_step(msw_ctx* c, T* lamh, T* lamu, bool want_end)
                      static_cast<T*>(c->me1), static_cast<T*>(c->me2), static_cast<T*>(c->me3),
                      static_cast<T*>(c->mc1), static_cast<T*>(c->mc2), static_cast<T*>(c->mc3),
                      static_cast<T*>(c->mv1),
-                     static_cast<T*>(c->mv2), member_ds<T>(c), lamh, lamu, acc, acc + C * B,
-                     c->stream);
+                     static_cast<T*>(c->mv2), member_ds<T>(c), c->adjq, lamh, lamu, acc,
+                     acc + C * B, c->stream);
 }
 
 int check_cfg(msw_ctx* c)
@@ -640,8 +650,8 @@ int msw_create(const msw_mesh_desc* mesh, const msw_config* cfg, int prec, int b
     // kernels index (entity, member) items with 32-bit arithmetic
     const long max_rows = std::max<long>({static_cast<long>(c->nC), static_cast<long>(c->nE),
                                           static_cast<long>(c->nV)});
-    if ((max_rows << c->lb) >= (1L << 31))
-        return fail(MSW_EINVAL, "mesh entities times batch must stay below 2^31");
+    if ((max_rows << c->lb) >= (1L << 31) || 68L * c->nE >= (1L << 31))
+        return fail(MSW_EINVAL, "mesh entities times batch (and 68 x edges) must stay below 2^31");
     c->stream = static_cast<hipStream_t>(hip_stream);
     c->cfg = *cfg;
     int rc = build_mesh(c.get(), mesh);
@@ -685,6 +695,18 @@ int msw_set_forcing(msw_ctx* ctx, const double* u_src)
                                hipMemcpyDeviceToDevice, ctx->stream));
     else
         HIP_TRY(hipMemsetAsync(ctx->usrc, 0, static_cast<size_t>(ctx->nE) * sizeof(double), ctx->stream));
+    return MSW_OK;
+}
+
+int msw_set_option(msw_ctx* ctx, int option, int value)
+{
+    if (!ctx) return fail(MSW_EINVAL, "NULL ctx");
+    if (option != MSW_OPT_TL_FUSED_FLUX) return fail(MSW_EINVAL, "unknown option");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->tl_fused = value != 0;
+    set_dcfg(ctx);
+    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
     return MSW_OK;
 }
 
@@ -816,7 +838,7 @@ int msw_diagnostics(msw_ctx* ctx, const double* h, const double* u, double* out)
     double* vc = o; o += C;
     double* gpn = o; o += E;
     double* gpt = o; o += E;
-    fwd_diag<double>(ctx->dm, ctx->dcfg, 0, h, u, pvv, hv, vort, ke, pc, he, pe, v, gpt, gpn, ctx->stream);
+    fwd_diag<double>(ctx->dm, ctx->dcfg, 0, h, u, pvv, hv, vort, ke, pc, he, pe, v, gpt, gpn, nullptr, ctx->stream);
     diag_extra(ctx->dm, u, vort, circ, div, vc, ctx->stream);
     HIP_TRY(hipGetLastError());
     return MSW_OK;

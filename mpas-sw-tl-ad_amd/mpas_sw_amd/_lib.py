@@ -72,11 +72,14 @@ class Config(C.Structure):
 
 # every symbol include/mpas_sw_amd.h declares
 EXPORTS = (
-    "msw_create", "msw_destroy", "msw_set_config", "msw_set_forcing", "msw_set_graphs", "msw_fwd_steps",
+    "msw_create", "msw_destroy", "msw_set_config", "msw_set_forcing", "msw_set_option",
+    "msw_set_graphs", "msw_fwd_steps",
     "msw_base_steps", "msw_tlm_steps", "msw_adj_step", "msw_adj_steps", "msw_diagnostics",
     "msw_diag_names", "msw_diag_kinds", "msw_reconstruct", "msw_dot", "msw_nonfinite",
     "msw_permute", "msw_device_bytes", "msw_abi_version", "msw_last_error",
 )
+
+OPT_TL_FUSED_FLUX = 1   # msw_option
 
 _lib = None
 
@@ -105,6 +108,7 @@ def load():
         "msw_destroy": ([vp], i32),
         "msw_set_config": ([vp, C.POINTER(Config)], i32),
         "msw_set_forcing": ([vp, vp], i32),
+        "msw_set_option": ([vp, i32, i32], i32),
         "msw_set_graphs": ([vp, i32], i32),
         "msw_fwd_steps": ([vp, vp, vp, i32], i32),
         "msw_base_steps": ([vp, vp, vp, i32, vp, vp], i32),

@@ -47,13 +47,15 @@ class SWModel:
     perm : explicit (cell, edge, vertex) permutations, new -> old (overrides sfc).
     u_src : wind-stress source on edges (reference order), read when
         wind_stress is set; zero by default as after var_allocation.
+    tl_fused_flux : form the TL tendency from the TL thickness flux (fewer
+        gathers per member; rounding differs from the reference's term order).
     """
 
     def __init__(self, mesh: Mesh, phys: dict, dt: float, *, apvm: float = 0.5, test_case: int = 2,
                  visc2: float = 0.0, visc4: float = 0.0, thickness_adv_order: int = 2,
                  monotonic: bool = False, wind_stress: bool = False, bottom_drag: bool = False,
                  batch: int = 1, precision: str = "fp64", sfc: bool = True, device=None,
-                 graphs: bool = False, perm=None, u_src=None):
+                 graphs: bool = False, perm=None, u_src=None, tl_fused_flux: bool = False):
         if precision not in _DT:
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.lib = _lib.load()
@@ -121,6 +123,8 @@ class SWModel:
         self._inv = {k: np.argsort(v).astype(np.int64) for k, v in self.perm.items()}
         if u_src is not None:
             self.set_forcing(u_src)
+        if tl_fused_flux:
+            self.set_option(_lib.OPT_TL_FUSED_FLUX, 1)
 
     # ------------------------------------------------------------------ utils
     def close(self):
@@ -152,6 +156,9 @@ class SWModel:
         if bottom_drag is not None:
             self.cfg.bottom_drag = int(bool(bottom_drag))
         _lib.check(self.lib.msw_set_config(self._h, C.byref(self.cfg)))
+
+    def set_option(self, option: int, value: int):
+        _lib.check(self.lib.msw_set_option(self._h, int(option), int(value)))
 
     def set_forcing(self, u_src):
         """Wind-stress source u_src [E] (reference order); None resets it to 0."""
