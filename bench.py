@@ -120,6 +120,64 @@ def calibration_copy(nbytes=1 << 30, reps=3):
     torch.cuda.empty_cache()
 
 
+def traffic_from_counters(fetch_dir, write_dir, nsteps, out_json, out_md, command):
+    """HBM bytes per TLM+ADJ step from two rocprofv3 --pmc runs of
+    `bench.py --steps S --warmup W --no-cpu --no-plain --no-fp32 --calibrate`
+    (FETCH_SIZE, then WRITE_SIZE; counters in KiB per dispatch).  Each counter
+    is scaled by what it reads for the 1 GiB calibration copy; the library's
+    kernels over the whole run are summed and divided by the S + W window
+    steps of the FP64 batch."""
+    import csv
+    import glob
+
+    def rows(d):
+        f = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not f:
+            raise FileNotFoundError(f"no counter_collection.csv under {d}")
+        return list(csv.DictReader(open(f[0])))
+
+    def scale(rs, counter, nbytes):
+        vals = [float(r["Counter_Value"]) for r in rs
+                if r["Counter_Name"] == counter and "copyBuffer" in r["Kernel_Name"]]
+        big = max(vals)
+        return nbytes / (big * 1024.0), big
+
+    fr, wr = rows(fetch_dir), rows(write_dir)
+    fs, fraw = scale(fr, "FETCH_SIZE", 1 << 30)
+    ws, wraw = scale(wr, "WRITE_SIZE", 1 << 30)
+    per = {}
+    for rs, counter, sc, key in ((fr, "FETCH_SIZE", fs, "read"), (wr, "WRITE_SIZE", ws, "write")):
+        for r in rs:
+            if r["Counter_Name"] != counter or "msw::" not in r["Kernel_Name"]:
+                continue
+            n = r["Kernel_Name"].replace("void ", "").replace("msw::(anonymous namespace)::", "").split("(")[0]
+            d = per.setdefault(n, {"read": 0.0, "write": 0.0, "calls": 0})
+            d[key] += float(r["Counter_Value"]) * 1024.0 * sc
+            if key == "read":
+                d["calls"] += 1
+    total = sum(d["read"] + d["write"] for d in per.values())
+    res = {"hbm_bytes_per_step": total / nsteps, "window_steps": nsteps,
+           "fetch_scale": fs, "write_scale": ws, "command": command,
+           "kernels": {k: {"read_bytes_per_call": v["read"] / max(v["calls"], 1),
+                           "write_bytes_per_call": v["write"] / max(v["calls"], 1),
+                           "calls": v["calls"]} for k, v in per.items()}}
+    with open(out_json, "w") as f:
+        json.dump(res, f, indent=1)
+    with open(out_md, "w") as f:
+        f.write("# HBM traffic of the batched TLM+ADJ step (level 7, TC6, 64 members, FP64)\n\n")
+        f.write(f"Commands (from the repository root):\n\n```\n{command}\n```\n\n")
+        f.write(f"Calibration: the 1 GiB device copy reads FETCH_SIZE = {fraw:.0f} KiB and "
+                f"WRITE_SIZE = {wraw:.0f} KiB, so FETCH_SIZE is scaled by {fs:.3f} and "
+                f"WRITE_SIZE by {ws:.3f}.\n\n")
+        f.write(f"HBM bytes per step (all kernels of the library, {nsteps} steps): "
+                f"{total / nsteps / 1e9:.3f} GB\n\n")
+        f.write("| kernel | calls | read MB/call | write MB/call |\n|---|---:|---:|---:|\n")
+        for k, v in sorted(per.items(), key=lambda kv: -(kv[1]["read"] + kv[1]["write"])):
+            c = max(v["calls"], 1)
+            f.write(f"| `{k}` | {v['calls']} | {v['read'] / c / 1e6:.1f} | {v['write'] / c / 1e6:.1f} |\n")
+    return res
+
+
 def cpu_baseline(mesh, phys, dh, du, steps=2):
     """Reference Fortran (oracle/_ref/msw_ref, one core) TLM and adjoint
     steps on the same mesh and case; member cell-steps/s of TLM+ADJ."""
@@ -148,7 +206,22 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 1 GiB device copy (reference for HBM counters)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
+                         "`--no-cpu --no-plain --no-fp32 --calibrate` into profiles/ and exit")
     args = ap.parse_args()
+    if args.traffic_from:
+        cmd = ("rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc1 -o run -- python3 bench.py "
+               f"--steps {args.steps} --warmup {args.warmup} --no-cpu --no-plain --no-fp32 --calibrate\n"
+               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc2 -o run -- python3 bench.py "
+               f"--steps {args.steps} --warmup {args.warmup} --no-cpu --no-plain --no-fp32 --calibrate\n"
+               f"python3 bench.py --steps {args.steps} --warmup {args.warmup} --traffic-from "
+               "gpurun_out/pmc1 gpurun_out/pmc2")
+        r = traffic_from_counters(args.traffic_from[0], args.traffic_from[1], args.steps + args.warmup,
+                                  os.path.join(ROOT, "profiles", "traffic_tlm_adj_l7_b64_fp64.json"),
+                                  os.path.join(ROOT, "profiles", "traffic_tlm_adj_l7_b64_fp64.md"), cmd)
+        print(json.dumps({"hbm_bytes_per_step": r["hbm_bytes_per_step"]}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the batched members would shard "
               "across GPUs with no per-step exchange (see DESIGN.md)", file=sys.stderr)

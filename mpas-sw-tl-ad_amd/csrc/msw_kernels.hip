@@ -102,6 +102,20 @@ inline int grid_for(long threads)
     for (long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x; t < (total); \
          t += static_cast<long>(gridDim.x) * kBlock)
 
+// Streaming accesses: member fields written by one pass and read by a later
+// one, and the RK arrays read once, bypass the L2 retention the neighbour
+// gathers rely on (nontemporal hint).
+template <typename T>
+__device__ __forceinline__ void stn(T* p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+template <typename T>
+__device__ __forceinline__ T ldn(const T* p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
 // row `row` of a members-innermost field: the row base is wave-uniform
 // whenever the row index is, so the load is one SGPR base + lane offset
 template <typename T>
@@ -145,9 +159,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd_vertex(DMesh m, int lb, const T*
 #pragma unroll
         for (int i = 0; i < 3; ++i) hh = hh + hc[i] * T(m.kite[3 * iv + i]);
         hh = hh / at;
-        rowp(pvv, iv, B)[b] = (T(m.fV[iv]) + vo) / hh;
-        if (hv) rowp(hv, iv, B)[b] = hh;
-        if (vort) rowp(vort, iv, B)[b] = vo;
+        stn(&rowp(pvv, iv, B)[b], (T(m.fV[iv]) + vo) / hh);
+        if (hv) stn(&rowp(hv, iv, B)[b], hh);
+        if (vort) stn(&rowp(vort, iv, B)[b], vo);
     }
 }
 
@@ -179,8 +193,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_cell(DMesh m, int lb, const T* _
         T p = T(0);
 #pragma unroll
         for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
-        rowp(ke, ic, B)[b] = k;
-        rowp(pc, ic, B)[b] = p;
+        stn(&rowp(ke, ic, B)[b], k);
+        stn(&rowp(pc, ic, B)[b], p);
     }
 }
 
@@ -223,11 +237,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_edge(DMesh m, DConfig c, int lb,
         T p = T(0.5) * p1 + T(0.5) * p2;
         p = p - apvm * vv * dt * gt;
         p = p - apvm * ue * dt * gn;
-        rowp(he, e, B)[b] = T(0.5) * (h1 + h2);
-        rowp(pe, e, B)[b] = p;
-        if (vout) rowp(vout, e, B)[b] = vv;
-        if (gpt_out) rowp(gpt_out, e, B)[b] = gt;
-        if (gpn_out) rowp(gpn_out, e, B)[b] = gn;
+        stn(&rowp(he, e, B)[b], T(0.5) * (h1 + h2));
+        stn(&rowp(pe, e, B)[b], p);
+        if (vout) stn(&rowp(vout, e, B)[b], vv);
+        if (gpt_out) stn(&rowp(gpt_out, e, B)[b], gt);
+        if (gpn_out) stn(&rowp(gpn_out, e, B)[b], gn);
         if (rec) {   // base path (B = 1): one record per edge for neighbour gathers
             double2* r = reinterpret_cast<double2*>(rec + 4L * e);
             r[0] = make_double2(static_cast<double>(p), static_cast<double>(ue));
@@ -245,15 +259,15 @@ __device__ __forceinline__ void rk_update(int stage, T w, T sw, T tend, T* x, T*
     acc = rowp(acc, row, B);
     pnext = rowp(pnext, row, B);
     if (stage == 0) {
-        const T xv = x[o];
-        acc[o] = xv + w * tend;
-        pnext[o] = frozen ? xv : xv + sw * tend;
+        const T xv = ldn(x + o);
+        stn(acc + o, xv + w * tend);
+        stn(pnext + o, frozen ? xv : xv + sw * tend);
     } else if (stage < 3) {
-        const T xv = x[o];
-        acc[o] = acc[o] + w * tend;
-        pnext[o] = frozen ? xv : xv + sw * tend;
+        const T xv = ldn(x + o);
+        stn(acc + o, ldn(acc + o) + w * tend);
+        stn(pnext + o, frozen ? xv : xv + sw * tend);
     } else {
-        if (!frozen) x[o] = acc[o] + w * tend;
+        if (!frozen) stn(x + o, ldn(acc + o) + w * tend);
     }
 }
 
@@ -308,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_div(DMesh m, int lb, const T* __rest
 #pragma unroll
         for (int k = 0; k < kC6; ++k) d = d + T(m.sgnCR[kC6 * ic + k]) * (ue[k] * T(m.dv[ie[k]]));
         const T r = T(1.0) / T(m.areaC[ic]);
-        rowp(div, ic, B)[b] = d * r;
+        stn(&rowp(div, ic, B)[b], d * r);
     }
 }
 
@@ -323,8 +337,8 @@ __global__ __launch_bounds__(kBlock) void k_del4_edge(DMesh m, int lb, const T* 
     {
         const int e = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
-        rowp(dsu, e, B)[b] = grad_div_curl(div, vort, m.coe[2 * e], m.coe[2 * e + 1], m.voe[2 * e],
-                                           m.voe[2 * e + 1], B, b, T(m.dc[e]), T(m.dv[e]));
+        stn(&rowp(dsu, e, B)[b], grad_div_curl(div, vort, m.coe[2 * e], m.coe[2 * e + 1], m.voe[2 * e],
+                                           m.voe[2 * e + 1], B, b, T(m.dc[e]), T(m.dv[e])));
     }
 }
 
@@ -348,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_del4_vertex(DMesh m, int lb, const T
 #pragma unroll
         for (int i = 0; i < 3; ++i) sc = sc + T(m.sgnV[3 * iv + i]) * (T(m.dc[ie[i]]) * x[i]);
         const T r = T(1.0) / T(m.areaT[iv]);
-        rowp(dsv, iv, B)[b] = sc * r;
+        stn(&rowp(dsv, iv, B)[b], sc * r);
     }
 }
 
@@ -372,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_del4_cell(DMesh m, int lb, const T* 
 #pragma unroll
         for (int k = 0; k < kC6; ++k) d = d + T(m.sgnCR[kC6 * ic + k]) * (x[k] * T(m.dv[ie[k]]));
         const T r = T(1.0) / T(m.areaC[ic]);
-        rowp(dsd, ic, B)[b] = d * r;
+        stn(&rowp(dsd, ic, B)[b], d * r);
     }
 }
 
@@ -473,9 +487,8 @@ __global__ __launch_bounds__(kBlock) void k_tl_vertex(DMesh m, int lb, const dou
         for (int i = 0; i < 3; ++i) ht = ht + hc[i] * T(m.kite[3 * iv + i]);
         ht = ht / at;
         const T hv = T(hvb[iv]);
-        rowp(pvv_tl, iv, B)[b] =
-            vt / hv - ht * (T(m.fV[iv]) + T(vortb[iv])) / (hv * hv);
-        if (vort_tl) rowp(vort_tl, iv, B)[b] = vt;
+        stn(&rowp(pvv_tl, iv, B)[b], vt / hv - ht * (T(m.fV[iv]) + T(vortb[iv])) / (hv * hv));
+        if (vort_tl) stn(&rowp(vort_tl, iv, B)[b], vt);
     }
 }
 
@@ -510,8 +523,8 @@ __global__ __launch_bounds__(kBlock) void k_tl_cell(DMesh m, int lb, const doubl
         T p = T(0);
 #pragma unroll
         for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
-        rowp(ke_tl, ic, B)[b] = k;
-        rowp(pc_tl, ic, B)[b] = p;
+        stn(&rowp(ke_tl, ic, B)[b], k);
+        stn(&rowp(pc_tl, ic, B)[b], p);
     }
 }
 
@@ -550,10 +563,10 @@ __global__ __launch_bounds__(kBlock) void k_tl_edge(DMesh m, DConfig c, int lb, 
         T p = T(0.5) * p1 + T(0.5) * p2;
         p = p - apvm * vt * dt * T(bs.gpt[e]) - apvm * T(bs.v[e]) * dt * gtt;
         p = p - apvm * ue * dt * T(bs.gpn[e]) - apvm * T(bs.u[e]) * dt * gnt;
-        rowp(pe_tl, e, B)[b] = p;
+        stn(&rowp(pe_tl, e, B)[b], p);
         const T het = T(0.5) * (h1 + h2);
         // fused form: the TL thickness flux u_tl*h_edge + u*h_edge_tl (per dvEdge)
-        rowp(he_tl, e, B)[b] = c.tl_fused ? ue * T(bs.he[e]) + T(bs.u[e]) * het : het;
+        stn(&rowp(he_tl, e, B)[b], c.tl_fused ? ue * T(bs.he[e]) + T(bs.u[e]) * het : het);
     }
 }
 
@@ -879,9 +892,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q
         T uad = T(s[1]) * flux_ad + T(s[5]) * ge + T(s[8]) * s1;
         const T pad = T(s[2]) * ge + T(s[9]) * s2;
         uad = uad - T(s[3]) * pad;
-        rowp(pe_ad, e, B)[b] = pad;
-        rowp(he_ad, e, B)[b] = had;
-        rowp(ua_p, e, B)[b] = uad;
+        stn(&rowp(pe_ad, e, B)[b], pad);
+        stn(&rowp(he_ad, e, B)[b], had);
+        stn(&rowp(ua_p, e, B)[b], uad);
     }
 }
 
@@ -931,11 +944,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
                 if (c.visc4 > 0.0) x = x + ld(dsu_ad, ie[i], B, b);
                 da = da - T(m.sgnC[kC6 * ic + i]) * x / T(m.dc[ie[i]]);
             }
-            rowp(dva, ic, B)[b] = da * T(r[24]);
+            stn(&rowp(dva, ic, B)[b], da * T(r[24]));
         }
-        rowp(pc_ad, ic, B)[b] = pcad;
-        rowp(ha_p, ic, B)[b] = had;
-        rowp(ke_ad, ic, B)[b] = kad * T(r[24]);
+        stn(&rowp(pc_ad, ic, B)[b], pcad);
+        stn(&rowp(ha_p, ic, B)[b], had);
+        stn(&rowp(ke_ad, ic, B)[b], kad * T(r[24]));
     }
 }
 
@@ -977,8 +990,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
             }
             circ = circ + vv / T(m.areaT[iv]);
         }
-        rowp(hv_ad, iv, B)[b] = T(r[6]) * pa;
-        rowp(circ_ad, iv, B)[b] = circ;
+        stn(&rowp(hv_ad, iv, B)[b], T(r[6]) * pa);
+        stn(&rowp(circ_ad, iv, B)[b], circ);
     }
 }
 
@@ -1004,7 +1017,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_del4_vertex(DMesh m, DConfig c, 
         T a = T(0);
 #pragma unroll
         for (int i = 0; i < 3; ++i) a = a + T(m.sgnV[3 * iv + i]) * (T(c.visc4) * g[i]) / T(m.dv[ie[i]]);
-        rowp(dsc_ad, iv, B)[b] = a * (T(1.0) / T(m.areaT[iv]));
+        stn(&rowp(dsc_ad, iv, B)[b], a * (T(1.0) / T(m.areaT[iv])));
     }
 }
 
@@ -1027,7 +1040,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_del4_cell(DMesh m, DConfig c, in
         T a = T(0);
 #pragma unroll
         for (int i = 0; i < kC6; ++i) a = a + T(m.sgnC[kC6 * ic + i]) * (T(c.visc4) * g[i]) / T(m.dc[ie[i]]);
-        rowp(dsd_ad, ic, B)[b] = a * (T(1.0) / T(m.areaC[ic]));
+        stn(&rowp(dsd_ad, ic, B)[b], a * (T(1.0) / T(m.areaC[ic])));
     }
 }
 
@@ -1048,7 +1061,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_del4_edge(DMesh m, int lb,
         const T dv = T(m.dv[e]), dc = T(m.dc[e]);
         T a = -dv * ld(dsd_ad, c2, B, b) + dv * ld(dsd_ad, c1, B, b);
         a = a + dc * ld(dsc_ad, v2, B, b) - dc * ld(dsc_ad, v1, B, b);
-        rowp(dsu_ad, e, B)[b] = a;
+        stn(&rowp(dsu_ad, e, B)[b], a);
     }
 }
 
@@ -1061,10 +1074,10 @@ __device__ __forceinline__ void adj_update(int stage, bool first, T w, T sw, T p
     acc = rowp(acc, row, B);
     g = rowp(g, row, B);
     if (stage > 0) {
-        acc[o] = first ? pa : acc[o] + pa;
-        g[o] = w * lam[o] + sw * pa;
+        stn(acc + o, first ? pa : ldn(acc + o) + pa);
+        stn(g + o, w * ldn(lam + o) + sw * pa);
     } else {
-        lam[o] = (acc[o] + lam[o]) + pa;
+        stn(lam + o, (ldn(acc + o) + ldn(lam + o)) + pa);
     }
 }
 
@@ -1160,7 +1173,7 @@ __global__ __launch_bounds__(kBlock) void k_reconstruct(int nC, int lb, const in
         }
         const T clat = T(trig[4 * ic]), slat = T(trig[4 * ic + 1]);
         const T clon = T(trig[4 * ic + 2]), slon = T(trig[4 * ic + 3]);
-        rowp(out, ic, B)[b] = X;
+        stn(&rowp(out, ic, B)[b], X);
         rowp(out + plane, ic, B)[b] = Y;
         rowp(out + 2 * plane, ic, B)[b] = Z;
         rowp(out + 3 * plane, ic, B)[b] = -X * slon + Y * clon;
