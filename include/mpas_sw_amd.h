@@ -153,6 +153,11 @@ const char *msw_diag_kinds(void);
 /* Replaces: mpas_reconstruct_2d (src/mpas_sw_operators.f90:283-370) for the
  * batch: out5 = [X, Y, Z, zonal, meridional], each [C][B] of `prec`. */
 int msw_reconstruct(msw_ctx *ctx, const void *u, void *out5);
+/* Replaces: mpas_reconstruct_2d_adj (src/mpas_sw_dyncore_adj.f90:740-831) as
+ * sw_rk4_adj applies it to the end-state adjoint (src/mpas_sw_module.f90:887-892):
+ * u_ad += R^T seeds5, seeds5 = [X, Y, Z, zonal, meridional] adjoint seeds, each
+ * [C][B] of `prec` (not modified; the reference zeroes them after use). */
+int msw_reconstruct_adj(msw_ctx *ctx, const void *seeds5, void *u_ad);
 
 /* Deterministic FP64 inner products per member:
  *   out[b] = sum_c wh[c] xh[c][b] yh[c][b] + sum_e wu[e] xu[e][b] yu[e][b]

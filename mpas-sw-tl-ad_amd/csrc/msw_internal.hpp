@@ -41,6 +41,7 @@ struct DMesh {
     const int* eoe;       // [E][12]
     const double* woe;    // [E][12]
     const int* eoeT;      // [E][12] slot of this edge in eoe[eoe[e][j]]
+    const int* eSlot;     // [E][2]  slot of the edge in eoc of its cell 1 and cell 2
     const double* woeT;   // [E][12] woe[eoe[e][j]][eoeT[e][j]]: this edge's weight in
                           //         the list of its j-th neighbour (0 when padded)
     const double* dc;     // [E]
@@ -151,6 +152,11 @@ template <typename T>
 Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, const T* vort,
                         const DissipScratch<T>& ds, hipStream_t s);
 
+// Coefficient rows of the TL / adjoint stage operator from base fields `b`
+// (B = 1); the member launchers below read them from `q`.
+void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s);
+void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s);
+
 // Tangent-linear stage on top of base fields `b` of the same stage.
 // vort_tl [V] and ds are used only when a viscosity is on.
 template <typename T>
@@ -176,6 +182,13 @@ void adj_seed(int n, T* g, const T* lam, double w3, hipStream_t s);
 template <typename T>
 void reconstruct(int nC, int lb, const int* nEoC, const int* eoc, const double* coeffs,
                  const double* trig, const T* u, T* out5, hipStream_t s);
+
+// Adjoint of the reconstruction (src/mpas_sw_dyncore_adj.f90:740-831):
+// u_ad += R^T seeds5, seeds5 = [X, Y, Z, zonal, meridional] adjoints, each
+// [C][B]; xyz is [3][C][B] scratch.
+template <typename T>
+void reconstruct_adj(const DMesh& m, int lb, const double* coeffs, const double* trig,
+                     const T* seeds5, T* xyz, T* u_ad, hipStream_t s);
 
 // Deterministic per-member weighted dot products: out[b] = sum_i w_i x[i][b] y[i][b]
 // (w may be null).  out is double[B].  `partial` holds nblk*B doubles.

@@ -1181,6 +1181,61 @@ __global__ __launch_bounds__(kBlock) void k_reconstruct(int nC, int lb, const in
     }
 }
 
+// Adjoint of the reconstruction (src/mpas_sw_dyncore_adj.f90:740-831): the
+// zonal/meridional seeds are rotated back onto X, Y, Z per cell, then each
+// edge gathers coeffs . (X, Y, Z) from its two cells.
+template <typename T, bool WU>
+__global__ __launch_bounds__(kBlock) void k_recon_adj_cell(int nC, int lb,
+                                                           const double* __restrict__ trig,
+                                                           const T* __restrict__ seeds,
+                                                           T* __restrict__ xyz)
+{
+    const int B = 1 << lb;
+    const size_t plane = static_cast<size_t>(nC) * B;
+    ENTITY_ITEM(nC)
+    {
+        const int ic = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        const T clat = T(trig[4 * ic]), slat = T(trig[4 * ic + 1]);
+        const T clon = T(trig[4 * ic + 2]), slon = T(trig[4 * ic + 3]);
+        T x = ld(seeds, ic, B, b), y = ld(seeds + plane, ic, B, b), z = ld(seeds + 2 * plane, ic, B, b);
+        const T zo = ld(seeds + 3 * plane, ic, B, b), me = ld(seeds + 4 * plane, ic, B, b);
+        z = z + clat * me;
+        y = y - slon * slat * me;
+        x = x - clon * slat * me;
+        y = y + clon * zo;
+        x = x - slon * zo;
+        rowp(xyz, ic, B)[b] = x;
+        rowp(xyz + plane, ic, B)[b] = y;
+        rowp(xyz + 2 * plane, ic, B)[b] = z;
+    }
+}
+
+template <typename T, bool WU>
+__global__ __launch_bounds__(kBlock) void k_recon_adj_edge(DMesh m, int lb,
+                                                           const double* __restrict__ coeffs,
+                                                           const T* __restrict__ xyz,
+                                                           T* __restrict__ u_ad)
+{
+    const int B = 1 << lb;
+    const size_t plane = static_cast<size_t>(m.nC) * B;
+    ENTITY_ITEM(m.nE)
+    {
+        const int e = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        T a = ld(u_ad, e, B, b);
+#pragma unroll
+        for (int k = 1; k >= 0; --k) {
+            const int ic = m.coe[2 * e + k];
+            const double* cf = coeffs + (static_cast<size_t>(ic) * kMaxEdges + m.eSlot[2 * e + k]) * 3;
+            a = a + T(cf[2]) * ld(xyz + 2 * plane, ic, B, b);
+            a = a + T(cf[1]) * ld(xyz + plane, ic, B, b);
+            a = a + T(cf[0]) * ld(xyz, ic, B, b);
+        }
+        rowp(u_ad, e, B)[b] = a;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic batched reductions in FP64.  Block k sums rows
 // [k*chunk, (k+1)*chunk) for every member; the host-side order of the final
@@ -1330,6 +1385,22 @@ Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, co
     return dp;
 }
 
+void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tl_coef, dim3(grid_items(16L * m.nE + 8L * m.nC)), dim3(kBlock), 0, s, m,
+                       b, c.tl_fused, q.e48, q.c32);
+}
+
+void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_coef_edge, dim3(grid_items(16L * m.nE)), dim3(kBlock), 0, s, m, c, b,
+                       q.e48, q.e8);
+    hipLaunchKernelGGL(k_adj_coef_vertex, dim3(grid_items(8L * m.nV)), dim3(kBlock), 0, s, m, c, b,
+                       q.v8);
+    hipLaunchKernelGGL(k_adj_coef_cell, dim3(grid_items(8L * m.nC)), dim3(kBlock), 0, s, m, c, b,
+                       q.c32);
+}
+
 template <typename T>
 void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
               const BaseStage& b, const T* h_tl, const T* u_tl, T* pvv_tl, T* ke_tl, T* pc_tl,
@@ -1339,8 +1410,6 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
     const bool visc = c.visc2 > 0.0 || c.visc4 > 0.0;
     LAUNCH_WU(k_tl_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, lb, b.hv, b.vort,
               h_tl, u_tl, pvv_tl, visc ? vort_tl : nullptr);
-    hipLaunchKernelGGL(k_tl_coef, dim3(grid_items(16L * m.nE + 8L * m.nC)), dim3(kBlock), 0, s, m,
-                       b, c.tl_fused, q.e48, q.c32);
     LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, q.c32, u_tl,
               pvv_tl, ke_tl, pc_tl);
     LAUNCH_WU(k_tl_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, b, h_tl,
@@ -1357,12 +1426,6 @@ void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, 
                T* ke_ad, T* hv_ad, T* circ_ad, const DissipScratch<T>& ds, const AdjCoef& q,
                T* lamh, T* lamu, T* acch, T* accu, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adj_coef_edge, dim3(grid_items(16L * m.nE)), dim3(kBlock), 0, s, m, c, b,
-                       q.e48, q.e8);
-    hipLaunchKernelGGL(k_adj_coef_vertex, dim3(grid_items(8L * m.nV)), dim3(kBlock), 0, s, m, c, b,
-                       q.v8);
-    hipLaunchKernelGGL(k_adj_coef_cell, dim3(grid_items(8L * m.nC)), dim3(kBlock), 0, s, m, c, b,
-                       q.c32);
     // ds.dsv: circulation adjoint of nabla^2 u [V]; ds.dsd: its divergence
     // adjoint [C]; ds.dsu: nabla^2 u adjoint [E]; ds.div: divergence
     // adjoint / area [C]
@@ -1392,6 +1455,16 @@ void adj_seed(int n, T* g, const T* lam, double w3, hipStream_t s)
 {
     hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, static_cast<long>(n), g,
                        lam, T(w3));
+}
+
+template <typename T>
+void reconstruct_adj(const DMesh& m, int lb, const double* coeffs, const double* trig,
+                     const T* seeds5, T* xyz, T* u_ad, hipStream_t s)
+{
+    LAUNCH_WU(k_recon_adj_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m.nC, lb, trig,
+              seeds5, xyz);
+    LAUNCH_WU(k_recon_adj_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, lb, coeffs,
+              xyz, u_ad);
 }
 
 template <typename T>
@@ -1471,6 +1544,8 @@ void nonfinite_flags(long n, int lb, const T* x, int* flags, hipStream_t s)
                                const DissipScratch<T>&, const AdjCoef&, T*, T*, T*, T*,             \
                                hipStream_t);                                                        \
     template void adj_seed<T>(int, T*, const T*, double, hipStream_t);                              \
+    template void reconstruct_adj<T>(const DMesh&, int, const double*, const double*, const T*, T*,  \
+                                     T*, hipStream_t);                                              \
     template void reconstruct<T>(int, int, const int*, const int*, const double*, const double*,    \
                                  const T*, T*, hipStream_t);                                        \
     template void dot_batched<T>(long, int, const T*, const T*, const double*, double*, int,         \

@@ -95,7 +95,11 @@ struct msw_ctx {
     double* bdsd = nullptr;           // [C]
     double* bdsv = nullptr;           // [V]
     double* usrc = nullptr;           // [E]  wind-stress source, internal order
-    AdjCoef adjq{};                   // adjoint stage coefficients (B = 1)
+    AdjCoef adjq{};                   // stage coefficients (B = 1) of the eager single-stream path
+    AdjCoef qs[4]{};                  // per-stage coefficients of the two-stream steps
+    double* st0 = nullptr;            // [C+E] copy of the TLM step-start state (stage 0)
+    hipStream_t side = nullptr;       // base trajectory stream (default priority)
+    hipEvent_t ev[8]{};               // fork, stage-ready x4, join
     bool tl_fused = false;            // MSW_OPT_TL_FUSED_FLUX
     void* me1 = nullptr;              // [E]
     void* me2 = nullptr;              // [E]
@@ -112,6 +116,9 @@ struct msw_ctx {
     {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        if (side) (void)hipStreamDestroy(side);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
         for (auto& b : bufs)
             if (b.p) (void)hipFree(b.p);
     }
@@ -312,6 +319,13 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
         dc[k] = d->dcEdge[o];
         dv[k] = d->dvEdge[o];
     }
+    // slot of each edge in the edgesOnCell list of its two cells
+    std::vector<int> eSlot(2 * static_cast<size_t>(E), 0);
+    for (int k = 0; k < C; ++k)
+        for (int i = 0; i < nEoC[k]; ++i) {
+            const int e = eoc[kMaxEdges * static_cast<size_t>(k) + i];
+            eSlot[2 * static_cast<size_t>(e) + (coe[2 * static_cast<size_t>(e)] == k ? 0 : 1)] = i;
+        }
     std::vector<double> woeT(static_cast<size_t>(E) * kMaxEdges2, 0.0);
     for (size_t k = 0; k < static_cast<size_t>(E); ++k)
         for (int j = 0; j < kMaxEdges2; ++j)
@@ -348,7 +362,7 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
     UP(m.nEoC, nEoC); UP(m.eoc, eoc); UP(m.sgnC, sgnC); UP(m.eocR, eocR); UP(m.sgnCR, sgnCR);
     UP(m.vocR, vocR); UP(m.kiteCR, kiteCR); UP(m.areaC, areaC); UP(m.hs, hs);
     UP(m.coe, coe); UP(m.voe, voe); UP(m.nEoE, nEoE); UP(m.eoe, eoe); UP(m.woe, woe);
-    UP(m.eoeT, eoeT); UP(m.woeT, woeT); UP(m.dc, dc); UP(m.dv, dv);
+    UP(m.eoeT, eoeT); UP(m.woeT, woeT); UP(m.eSlot, eSlot); UP(m.dc, dc); UP(m.dv, dv);
     UP(m.eovR, eovR); UP(m.sgnV, sgnV); UP(m.cov, cov); UP(m.kite, kite); UP(m.areaT, areaT);
     UP(m.fV, fV);
     UP(c->coeffs, coeffs); UP(c->trig, trig);
@@ -394,6 +408,15 @@ int alloc_buffers(msw_ctx* c)
     c->adjq.e8 = q + 48 * E;
     c->adjq.v8 = q + 64 * E;
     c->adjq.c32 = q + 64 * E + 8 * V;
+    for (int st = 0; st < 4; ++st) {
+        double* qq = nullptr;
+        if ((rc = c->alloc(&qq, 64 * E + 8 * V + 32 * C))) return rc;
+        c->qs[st].e48 = qq;
+        c->qs[st].e8 = qq + 48 * E;
+        c->qs[st].v8 = qq + 64 * E;
+        c->qs[st].c32 = qq + 64 * E + 8 * V;
+    }
+    if ((rc = c->alloc(&c->st0, C + E))) return rc;
     c->dot_nblk = dot_blocks(std::max<long>(C, E));
     if ((rc = c->alloc(&c->dpartial, static_cast<size_t>(c->dot_nblk) * B + 2 * B))) return rc;
     return MSW_OK;
@@ -441,12 +464,27 @@ DissipScratch<double> base_ds(msw_ctx* c) { return {c->bdiv, c->bdsu, c->bdsd, c
 
 // base (B = 1) tendency + RK update of one stage, with the dissipation terms
 void base_tend(msw_ctx* c, const BaseStage& b, int s, const RKCoeffs& k, double* xh, double* xu,
-               double* nh)
+               double* nh, hipStream_t st)
 {
-    const Dissip<double> dp = dissip_fields<double>(c->dm, c->dcfg, 0, b.u, b.vort, base_ds(c), c->stream);
+    const Dissip<double> dp = dissip_fields<double>(c->dm, c->dcfg, 0, b.u, b.vort, base_ds(c), st);
     const size_t C = c->nC;
     fwd_tend_update_he<double>(c->dm, c->dcfg, 0, s, k, b.h, b.u, b.he, b.pe, b.ke, dp, xh, xu,
-                               c->bacc, c->bacc + C, nh, nh + C, c->stream);
+                               c->bacc, c->bacc + C, nh, nh + C, st);
+}
+
+// The side stream joins the main stream's work (and a capture in progress).
+int fork_side(msw_ctx* c)
+{
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->side, c->ev[0], 0));
+    return MSW_OK;
+}
+
+int join_side(msw_ctx* c)
+{
+    HIP_TRY(hipEventRecord(c->ev[5], c->side));
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->ev[5], 0));
+    return MSW_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -495,17 +533,21 @@ void enqueue_base_step(msw_ctx* c, double* h, double* u)
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
                          b.gpt, b.gpn, b.rec, c->stream);
         double* nh = s < 3 ? c->st[s + 1].h : c->bend;
-        base_tend(c, b, s, k, h, u, nh);
+        base_tend(c, b, s, k, h, u, nh, c->stream);
     }
     (void)C;
 }
 
+// One TLM step.  The FP64 base trajectory (diagnostics, coefficient rows and
+// RK update of each stage, B = 1) runs on the side stream, a stage ahead of
+// the member kernels on the main stream, which wait only for the base
+// fields of the stage they are about to use.
 template <typename T>
-void enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
+int enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
 {
     const DMesh& m = c->dm;
     const RKCoeffs k = rk_coeffs(c->cfg.dt);
-    const size_t C = c->nC, B = c->B;
+    const size_t C = c->nC, E = c->nE, B = c->B;
     T* acc = static_cast<T*>(c->macc);
     T* pA = static_cast<T*>(c->mpA);
     T* pB = static_cast<T*>(c->mpB);
@@ -514,59 +556,85 @@ void enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
     T* pc_tl = static_cast<T*>(c->mc2);
     T* pe_tl = static_cast<T*>(c->me1);
     T* he_tl = static_cast<T*>(c->me2);
-    c->st[0].h = h;
-    c->st[0].u = u;
-    const T* th = htl;
-    const T* tu = utl;
+    hipStream_t ss = c->side;
+    int rc = fork_side(c);
+    if (rc) return rc;
+    // stage 0 reads a copy, so the base RK update may overwrite (h, u) while
+    // the member kernels of earlier stages still run
+    HIP_TRY(hipMemcpyAsync(c->st0, h, C * sizeof(double), hipMemcpyDeviceToDevice, ss));
+    HIP_TRY(hipMemcpyAsync(c->st0 + C, u, E * sizeof(double), hipMemcpyDeviceToDevice, ss));
+    c->st[0].h = c->st0;
+    c->st[0].u = c->st0 + C;
     for (int s = 0; s < 4; ++s) {
         BaseStage& b = c->st[s];
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
-                         b.gpt, b.gpn, b.rec, c->stream);
-        T* nb = (s % 2 == 0) ? pA : pB;
-        tl_stage<T>(m, c->dcfg, c->lb, s, k, b, th, tu, pvv_tl, ke_tl, pc_tl, pe_tl, he_tl,
-                    static_cast<T*>(c->xvort), member_ds<T>(c), c->adjq, htl, utl, acc, acc + C * B,
-                    nb, nb + C * B, c->stream);
+                         b.gpt, b.gpn, b.rec, ss);
+        tl_coef(m, c->dcfg, b, c->qs[s], ss);
+        HIP_TRY(hipEventRecord(c->ev[1 + s], ss));
         double* nh = s < 3 ? c->st[s + 1].h : c->bend;
-        base_tend(c, b, s, k, h, u, nh);
+        base_tend(c, b, s, k, h, u, nh, ss);
+    }
+    const T* th = htl;
+    const T* tu = utl;
+    for (int s = 0; s < 4; ++s) {
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev[1 + s], 0));
+        T* nb = (s % 2 == 0) ? pA : pB;
+        tl_stage<T>(m, c->dcfg, c->lb, s, k, c->st[s], th, tu, pvv_tl, ke_tl, pc_tl, pe_tl, he_tl,
+                    static_cast<T*>(c->xvort), member_ds<T>(c), c->qs[s], htl, utl, acc,
+                    acc + C * B, nb, nb + C * B, c->stream);
         th = nb;
         tu = nb + C * B;
     }
+    return join_side(c);
 }
 
 // One adjoint step; the base step-start state must already be in c->bx.
+// The side stream recomputes the four base stages, then forms the
+// coefficient rows of stages 3, 2, 1, 0 while the main stream runs the
+// member adjoint stages in that order.
 template <typename T>
-void enqueue_adj_step(msw_ctx* c, T* lamh, T* lamu, bool want_end)
+int enqueue_adj_step(msw_ctx* c, T* lamh, T* lamu, bool want_end)
 {
     const DMesh& m = c->dm;
     const RKCoeffs k = rk_coeffs(c->cfg.dt);
     const size_t C = c->nC, E = c->nE, B = c->B;
     double* xh = c->bx;
     double* xu = c->bx + C;
+    hipStream_t ss = c->side;
+    int rc = fork_side(c);
+    if (rc) return rc;
     c->st[0].h = xh;
     c->st[0].u = xu;
     for (int s = 0; s < 4; ++s) {
         BaseStage& b = c->st[s];
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
-                         b.gpt, b.gpn, b.rec, c->stream);
+                         b.gpt, b.gpn, b.rec, ss);
         if (s < 3 || want_end) {
             // stage 3 writes x_new into its "x" argument: give it the end buffer
             double* nh = s < 3 ? c->st[s + 1].h : c->bend;
             double* wxh = s < 3 ? xh : c->bend;
             double* wxu = s < 3 ? xu : c->bend + C;
-            base_tend(c, b, s, k, wxh, wxu, nh);
+            base_tend(c, b, s, k, wxh, wxu, nh, ss);
         }
+    }
+    for (int s = 3; s >= 0; --s) {
+        adj_coef(m, c->dcfg, c->st[s], c->qs[s], ss);
+        HIP_TRY(hipEventRecord(c->ev[1 + s], ss));
     }
     T* g = static_cast<T*>(c->mpA);
     T* acc = static_cast<T*>(c->macc);
     adj_seed<T>(static_cast<int>(C * B), g, lamh, k.w[3], c->stream);
     adj_seed<T>(static_cast<int>(E * B), g + C * B, lamu, k.w[3], c->stream);
-    for (int s = 3; s >= 0; --s)
+    for (int s = 3; s >= 0; --s) {
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev[1 + s], 0));
         adj_stage<T>(m, c->dcfg, c->lb, s, s == 3, k, c->st[s], g, g + C * B,
                      static_cast<T*>(c->me1), static_cast<T*>(c->me2), static_cast<T*>(c->me3),
                      static_cast<T*>(c->mc1), static_cast<T*>(c->mc2), static_cast<T*>(c->mc3),
                      static_cast<T*>(c->mv1),
-                     static_cast<T*>(c->mv2), member_ds<T>(c), c->adjq, lamh, lamu, acc,
+                     static_cast<T*>(c->mv2), member_ds<T>(c), c->qs[s], lamh, lamu, acc,
                      acc + C * B, c->stream);
+    }
+    return join_side(c);
 }
 
 int check_cfg(msw_ctx* c)
@@ -586,7 +654,8 @@ int run_steps(msw_ctx* c, int kind, const void* p0, const void* p1, const void* 
     if (!c->use_graphs) {
         for (int i = 0; i < nsteps; ++i) {
             if (pre) (*pre)(i);
-            body();
+            const int brc = body();
+            if (brc) return brc;
         }
         HIP_TRY(hipGetLastError());
         return MSW_OK;
@@ -598,12 +667,17 @@ int run_steps(msw_ctx* c, int kind, const void* p0, const void* p1, const void* 
         hipStream_t user = c->stream;
         c->stream = c->cap_stream;
         hipGraph_t graph = nullptr;
+        int brc = MSW_OK;
         hipError_t e = hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            body();
+            brc = body();
             e = hipStreamEndCapture(c->cap_stream, &graph);
         }
         c->stream = user;
+        if (brc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return brc;
+        }
         if (e != hipSuccess) return fail(MSW_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
         hipGraphExec_t exec = nullptr;
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -658,6 +732,8 @@ int msw_create(const msw_mesh_desc* mesh, const msw_config* cfg, int prec, int b
     if (rc) return rc;
     if ((rc = alloc_buffers(c.get()))) return rc;
     if ((rc = c->alloc(&c->usrc, c->nE))) return rc;
+    HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_TRY(hipMemset(c->usrc, 0, static_cast<size_t>(c->nE) * sizeof(double)));
     if ((rc = ensure_dissip(c.get()))) return rc;
     set_dcfg(c.get());
@@ -724,9 +800,9 @@ int msw_fwd_steps(msw_ctx* ctx, void* h, void* u, int nsteps)
     if (rc) return rc;
     if (ctx->prec == MSW_FP64)
         return run_steps(ctx, 1, h, u, nullptr, nullptr, nsteps,
-                         [&] { enqueue_fwd_step<double>(ctx, static_cast<double*>(h), static_cast<double*>(u)); });
+                         [&] { enqueue_fwd_step<double>(ctx, static_cast<double*>(h), static_cast<double*>(u)); return MSW_OK; });
     return run_steps(ctx, 2, h, u, nullptr, nullptr, nsteps,
-                     [&] { enqueue_fwd_step<float>(ctx, static_cast<float*>(h), static_cast<float*>(u)); });
+                     [&] { enqueue_fwd_step<float>(ctx, static_cast<float*>(h), static_cast<float*>(u)); return MSW_OK; });
 }
 
 int msw_base_steps(msw_ctx* ctx, double* h, double* u, int nsteps, double* traj_h, double* traj_u)
@@ -739,7 +815,7 @@ int msw_base_steps(msw_ctx* ctx, double* h, double* u, int nsteps, double* traj_
         (void)hipMemcpyAsync(traj_h + i * C, h, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
         (void)hipMemcpyAsync(traj_u + i * E, u, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
     };
-    return run_steps(ctx, 3, h, u, nullptr, nullptr, nsteps, [&] { enqueue_base_step(ctx, h, u); },
+    return run_steps(ctx, 3, h, u, nullptr, nullptr, nsteps, [&] { enqueue_base_step(ctx, h, u); return MSW_OK; },
                      traj_h ? &pre : nullptr);
 }
 
@@ -758,10 +834,10 @@ int msw_tlm_steps(msw_ctx* ctx, double* h, double* u, void* h_tl, void* u_tl, in
     const std::function<void(int)>* pp = traj_h ? &pre : nullptr;
     if (ctx->prec == MSW_FP64)
         return run_steps(ctx, 4, h, u, h_tl, u_tl, nsteps, [&] {
-            enqueue_tlm_step<double>(ctx, h, u, static_cast<double*>(h_tl), static_cast<double*>(u_tl));
+            return enqueue_tlm_step<double>(ctx, h, u, static_cast<double*>(h_tl), static_cast<double*>(u_tl));
         }, pp);
     return run_steps(ctx, 5, h, u, h_tl, u_tl, nsteps, [&] {
-        enqueue_tlm_step<float>(ctx, h, u, static_cast<float*>(h_tl), static_cast<float*>(u_tl));
+        return enqueue_tlm_step<float>(ctx, h, u, static_cast<float*>(h_tl), static_cast<float*>(u_tl));
     }, pp);
 }
 
@@ -771,10 +847,10 @@ static int adj_common(msw_ctx* ctx, void* h_ad, void* u_ad, int nsteps, bool wan
     if (ctx->dcfg.frozen_u) return fail(MSW_EUNSUPPORTED, "adjoint of test case 1 is not implemented");
     if (ctx->prec == MSW_FP64)
         return run_steps(ctx, want_end ? 6 : 7, h_ad, u_ad, nullptr, nullptr, nsteps, [&] {
-            enqueue_adj_step<double>(ctx, static_cast<double*>(h_ad), static_cast<double*>(u_ad), want_end);
+            return enqueue_adj_step<double>(ctx, static_cast<double*>(h_ad), static_cast<double*>(u_ad), want_end);
         }, &pre);
     return run_steps(ctx, want_end ? 8 : 9, h_ad, u_ad, nullptr, nullptr, nsteps, [&] {
-        enqueue_adj_step<float>(ctx, static_cast<float*>(h_ad), static_cast<float*>(u_ad), want_end);
+        return enqueue_adj_step<float>(ctx, static_cast<float*>(h_ad), static_cast<float*>(u_ad), want_end);
     }, &pre);
 }
 
@@ -853,6 +929,22 @@ int msw_reconstruct(msw_ctx* ctx, const void* u, void* out5)
     else
         reconstruct<float>(ctx->nC, ctx->lb, ctx->dm.nEoC, ctx->dm.eoc, ctx->coeffs, ctx->trig,
                            static_cast<const float*>(u), static_cast<float*>(out5), ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return MSW_OK;
+}
+
+int msw_reconstruct_adj(msw_ctx* ctx, const void* seeds5, void* u_ad)
+{
+    if (!ctx || !seeds5 || !u_ad) return fail(MSW_EINVAL, "bad argument");
+    // scratch: the member C-fields mc1..mc3 are contiguous in the member block
+    if (ctx->prec == MSW_FP64)
+        reconstruct_adj<double>(ctx->dm, ctx->lb, ctx->coeffs, ctx->trig,
+                                static_cast<const double*>(seeds5), static_cast<double*>(ctx->mc1),
+                                static_cast<double*>(u_ad), ctx->stream);
+    else
+        reconstruct_adj<float>(ctx->dm, ctx->lb, ctx->coeffs, ctx->trig,
+                               static_cast<const float*>(seeds5), static_cast<float*>(ctx->mc1),
+                               static_cast<float*>(u_ad), ctx->stream);
     HIP_TRY(hipGetLastError());
     return MSW_OK;
 }

@@ -423,13 +423,23 @@ class _Module:
         A = self._arrays
         if "u_ad" not in A:
             raise RuntimeError("var_allocation_adj() has not been called")
-        for n in _DIAG + _RECON:
+        for n in _DIAG:
             if np.any(A[n + "_ad"] != 0):
                 raise NotImplementedError(f"a non-zero seed in {n}_ad is not supported on the GPU path")
         h = self._up(m, A["h"][0, 0], "c")
         u = self._up(m, A["u"][0, 0], "e")
         ha = self._up(m, A["h_ad"][0, 0], "c")
         ua = self._up(m, A["u_ad"][0, 0], "e")
+        if any(np.any(A[n + "_ad"] != 0) for n in _RECON):
+            # mpas_reconstruct_2d_adj on the end-state adjoint, which consumes
+            # the seeds (src/mpas_sw_module.f90:887-892)
+            import torch
+            seeds = torch.stack([m.to_internal(A[n + "_ad"][0], "c") for n in _RECON])
+            ua = ua.to(m.dtype)
+            m.reconstruct_adj(seeds, ua)
+            ua = ua.to(torch.float64)
+            for n in _RECON:
+                A[n + "_ad"][...] = 0.0
         he, ue = m.adj_step(h, u, ha, ua, want_end=True)
         A["h_ad"][0, 0, :] = m.to_reference(ha, "c")
         A["u_ad"][0, 0, :] = m.to_reference(ua, "e")

@@ -282,6 +282,16 @@ class SWModel:
         _lib.check(self.lib.msw_reconstruct(self._h, _ptr(u), _ptr(out)))
         return out
 
+    def reconstruct_adj(self, seeds, u_ad):
+        """u_ad += R^T seeds in place; seeds [5, C(, B)] = adjoints of X, Y, Z,
+        zonal, meridional (the order `reconstruct` returns)."""
+        self._check(u_ad, "e", name="u_ad")
+        seeds = seeds.to(self.dtype).contiguous()
+        if tuple(seeds.shape) != (5,) + self._shape("c"):
+            raise ValueError(f"seeds must have shape {(5,) + self._shape('c')}")
+        _lib.check(self.lib.msw_reconstruct_adj(self._h, _ptr(seeds), _ptr(u_ad)))
+        return u_ad
+
     def dot(self, xh=None, xu=None, yh=None, yu=None, wh=None, wu=None):
         """Per-member FP64 inner products (see msw_dot); returns float64 [B]."""
         out = torch.empty(self.batch, dtype=torch.float64, device=self.device)

@@ -29,7 +29,7 @@ def available() -> bool:
 
 def write_input(path, mesh, test_case, dt, apvm=0.5, visc2=0.0, visc4=0.0, adv_order=2,
                 monotonic=False, wind_stress=False, bottom_drag=False,
-                state=None, tl=None, ad=None, u_src=None):
+                state=None, tl=None, ad=None, u_src=None, recon_seeds=None):
     hdr = np.zeros(16, np.int32)
     hdr[0] = MAGIC
     hdr[1] = 1
@@ -45,6 +45,7 @@ def write_input(path, mesh, test_case, dt, apvm=0.5, visc2=0.0, visc4=0.0, adv_o
     cfg = np.zeros(8, np.float64)
     cfg[:4] = [dt, apvm, visc2, visc4]
     cfg[4] = 1.0 if u_src is not None else 0.0
+    cfg[5] = 1.0 if recon_seeds is not None else 0.0
     f8 = lambda x: np.asarray(x, np.float64).ravel(order="C")
     i4 = lambda x: np.asarray(x, np.int32).ravel(order="C")
     with open(path, "wb") as f:
@@ -73,6 +74,8 @@ def write_input(path, mesh, test_case, dt, apvm=0.5, visc2=0.0, visc4=0.0, adv_o
                 f.write(f8(pair[1]).tobytes())
         if u_src is not None:
             f.write(f8(u_src).tobytes())
+        if recon_seeds is not None:   # (5, C): X, Y, Z, zonal, meridional adjoint seeds
+            f.write(f8(np.asarray(recon_seeds, np.float64).reshape(5, -1)).tobytes())
 
 
 def read_output(path):

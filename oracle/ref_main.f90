@@ -19,7 +19,7 @@ program msw_ref
   real(kind=RKIND) :: cfg(8)
   integer :: has_state, has_tl, has_ad
   real(kind=RKIND), allocatable :: st_h(:), st_u(:), tl_h(:), tl_u(:), ad_h(:), ad_u(:)
-  real(kind=RKIND), allocatable :: traj_h(:,:), traj_u(:,:), usrc_in(:)
+  real(kind=RKIND), allocatable :: traj_h(:,:), traj_u(:,:), usrc_in(:), rseed(:,:)
   integer(8) :: t0, t1, rate
   integer, parameter :: uin = 21, uout = 22
 
@@ -81,6 +81,13 @@ program msw_ref
      allocate(usrc_in(nEdges))
      read(uin) usrc_in
   end if
+  ! cfg(6) /= 0: seeds of the reconstructed-velocity adjoints follow
+  ! (uReconstructX/Y/Z/Zonal/Meridional_ad, src/mpas_sw_module.f90:168-172),
+  ! consumed by the first sw_rk4_adj
+  if (cfg(6) /= 0.0) then
+     allocate(rseed(nCells, 5))
+     read(uin) rseed
+  end if
   close(uin)
 
   ! ---- initial conditions, as initial_conditions does
@@ -104,6 +111,13 @@ program msw_ref
   call var_allocation_tlm()
   call var_allocation_adj()
   call sw_mpas_init_block()
+  if (allocated(rseed)) then
+     uReconstructX_ad(1,:) = rseed(:,1)
+     uReconstructY_ad(1,:) = rseed(:,2)
+     uReconstructZ_ad(1,:) = rseed(:,3)
+     uReconstructZonal_ad(1,:) = rseed(:,4)
+     uReconstructMeridional_ad(1,:) = rseed(:,5)
+  end if
 
   open(uout, file=trim(fout), access='stream', form='unformatted', status='replace')
 

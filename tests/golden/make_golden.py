@@ -67,11 +67,26 @@ def dissipation_case():
          h_ad=a["h_ad"], u_ad=a["u_ad"], gh=gh, gu=gu, tad_h_ad=tad["h_ad"], tad_u_ad=tad["u_ad"])
 
 
+def recon_adjoint_case():
+    """Level 3, TC5, dt 900 s: two adjoint steps whose only seeds are the
+    reconstructed-velocity adjoints (uReconstruct{X,Y,Z,Zonal,Meridional}_ad),
+    which the first sw_rk4_adj consumes."""
+    m3 = get_mesh(3)
+    tc, dt, n = 5, 900.0, 2
+    rng = np.random.default_rng(SEED + 5)
+    seeds = rng.standard_normal((5, m3.nCells))
+    zero = (np.zeros(m3.nCells), np.zeros(m3.nEdges))
+    ic = ref.run(m3, "ic", test_case=tc, dt=dt)
+    a = ref.run(m3, "adj", nsteps=n, test_case=tc, dt=dt, ad=zero, recon_seeds=seeds)
+    save("tc5_l3_recon_adjoint", m3, dict(test_case=tc, dt=dt, nsteps=n), h0=ic["h"], u0=ic["u"],
+         seeds=seeds, h_ad=a["h_ad"], u_ad=a["u_ad"])
+
+
 def main(names):
     assert ref.available(), "build the reference first: make -C oracle ref"
     if names:
         for n in names:
-            {"tc5_l3_dissipation": dissipation_case}[n]()
+            {"tc5_l3_dissipation": dissipation_case, "tc5_l3_recon_adjoint": recon_adjoint_case}[n]()
         return
 
     # 1. Williamson TC2, level 5, dt = 300 s, 288 steps (one day)
@@ -124,6 +139,9 @@ def main(names):
 
     # 5. dissipation and forcing branches
     dissipation_case()
+
+    # 6. reconstructed-velocity adjoint seeds
+    recon_adjoint_case()
 
 
 if __name__ == "__main__":

@@ -127,3 +127,27 @@ def test_dissipation_and_forcing_bitwise():
     ha, ua = o.tend_full_adj(g["h0"], g["u0"], g["gh"], g["gu"])
     np.testing.assert_array_equal(ha, g["tad_h_ad"])
     np.testing.assert_array_equal(ua, g["tad_u_ad"])
+
+
+def test_reconstruction_adjoint_seeds_bitwise():
+    """Seeds in uReconstruct{X,Y,Z,Zonal,Meridional}_ad are applied to the
+    end-state adjoint by the first adjoint step and consumed."""
+    g, meta = golden("tc5_l3_recon_adjoint")
+    n, dt = meta["nsteps"], meta["dt"]
+    assert np.abs(g["u_ad"]).max() > 0
+    o = _oracle_from(3, 5, dt, g["h0"], g["u0"])
+    traj = []
+    for _ in range(n):
+        traj.append((o["h1"].copy(), o["u1"].copy()))
+        o.rk4()
+    o["h_ad1"][:] = 0.0
+    o["u_ad1"][:] = 0.0
+    for k, name in enumerate(("X", "Y", "Z", "Zonal", "Meridional")):
+        o[f"uReconstruct{name}_ad"][:] = g["seeds"][k]
+    for s in range(n - 1, -1, -1):
+        o["h1"][:] = traj[s][0]
+        o["u1"][:] = traj[s][1]
+        o.rk4_adj()
+    np.testing.assert_array_equal(o["h_ad1"], g["h_ad"])
+    np.testing.assert_array_equal(o["u_ad1"], g["u_ad"])
+    assert not np.any(o["uReconstructZonal_ad"])
