@@ -150,6 +150,13 @@ int msw_diagnostics(msw_ctx *ctx, const double *h, const double *u, double *out)
 const char *msw_diag_names(void);
 const char *msw_diag_kinds(void);
 
+/* Global FP64 invariants of one FP64 state (B = 1), deterministic
+ * reductions; no counterpart in the reference's step (its drivers print
+ * numpy sums only).  out: device double[2] = { mass sum_c A_c h_c,
+ * total energy sum_c A_c (g h_c (h_c/2 + h_s,c) + h_c ke_c) } with ke the
+ * kinetic energy of src/mpas_sw_dyncore.f90:245-256. */
+int msw_invariants(msw_ctx *ctx, const double *h, const double *u, double *out);
+
 /* Replaces: mpas_reconstruct_2d (src/mpas_sw_operators.f90:283-370) for the
  * batch: out5 = [X, Y, Z, zonal, meridional], each [C][B] of `prec`. */
 int msw_reconstruct(msw_ctx *ctx, const void *u, void *out5);

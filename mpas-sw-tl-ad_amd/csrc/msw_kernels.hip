@@ -13,9 +13,8 @@
 // arithmetic, so a wavefront keeps many loads in flight instead of walking a
 // chain of dependent loads.  Padded terms add exact zeros.
 //
-// Blocks are mapped XCD-aware: the hardware deals block i to XCD i % 8, and
-// xcd_block() turns that into one contiguous range of the space-filling-curve
-// order per XCD, so the neighbours a gather touches sit in the same L2.
+// Blocks are mapped XCD-aware (xcd_block(): stripes of consecutive curve
+// positions dealt to the XCDs in turn).
 //
 // All gathers visit neighbours in the order the reference's scatter loops
 // add them (the "R" tables of DMesh) and every expression keeps the
@@ -41,29 +40,40 @@ __device__ __forceinline__ int entity_of(unsigned t, int lb)
 }
 
 // Blocks are dealt round-robin to the 8 XCDs (block i runs on XCD i % 8).
-// The launch grid is 8 * per blocks and XCD x takes the logical blocks
-// [x * per, (x + 1) * per), one contiguous stretch of the curve order.
+// Logical blocks are grouped in stripes of kStripe consecutive blocks (16
+// entities at B = 64) and the stripes are dealt to the XCDs in turn: each
+// XCD's L2 sees short runs of consecutive entities along the curve, and the
+// rows that neighbouring stripes share on different XCDs are caught by the
+// memory-side cache all XCDs share.  Measured against one contiguous range
+// per XCD and against plain block order, stripes of 4 were fastest (FP64,
+// B = 64, level 7: 12.68 vs 13.45 and 13.14 ms per TLM+ADJ step).
 // Every thread handles one item (one member of one entity); all index
 // arithmetic is 32-bit (items < 2^31 is checked when a context is made).
-__device__ __forceinline__ unsigned xcd_block()
+constexpr unsigned kStripe = 4;
+// logical block of hardware block i (< a multiple of 8*kStripe): stripes of
+// kStripe consecutive logical blocks are dealt to the 8 XCDs in turn
+__device__ __forceinline__ unsigned stripe_block(unsigned i)
 {
-    return (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const unsigned x = i & 7u, j = i >> 3;
+    return (j / kStripe) * (8u * kStripe) + x * kStripe + (j % kStripe);
 }
+__device__ __forceinline__ unsigned xcd_block() { return stripe_block(blockIdx.x); }
 
-// blocks for `items` threads, rounded up to a multiple of 8
+__device__ __forceinline__ unsigned round_blocks_dev(unsigned nb) { const unsigned q = 8u * kStripe; return ((nb + q - 1) / q) * q; }
+inline long round_blocks(long nb) { const long q = 8L * kStripe; return ((nb + q - 1) / q) * q; }
+
+// blocks for `items` threads, rounded up to a multiple of 8*kStripe
 inline int grid_items(long items)
 {
-    const long per = (items + 8L * kBlock - 1) / (8L * kBlock);
-    return static_cast<int>(8 * (per > 0 ? per : 1));
+    const long nb = (items + kBlock - 1) / kBlock;
+    return static_cast<int>(round_blocks(nb > 0 ? nb : 1));
 }
 
-// grid for the fused edge + cell kernels: each XCD takes one eighth of the
-// edges and the matching eighth of the cells
 inline int grid_edge_cell(int nE, int nC, int lb)
 {
-    const long pe = ((static_cast<long>(nE) << lb) + 8L * kBlock - 1) / (8L * kBlock);
-    const long pc = ((static_cast<long>(nC) << lb) + 8L * kBlock - 1) / (8L * kBlock);
-    return static_cast<int>(8 * (pe + pc));
+    const long be = ((static_cast<long>(nE) << lb) + kBlock - 1) / kBlock;
+    const long bc = ((static_cast<long>(nC) << lb) + kBlock - 1) / kBlock;
+    return static_cast<int>(round_blocks(be) + round_blocks(bc));
 }
 
 // grid-stride form for the elementwise kernels (any grid)
@@ -82,15 +92,14 @@ inline int grid_for(long threads)
     const unsigned t = xcd_block() * kBlock + threadIdx.x;                     \
     if (t >= total_) return;
 
-// one item per thread over the edges (is_edge) or the cells
+// one item per thread over the edges (is_edge: the first round_blocks(nE
+// items) blocks) or the cells (the rest)
 #define EDGE_CELL_ITEM(is_edge, t)                                             \
     const unsigned tE_ = static_cast<unsigned>(m.nE) << lb;                    \
     const unsigned tC_ = static_cast<unsigned>(m.nC) << lb;                    \
-    const unsigned pE_ = (tE_ + 8u * kBlock - 1) / (8u * kBlock);              \
-    const unsigned pC_ = (tC_ + 8u * kBlock - 1) / (8u * kBlock);              \
-    const unsigned x_ = blockIdx.x & 7u, j_ = blockIdx.x >> 3;                 \
-    const bool is_edge = j_ < pE_;                                             \
-    const unsigned t = (is_edge ? x_ * pE_ + j_ : x_ * pC_ + (j_ - pE_)) * kBlock + threadIdx.x; \
+    const unsigned gE_ = static_cast<unsigned>(round_blocks_dev((tE_ + kBlock - 1) / kBlock)); \
+    const bool is_edge = blockIdx.x < gE_;                                     \
+    const unsigned t = stripe_block(is_edge ? blockIdx.x : blockIdx.x - gE_) * kBlock + threadIdx.x; \
     if (t >= (is_edge ? tE_ : tC_)) return;
 
 // one flat item per thread (no member axis), blocks dealt XCD-aware

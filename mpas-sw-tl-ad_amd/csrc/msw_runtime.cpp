@@ -104,6 +104,7 @@ struct msw_ctx {
     void* me1 = nullptr;              // [E]
     void* me2 = nullptr;              // [E]
     void* me3 = nullptr;              // [E]
+    double* inv = nullptr;            // [2C + 3V + 2E + 2C] invariants scratch, made on first use
     double* dpartial = nullptr;       // dot partials
     int dot_nblk = 0;
 
@@ -916,6 +917,39 @@ int msw_diagnostics(msw_ctx* ctx, const double* h, const double* u, double* out)
     double* gpt = o; o += E;
     fwd_diag<double>(ctx->dm, ctx->dcfg, 0, h, u, pvv, hv, vort, ke, pc, he, pe, v, gpt, gpn, nullptr, ctx->stream);
     diag_extra(ctx->dm, u, vort, circ, div, vc, ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return MSW_OK;
+}
+
+int msw_invariants(msw_ctx* ctx, const double* h, const double* u, double* out)
+{
+    if (!ctx || !h || !u || !out) return fail(MSW_EINVAL, "bad argument");
+    const size_t C = ctx->nC, E = ctx->nE, V = ctx->nV;
+    const DMesh& m = ctx->dm;
+    hipStream_t s = ctx->stream;
+    if (!ctx->inv) {
+        int rc = ctx->alloc(&ctx->inv, 4 * C + 3 * V + 2 * E);
+        if (rc) return rc;
+        // weights 0.5*g*A and g*A of the potential-energy terms
+        adj_seed<double>(static_cast<int>(C), ctx->inv + 2 * C + 3 * V + 2 * E, m.areaC, 0.5 * kGravity, s);
+        adj_seed<double>(static_cast<int>(C), ctx->inv + 3 * C + 3 * V + 2 * E, m.areaC, kGravity, s);
+    }
+    double* ke = ctx->inv;
+    double* pc = ke + C;
+    double* pvv = pc + C;
+    double* hv = pvv + V;
+    double* vort = hv + V;
+    double* he = vort + V;
+    double* pe = he + E;
+    const double* wE = pe + E;
+    const double* wP = wE + C;
+    fwd_diag<double>(m, ctx->dcfg, 0, h, u, pvv, hv, vort, ke, pc, he, pe, nullptr, nullptr, nullptr,
+                     nullptr, s);
+    const int nb = ctx->dot_nblk;
+    dot_batched<double>(static_cast<long>(C), 0, h, nullptr, m.areaC, ctx->dpartial, nb, out, 0, s);
+    dot_batched<double>(static_cast<long>(C), 0, h, h, wE, ctx->dpartial, nb, out + 1, 0, s);
+    dot_batched<double>(static_cast<long>(C), 0, h, m.hs, wP, ctx->dpartial, nb, out + 1, 1, s);
+    dot_batched<double>(static_cast<long>(C), 0, h, ke, m.areaC, ctx->dpartial, nb, out + 1, 1, s);
     HIP_TRY(hipGetLastError());
     return MSW_OK;
 }

@@ -282,6 +282,14 @@ class SWModel:
         _lib.check(self.lib.msw_reconstruct(self._h, _ptr(u), _ptr(out)))
         return out
 
+    def invariants(self, h, u):
+        """FP64 [mass, total energy] of one FP64 state (see msw_invariants)."""
+        self._check(h, "c", member=False, name="h")
+        self._check(u, "e", member=False, name="u")
+        out = torch.empty(2, dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.msw_invariants(self._h, _ptr(h), _ptr(u), _ptr(out)))
+        return out
+
     def reconstruct_adj(self, seeds, u_ad):
         """u_ad += R^T seeds in place; seeds [5, C(, B)] = adjoints of X, Y, Z,
         zonal, meridional (the order `reconstruct` returns)."""

@@ -438,3 +438,27 @@ def test_reconstruction_adjoint_seeds_match_oracle():
             o.rk4_adj()
         assert rel_l2(H[:, b], o["h_ad1"]) < 1e-11
         assert rel_l2(U[:, b], o["u_ad1"]) < 1e-11
+
+
+def test_invariants_match_oracle_and_mass_is_conserved():
+    """FP64 mass and total energy as device reductions, against the same sums
+    formed in numpy from the oracle's state and kinetic energy."""
+    print("invariants L3 tc5", flush=True)
+    tc, dt = 5, 900.0
+    mesh, phys = case(LEVEL, tc)
+    o = make_oracle(LEVEL, tc, dt)
+    o.rk4(8)
+    o.diag(o["h1"].copy(), o["u1"].copy())
+    g = 9.80616
+    A, hs, h = np.asarray(phys["areaCell"]), np.asarray(phys["h_s"]), o["h1"]
+    mass = np.sum(A * h)
+    energy = np.sum(A * (g * h * (h / 2 + hs) + h * o["ke"]))
+    m = model(LEVEL, tc, dt)
+    hd = m.to_internal(phys["h"], "c", member=False)
+    ud = m.to_internal(phys["u"], "e", member=False)
+    m0 = m.invariants(hd, ud).cpu().numpy()
+    m.base(hd, ud, 8)
+    inv = m.invariants(hd, ud).cpu().numpy()
+    assert abs(inv[0] - mass) <= 1e-13 * abs(mass)
+    assert abs(inv[1] - energy) <= 1e-13 * abs(energy)
+    assert abs(inv[0] - m0[0]) <= 1e-13 * abs(m0[0])
