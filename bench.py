@@ -90,6 +90,8 @@ def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True, f
     sel = slice(0, batch)
     ht0 = m.to_internal(dh[:, sel] if batch > 1 else dh[:, 0], "c")
     ut0 = m.to_internal(du[:, sel] if batch > 1 else du[:, 0], "e")
+    tune = m.autotune(h0, u0)     # launch stripe per XCD, chosen by timing (not timed here)
+    log(f"stripe autotune B={batch} {precision}: {tune}")
     n = max(steps, warmup)
     bufs = (torch.empty_like(h0), torch.empty_like(u0), torch.empty_like(ht0), torch.empty_like(ut0),
             torch.empty((n, m.nC), dtype=torch.float64, device=m.device),
@@ -103,7 +105,7 @@ def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True, f
     torch.cuda.empty_cache()
     T = t_tlm + t_adj
     return dict(seconds=T, wall=wall, t_tlm=t_tlm, t_adj=t_adj,
-                rate=mesh.nCells * batch * steps / T, finite=ok)
+                rate=mesh.nCells * batch * steps / T, finite=ok, stripe_log2=min(tune, key=tune.get))
 
 
 def calibration_copy(nbytes=1 << 30, reps=3):
@@ -292,7 +294,8 @@ def main():
         "config": {"workload": "batched TLM+ADJ RK4, level-7 icosahedral Voronoi mesh, TC6, dt 75 s",
                    "cells": mesh.nCells, "edges": mesh.nEdges, "vertices": mesh.nVertices,
                    "batch": BATCH, "dt": DT, "window_steps": K,
-                   "tl_tendency": "fused TL flux (MSW_OPT_TL_FUSED_FLUX)"},
+                   "tl_tendency": "fused TL flux (MSW_OPT_TL_FUSED_FLUX)",
+                   "xcd_stripe_log2": res["stripe_log2"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp64_reference_term_order": None if exact is None else {

@@ -102,14 +102,19 @@ int msw_set_forcing(msw_ctx *ctx, const double *u_src);
 /* Use a captured hipGraph per step for the step functions (default off). */
 int msw_set_graphs(msw_ctx *ctx, int enable);
 
-/* Numerical options without a counterpart in the reference.
+/* Options without a counterpart in the reference.
  * MSW_OPT_TL_FUSED_FLUX (default 0): 1 forms the TL tendency of the TLM step
  * (src/mpas_sw_dyncore_tlm.f90:541-595) from the TL thickness flux
  * u_tl*h_edge + u*h_edge_tl at each edge, regrouping the reference's sums so
  * each member gathers two fields over edgesOnEdge instead of three; results
  * then differ from the reference in rounding only.  0 keeps the reference's
- * term order (bit-identical FP64 TLM). */
-enum msw_option { MSW_OPT_TL_FUSED_FLUX = 1 };
+ * term order (bit-identical FP64 TLM).
+ *
+ * MSW_OPT_XCD_STRIPE (default 2): log2 of the number of consecutive blocks of
+ * a launch that go to one XCD before the next XCD's turn (0..6).  Results do
+ * not depend on it; the fastest value depends on mesh, batch and precision
+ * (the Python layer's SWModel.autotune times the candidates). */
+enum msw_option { MSW_OPT_TL_FUSED_FLUX = 1, MSW_OPT_XCD_STRIPE = 2 };
 int msw_set_option(msw_ctx *ctx, int option, int value);
 
 /* Replaces: sw_rk4 (src/mpas_sw_module.f90:528-635), nsteps times, on a

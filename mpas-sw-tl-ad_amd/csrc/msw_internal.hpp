@@ -24,6 +24,7 @@ constexpr double kDragCoef = 1.0e-3;   // bottom drag, src/mpas_sw_dyncore.f90:6
 // or the edge itself) and a zero sign/weight/kite, so padded terms are 0.
 struct DMesh {
     int nC, nE, nV;
+    int stripe_log2;      // blocks are dealt to the 8 XCDs in stripes of 2^stripe_log2
     // cells
     const int* nEoC;      // [C]
     const int* eoc;       // [C][6]  edgesOnCell, slot order

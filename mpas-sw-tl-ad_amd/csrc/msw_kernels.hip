@@ -13,7 +13,7 @@
 // arithmetic, so a wavefront keeps many loads in flight instead of walking a
 // chain of dependent loads.  Padded terms add exact zeros.
 //
-// Blocks are mapped XCD-aware (xcd_block(): stripes of consecutive curve
+// Blocks are mapped XCD-aware (stripe_block(): stripes of consecutive curve
 // positions dealt to the XCDs in turn).
 //
 // All gathers visit neighbours in the order the reference's scatter loops
@@ -40,8 +40,8 @@ __device__ __forceinline__ int entity_of(unsigned t, int lb)
 }
 
 // Blocks are dealt round-robin to the 8 XCDs (block i runs on XCD i % 8).
-// Logical blocks are grouped in stripes of kStripe consecutive blocks (16
-// entities at B = 64) and the stripes are dealt to the XCDs in turn: each
+// Logical blocks are grouped in stripes of consecutive blocks (4 blocks = 16
+// entities at B = 64 by default) and the stripes are dealt to the XCDs in turn: each
 // XCD's L2 sees short runs of consecutive entities along the curve, and the
 // rows that neighbouring stripes share on different XCDs are caught by the
 // memory-side cache all XCDs share.  Measured against one contiguous range
@@ -49,31 +49,39 @@ __device__ __forceinline__ int entity_of(unsigned t, int lb)
 // B = 64, level 7: 12.68 vs 13.45 and 13.14 ms per TLM+ADJ step).
 // Every thread handles one item (one member of one entity); all index
 // arithmetic is 32-bit (items < 2^31 is checked when a context is made).
-constexpr unsigned kStripe = 4;
-// logical block of hardware block i (< a multiple of 8*kStripe): stripes of
-// kStripe consecutive logical blocks are dealt to the 8 XCDs in turn
-__device__ __forceinline__ unsigned stripe_block(unsigned i)
+// Stripe length 2^sl blocks (DMesh::stripe_log2, chosen per context; the
+// autotuner of the Python layer times the candidates).  Hardware block i
+// (i below a multiple of 8 * 2^sl) runs logical block stripe_block(i, sl).
+__device__ __forceinline__ unsigned stripe_block(unsigned i, int sl)
 {
     const unsigned x = i & 7u, j = i >> 3;
-    return (j / kStripe) * (8u * kStripe) + x * kStripe + (j % kStripe);
+    return ((j >> sl) << (3 + sl)) + (x << sl) + (j & ((1u << sl) - 1u));
 }
-__device__ __forceinline__ unsigned xcd_block() { return stripe_block(blockIdx.x); }
 
-__device__ __forceinline__ unsigned round_blocks_dev(unsigned nb) { const unsigned q = 8u * kStripe; return ((nb + q - 1) / q) * q; }
-inline long round_blocks(long nb) { const long q = 8L * kStripe; return ((nb + q - 1) / q) * q; }
+__device__ __forceinline__ unsigned round_blocks_dev(unsigned nb, int sl)
+{
+    const unsigned q = 8u << sl;
+    return ((nb + q - 1) / q) * q;
+}
 
-// blocks for `items` threads, rounded up to a multiple of 8*kStripe
-inline int grid_items(long items)
+inline long round_blocks(long nb, int sl)
+{
+    const long q = 8L << sl;
+    return ((nb + q - 1) / q) * q;
+}
+
+// blocks for `items` threads, rounded up to a multiple of 8 * 2^sl
+inline int grid_items(long items, int sl)
 {
     const long nb = (items + kBlock - 1) / kBlock;
-    return static_cast<int>(round_blocks(nb > 0 ? nb : 1));
+    return static_cast<int>(round_blocks(nb > 0 ? nb : 1, sl));
 }
 
-inline int grid_edge_cell(int nE, int nC, int lb)
+inline int grid_edge_cell(int nE, int nC, int lb, int sl)
 {
     const long be = ((static_cast<long>(nE) << lb) + kBlock - 1) / kBlock;
     const long bc = ((static_cast<long>(nC) << lb) + kBlock - 1) / kBlock;
-    return static_cast<int>(round_blocks(be) + round_blocks(bc));
+    return static_cast<int>(round_blocks(be, sl) + round_blocks(bc, sl));
 }
 
 // grid-stride form for the elementwise kernels (any grid)
@@ -89,7 +97,7 @@ inline int grid_for(long threads)
 // one item per thread over `count` entities
 #define ENTITY_ITEM(count)                                                     \
     const unsigned total_ = static_cast<unsigned>(count) << lb;                \
-    const unsigned t = xcd_block() * kBlock + threadIdx.x;                     \
+    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * kBlock + threadIdx.x; \
     if (t >= total_) return;
 
 // one item per thread over the edges (is_edge: the first round_blocks(nE
@@ -97,14 +105,14 @@ inline int grid_for(long threads)
 #define EDGE_CELL_ITEM(is_edge, t)                                             \
     const unsigned tE_ = static_cast<unsigned>(m.nE) << lb;                    \
     const unsigned tC_ = static_cast<unsigned>(m.nC) << lb;                    \
-    const unsigned gE_ = static_cast<unsigned>(round_blocks_dev((tE_ + kBlock - 1) / kBlock)); \
+    const unsigned gE_ = round_blocks_dev((tE_ + kBlock - 1) / kBlock, m.stripe_log2); \
     const bool is_edge = blockIdx.x < gE_;                                     \
-    const unsigned t = stripe_block(is_edge ? blockIdx.x : blockIdx.x - gE_) * kBlock + threadIdx.x; \
+    const unsigned t = stripe_block(is_edge ? blockIdx.x : blockIdx.x - gE_, m.stripe_log2) * kBlock + threadIdx.x; \
     if (t >= (is_edge ? tE_ : tC_)) return;
 
 // one flat item per thread (no member axis), blocks dealt XCD-aware
 #define FLAT_ITEM(count)                                                       \
-    const unsigned t = xcd_block() * kBlock + threadIdx.x;                     \
+    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * kBlock + threadIdx.x; \
     if (t >= static_cast<unsigned>(count)) return;
 
 #define GRID_LOOP(total)                                                               \
@@ -761,7 +769,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_coef_edge(DMesh m, DConfig c, Ba
 {
     // 16 lanes per edge (all of them active through the reduction below):
     // lanes 0-11 fill the edgesOnEdge slots, lanes 12-15 the per-edge values
-    const unsigned t = xcd_block() * kBlock + threadIdx.x;
+    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * kBlock + threadIdx.x;
     const unsigned total = 16u * static_cast<unsigned>(m.nE);
     if (t >= total) return;   // whole 16-lane groups only, so the shuffles stay inside live lanes
     const int e = static_cast<int>(t >> 4), j = static_cast<int>(t & 15);
@@ -1194,14 +1202,14 @@ __global__ __launch_bounds__(kBlock) void k_reconstruct(int nC, int lb, const in
 // zonal/meridional seeds are rotated back onto X, Y, Z per cell, then each
 // edge gathers coeffs . (X, Y, Z) from its two cells.
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_recon_adj_cell(int nC, int lb,
+__global__ __launch_bounds__(kBlock) void k_recon_adj_cell(DMesh m, int lb,
                                                            const double* __restrict__ trig,
                                                            const T* __restrict__ seeds,
                                                            T* __restrict__ xyz)
 {
     const int B = 1 << lb;
-    const size_t plane = static_cast<size_t>(nC) * B;
-    ENTITY_ITEM(nC)
+    const size_t plane = static_cast<size_t>(m.nC) * B;
+    ENTITY_ITEM(m.nC)
     {
         const int ic = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
@@ -1356,9 +1364,9 @@ template <typename T>
 void fwd_diag(const DMesh& m, const DConfig& c, int lb, const T* h, const T* u, T* pvv, T* hv,
               T* vort, T* ke, T* pc, T* he, T* pe, T* v, T* gpt, T* gpn, double* rec, hipStream_t s)
 {
-    LAUNCH_WU(k_fwd_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, lb, h, u, pvv, hv, vort);
-    LAUNCH_WU(k_fwd_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, u, pvv, ke, pc);
-    LAUNCH_WU(k_fwd_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, h, u, pvv, pc, he, pe, v,
+    LAUNCH_WU(k_fwd_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, lb, h, u, pvv, hv, vort);
+    LAUNCH_WU(k_fwd_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, lb, u, pvv, ke, pc);
+    LAUNCH_WU(k_fwd_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, c, lb, h, u, pvv, pc, he, pe, v,
               gpt, gpn, rec);
 }
 
@@ -1368,7 +1376,7 @@ void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, con
                         const Dissip<T>& dp, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
                         hipStream_t s)
 {
-    LAUNCH_WU(k_fwd_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, T(k.w[stage]),
+    LAUNCH_WU(k_fwd_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2), s, m, c, lb, stage, T(k.w[stage]),
               T(k.sw[stage]), h, u, he, pe, ke, dp, xh, xu, acch, accu, ph, pu);
 }
 
@@ -1378,15 +1386,15 @@ Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, co
 {
     Dissip<T> dp{nullptr, nullptr, nullptr, nullptr};
     if (!(c.visc2 > 0.0 || c.visc4 > 0.0)) return dp;
-    LAUNCH_WU(k_div, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, u, ds.div);
+    LAUNCH_WU(k_div, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, lb, u, ds.div);
     dp.div = ds.div;
     dp.vort = vort;
     if (c.visc4 > 0.0) {
-        LAUNCH_WU(k_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, lb, ds.div,
+        LAUNCH_WU(k_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, lb, ds.div,
                   vort, ds.dsu);
-        LAUNCH_WU(k_del4_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, lb, ds.dsu,
+        LAUNCH_WU(k_del4_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, lb, ds.dsu,
                   ds.dsv);
-        LAUNCH_WU(k_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, ds.dsu,
+        LAUNCH_WU(k_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, lb, ds.dsu,
                   ds.dsd);
         dp.dsd = ds.dsd;
         dp.dsv = ds.dsv;
@@ -1396,17 +1404,17 @@ Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, co
 
 void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_tl_coef, dim3(grid_items(16L * m.nE + 8L * m.nC)), dim3(kBlock), 0, s, m,
+    hipLaunchKernelGGL(k_tl_coef, dim3(grid_items(16L * m.nE + 8L * m.nC, m.stripe_log2)), dim3(kBlock), 0, s, m,
                        b, c.tl_fused, q.e48, q.c32);
 }
 
 void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adj_coef_edge, dim3(grid_items(16L * m.nE)), dim3(kBlock), 0, s, m, c, b,
+    hipLaunchKernelGGL(k_adj_coef_edge, dim3(grid_items(16L * m.nE, m.stripe_log2)), dim3(kBlock), 0, s, m, c, b,
                        q.e48, q.e8);
-    hipLaunchKernelGGL(k_adj_coef_vertex, dim3(grid_items(8L * m.nV)), dim3(kBlock), 0, s, m, c, b,
+    hipLaunchKernelGGL(k_adj_coef_vertex, dim3(grid_items(8L * m.nV, m.stripe_log2)), dim3(kBlock), 0, s, m, c, b,
                        q.v8);
-    hipLaunchKernelGGL(k_adj_coef_cell, dim3(grid_items(8L * m.nC)), dim3(kBlock), 0, s, m, c, b,
+    hipLaunchKernelGGL(k_adj_coef_cell, dim3(grid_items(8L * m.nC, m.stripe_log2)), dim3(kBlock), 0, s, m, c, b,
                        q.c32);
 }
 
@@ -1417,14 +1425,14 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
               T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s)
 {
     const bool visc = c.visc2 > 0.0 || c.visc4 > 0.0;
-    LAUNCH_WU(k_tl_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, lb, b.hv, b.vort,
+    LAUNCH_WU(k_tl_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, lb, b.hv, b.vort,
               h_tl, u_tl, pvv_tl, visc ? vort_tl : nullptr);
-    LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, lb, q.c32, u_tl,
+    LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, lb, q.c32, u_tl,
               pvv_tl, ke_tl, pc_tl);
-    LAUNCH_WU(k_tl_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, c, lb, b, h_tl,
+    LAUNCH_WU(k_tl_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, c, lb, b, h_tl,
               u_tl, pvv_tl, pc_tl, pe_tl, he_tl);
     const Dissip<T> dp = dissip_fields<T>(m, c, lb, u_tl, vort_tl, ds, s);
-    LAUNCH_WU(k_tl_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, T(k.w[stage]),
+    LAUNCH_WU(k_tl_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2), s, m, c, lb, stage, T(k.w[stage]),
               T(k.sw[stage]), b, q.e48, q.c32, h_tl, u_tl, pe_tl, he_tl, ke_tl, dp, xh, xu, acch,
               accu, ph, pu);
 }
@@ -1439,22 +1447,22 @@ void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, 
     // adjoint [C]; ds.dsu: nabla^2 u adjoint [E]; ds.div: divergence
     // adjoint / area [C]
     if (c.visc4 > 0.0) {
-        LAUNCH_WU(k_adj_del4_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, c, lb,
+        LAUNCH_WU(k_adj_del4_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, c, lb,
                   gu, ds.dsv);
-        LAUNCH_WU(k_adj_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, c, lb, gu,
+        LAUNCH_WU(k_adj_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, c, lb, gu,
                   ds.dsd);
-        LAUNCH_WU(k_adj_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, lb,
+        LAUNCH_WU(k_adj_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, lb,
                   ds.dsd, ds.dsv, ds.dsu);
     }
-    LAUNCH_WU(k_adj_edge1, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, lb, q, gh, gu,
+    LAUNCH_WU(k_adj_edge1, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, lb, q, gh, gu,
               pe_ad, he_ad, ua_p);
-    LAUNCH_WU(k_adj_cell1, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m, c, lb, q, gu,
+    LAUNCH_WU(k_adj_cell1, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, c, lb, q, gu,
               pe_ad, he_ad, pc_ad, ha_p, ke_ad, ds.dsu, ds.div);
-    LAUNCH_WU(k_adj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb), s, m, c, lb, q, pe_ad,
+    LAUNCH_WU(k_adj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, c, lb, q, pe_ad,
               pc_ad, hv_ad, circ_ad, gu, ds.dsu);
     const double w = stage > 0 ? k.w[stage - 1] : 0.0;
     const double sw = stage > 0 ? k.sw[stage - 1] : 0.0;
-    LAUNCH_WU(k_adj_final, T, lb, grid_edge_cell(m.nE, m.nC, lb), s, m, c, lb, stage, first, T(w),
+    LAUNCH_WU(k_adj_final, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2), s, m, c, lb, stage, first, T(w),
               T(sw), q, pe_ad, ua_p, ha_p, hv_ad, circ_ad, ke_ad, ds.div, lamh, lamu, acch, accu, gh,
               gu);
 }
@@ -1470,9 +1478,9 @@ template <typename T>
 void reconstruct_adj(const DMesh& m, int lb, const double* coeffs, const double* trig,
                      const T* seeds5, T* xyz, T* u_ad, hipStream_t s)
 {
-    LAUNCH_WU(k_recon_adj_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb), s, m.nC, lb, trig,
+    LAUNCH_WU(k_recon_adj_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, lb, trig,
               seeds5, xyz);
-    LAUNCH_WU(k_recon_adj_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb), s, m, lb, coeffs,
+    LAUNCH_WU(k_recon_adj_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, lb, coeffs,
               xyz, u_ad);
 }
 

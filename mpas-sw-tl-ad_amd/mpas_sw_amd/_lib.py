@@ -80,6 +80,7 @@ EXPORTS = (
 )
 
 OPT_TL_FUSED_FLUX = 1   # msw_option
+OPT_XCD_STRIPE = 2
 
 _lib = None
 

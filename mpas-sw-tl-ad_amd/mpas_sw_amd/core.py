@@ -125,6 +125,7 @@ class SWModel:
             self.set_forcing(u_src)
         if tl_fused_flux:
             self.set_option(_lib.OPT_TL_FUSED_FLUX, 1)
+        self.stripe_log2 = 2
 
     # ------------------------------------------------------------------ utils
     def close(self):
@@ -159,6 +160,39 @@ class SWModel:
 
     def set_option(self, option: int, value: int):
         _lib.check(self.lib.msw_set_option(self._h, int(option), int(value)))
+
+    def autotune(self, h, u, nsteps: int = 2, candidates=(0, 1, 2, 3, 4), seed: int = 0):
+        """Choose the launch stripe (MSW_OPT_XCD_STRIPE: 2^k consecutive blocks
+        per XCD turn) by timing a TLM sweep plus an adjoint sweep of `nsteps`
+        from the FP64 base state (h, u) with random member perturbations, each
+        candidate once after a warm-up.  Sets the fastest and returns
+        {k: seconds}.  (h, u) are not modified."""
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        shp_c, shp_e = self._shape("c"), self._shape("e")
+        ht0 = torch.randn(shp_c, generator=g, dtype=torch.float64).to(self.device, self.dtype)
+        ut0 = (0.1 * torch.randn(shp_e, generator=g, dtype=torch.float64)).to(self.device, self.dtype)
+        th = torch.empty((nsteps, self.nC), dtype=torch.float64, device=self.device)
+        tu = torch.empty((nsteps, self.nE), dtype=torch.float64, device=self.device)
+        bufs = [torch.empty_like(h), torch.empty_like(u), torch.empty_like(ht0), torch.empty_like(ut0)]
+        times = {}
+        for k in candidates:
+            self.set_option(_lib.OPT_XCD_STRIPE, int(k))
+            for rep in range(2):   # the first run captures the step graphs
+                for b, src in zip(bufs, (h, u, ht0, ut0)):
+                    b.copy_(src)
+                self.stream.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(self.stream)
+                self.tlm(bufs[0], bufs[1], bufs[2], bufs[3], nsteps, th, tu)
+                self.adj(th, tu, bufs[2], bufs[3], nsteps)
+                e1.record(self.stream)
+                e1.synchronize()
+            times[int(k)] = e0.elapsed_time(e1) / 1e3
+        best = min(times, key=times.get)
+        self.set_option(_lib.OPT_XCD_STRIPE, best)
+        self.stripe_log2 = best
+        return times
 
     def set_forcing(self, u_src):
         """Wind-stress source u_src [E] (reference order); None resets it to 0."""

@@ -462,3 +462,34 @@ def test_invariants_match_oracle_and_mass_is_conserved():
     assert abs(inv[0] - mass) <= 1e-13 * abs(mass)
     assert abs(inv[1] - energy) <= 1e-13 * abs(energy)
     assert abs(inv[0] - m0[0]) <= 1e-13 * abs(m0[0])
+
+
+def test_launch_stripe_does_not_change_results_and_autotune():
+    """The block-to-XCD stripe only reorders work: TLM and adjoint results are
+    bitwise the same for every stripe; the autotuner picks one of them."""
+    print("stripes L3 tc6 B=4", flush=True)
+    from mpas_sw_amd import _lib
+    tc, dt, B, n = 6, 600.0, 4, 3
+    mesh, phys = case(LEVEL, tc)
+    dh, du = perturbations(mesh, B)
+    m = model(LEVEL, tc, dt, batch=B, graphs=True, tl_fused_flux=True)
+    outs = []
+    for k in (0, 2, 5):
+        m.set_option(_lib.OPT_XCD_STRIPE, k)
+        h = m.to_internal(phys["h"], "c", member=False)
+        u = m.to_internal(phys["u"], "e", member=False)
+        ht, ut = m.to_internal(dh, "c"), m.to_internal(du, "e")
+        th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+        tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+        m.tlm(h, u, ht, ut, n, th, tu)
+        tl = ht.clone()
+        m.adj(th, tu, ht, ut, n)
+        torch.cuda.synchronize()
+        outs.append((tl, ht.clone(), ut.clone()))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert torch.equal(a, b)
+    h = m.to_internal(phys["h"], "c", member=False)
+    u = m.to_internal(phys["u"], "e", member=False)
+    times = m.autotune(h, u, nsteps=1, candidates=(0, 2))
+    assert set(times) == {0, 2} and m.stripe_log2 in times
