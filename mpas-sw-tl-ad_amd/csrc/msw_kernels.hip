@@ -663,7 +663,7 @@ __device__ __forceinline__ T tend_forcing_tl(const DMesh& m, const DConfig& c, c
 }
 
 // tend_u_tl (:564-595) and tend_h_tl (:541-561), then RK update
-template <typename T, bool WU>
+template <typename T, bool WU, bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, int stage, T w,
                                                     T sw, BaseStage bs, const double* __restrict__ e48,
                                                     const double* __restrict__ c32,
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
             const T h1 = ld(h_tl, c1, B, b), h2 = ld(h_tl, c2, B, b);
             const double* r = e48 + 48L * e;
             T qt = T(0);
-            if (c.tl_fused) {
+            if constexpr (FUSED) {
                 // he_tl holds the TL flux F_tl = u_tl*he + u*he_tl
                 T fx[kE2], px[kE2];
 #pragma unroll
@@ -719,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
             T tend = qt - (k2 - k1 + T(kGravity) * (h2 - h1)) / T(m.dc[e]);
             if (c.visc2 > 0.0 || c.visc4 > 0.0 || c.wind || c.drag)
                 tend = tend_forcing_tl(m, c, dp, bs, e, c1, c2, B, b, tend, ld(u_tl, e, B, b),
-                                       c.tl_fused ? T(0.5) * (h1 + h2) : ld(he_tl, e, B, b), k1, k2);
+                                       FUSED ? T(0.5) * (h1 + h2) : ld(he_tl, e, B, b), k1, k2);
             rk_update(stage, w, sw, tend, xu, accu, pu, e, B, b,
                       c.frozen_u != 0);
         } else {
@@ -729,7 +729,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
             for (int k = 0; k < kC6; ++k) ie[k] = m.eocR[kC6 * ic + k];
             const double* r = c32 + 32L * ic;
             T th = T(0);
-            if (c.tl_fused) {
+            if constexpr (FUSED) {
                 T fx[kC6];
 #pragma unroll
                 for (int k = 0; k < kC6; ++k) fx[k] = ld(he_tl, ie[k], B, b);
@@ -1432,9 +1432,23 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
     LAUNCH_WU(k_tl_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, c, lb, b, h_tl,
               u_tl, pvv_tl, pc_tl, pe_tl, he_tl);
     const Dissip<T> dp = dissip_fields<T>(m, c, lb, u_tl, vort_tl, ds, s);
-    LAUNCH_WU(k_tl_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2), s, m, c, lb, stage, T(k.w[stage]),
-              T(k.sw[stage]), b, q.e48, q.c32, h_tl, u_tl, pe_tl, he_tl, ke_tl, dp, xh, xu, acch,
-              accu, ph, pu);
+    // the fused and reference-order forms are separate instances, so each
+    // gets its own register allocation
+    const int g = grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2);
+#define TL_TEND_ARGS s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), b, q.e48, q.c32, h_tl, u_tl, \
+                     pe_tl, he_tl, ke_tl, dp, xh, xu, acch, accu, ph, pu
+    if (lb >= 6) {
+        if (c.tl_fused)
+            hipLaunchKernelGGL((k_tl_tend<T, true, true>), dim3(g), dim3(kBlock), 0, TL_TEND_ARGS);
+        else
+            hipLaunchKernelGGL((k_tl_tend<T, true, false>), dim3(g), dim3(kBlock), 0, TL_TEND_ARGS);
+    } else {
+        if (c.tl_fused)
+            hipLaunchKernelGGL((k_tl_tend<T, false, true>), dim3(g), dim3(kBlock), 0, TL_TEND_ARGS);
+        else
+            hipLaunchKernelGGL((k_tl_tend<T, false, false>), dim3(g), dim3(kBlock), 0, TL_TEND_ARGS);
+    }
+#undef TL_TEND_ARGS
 }
 
 template <typename T>
