@@ -1,0 +1,189 @@
+"""GPU runs of the reference's own cases and of the acceptance checks, at the
+sizes of the workloads they stand for (forward one day at level 5, TLM and
+adjoint windows against the reference Fortran's output, a level-6 window for
+the linearisation, dot-product and gradient tests, FP32 accuracy after one
+day).
+
+The golden vectors were written by the reference Fortran itself
+(tests/golden/make_golden.py), so these compare the HIP path with the
+reference directly, not only with the oracle.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import case, golden, perturbations, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(level, tc, dt, **kw):
+    from mpas_sw_amd.core import SWModel
+    mesh, phys = case(level, tc)
+    return SWModel(mesh, phys, dt, test_case=tc, **kw)
+
+
+def test_forward_one_day_equals_reference_fortran():
+    """TC2, level 5, dt 300 s, 288 steps: the GPU state is the reference's to the bit."""
+    print("fwd 1 day L5 tc2 vs reference", flush=True)
+    g, meta = golden("tc2_l5_fwd_1day")
+    m = _model(5, 2, meta["dt"], graphs=True)
+    h = m.to_internal(g["h0"], "c")
+    u = m.to_internal(g["u0"], "e")
+    m.fwd(h, u, meta["nsteps"])
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(m.to_reference(h, "c"), g["h"])
+    np.testing.assert_array_equal(m.to_reference(u, "e"), g["u"])
+
+
+@pytest.mark.parametrize("tc", [5, 6])
+def test_tlm_and_adjoint_windows_against_reference_fortran(tc):
+    """24-step TLM window bit-identical to the reference (reference term
+    order), fused-flux TLM and the adjoint window within 1e-11."""
+    print(f"tlm/adj window L4 tc{tc} vs reference", flush=True)
+    g, meta = golden(f"tc{tc}_l4_tlm_adj")
+    n, dt = meta["nsteps"], meta["dt"]
+    for fused in (False, True):
+        m = _model(4, tc, dt, graphs=True, tl_fused_flux=fused)
+        h = m.to_internal(g["h0"], "c", member=False)
+        u = m.to_internal(g["u0"], "e", member=False)
+        ht, ut = m.to_internal(g["dh"], "c"), m.to_internal(g["du"], "e")
+        m.tlm(h, u, ht, ut, n)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(m.to_reference(h, "c"), g["h"])
+        if fused:
+            assert rel_l2(m.to_reference(ht, "c"), g["h_tl"]) < 1e-11
+            assert rel_l2(m.to_reference(ut, "e"), g["u_tl"]) < 1e-11
+        else:
+            np.testing.assert_array_equal(m.to_reference(ht, "c"), g["h_tl"])
+            np.testing.assert_array_equal(m.to_reference(ut, "e"), g["u_tl"])
+    h = m.to_internal(g["h0"], "c", member=False)
+    u = m.to_internal(g["u0"], "e", member=False)
+    th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+    tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+    m.base(h, u, n, th, tu)
+    ha, ua = m.to_internal(g["dh"], "c"), m.to_internal(g["du"], "e")
+    m.adj(th, tu, ha, ua, n)
+    torch.cuda.synchronize()
+    assert rel_l2(m.to_reference(ha, "c"), g["h_ad"]) < 1e-11
+    assert rel_l2(m.to_reference(ua, "e"), g["u_ad"]) < 1e-11
+
+
+def test_dissipation_window_against_reference_fortran():
+    """del2 + del4 + drag + wind stress, level 3, 12 steps: forward and TLM
+    bit-identical to the reference, adjoint within 1e-11."""
+    print("dissipation window L3 tc5 vs reference", flush=True)
+    g, meta = golden("tc5_l3_dissipation")
+    n, dt = meta["nsteps"], meta["dt"]
+    kw = dict(visc2=meta["visc2"], visc4=meta["visc4"], wind_stress=meta["wind_stress"],
+              bottom_drag=meta["bottom_drag"], u_src=g["u_src"])
+    m = _model(3, 5, dt, **kw)
+    h, u = m.to_internal(g["h0"], "c"), m.to_internal(g["u0"], "e")
+    m.fwd(h, u, n)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(m.to_reference(h, "c"), g["h_fwd"])
+    np.testing.assert_array_equal(m.to_reference(u, "e"), g["u_fwd"])
+    h = m.to_internal(g["h0"], "c", member=False)
+    u = m.to_internal(g["u0"], "e", member=False)
+    ht, ut = m.to_internal(g["dh"], "c"), m.to_internal(g["du"], "e")
+    m.tlm(h, u, ht, ut, n)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(m.to_reference(ht, "c"), g["h_tl"])
+    np.testing.assert_array_equal(m.to_reference(ut, "e"), g["u_tl"])
+    h = m.to_internal(g["h0"], "c", member=False)
+    u = m.to_internal(g["u0"], "e", member=False)
+    th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+    tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+    m.base(h, u, n, th, tu)
+    ha, ua = m.to_internal(g["dh"], "c"), m.to_internal(g["du"], "e")
+    m.adj(th, tu, ha, ua, n)
+    torch.cuda.synchronize()
+    assert rel_l2(m.to_reference(ha, "c"), g["h_ad"]) < 1e-11
+    assert rel_l2(m.to_reference(ua, "e"), g["u_ad"]) < 1e-11
+
+
+@pytest.mark.parametrize("tc", [2, 5])
+def test_fp32_forward_one_day_within_1e5_of_fp64(tc):
+    """FP32 members after one day (level 5, dt 300 s, 288 steps) within 1e-5
+    relative L2 of the FP64 state."""
+    print(f"fp32 forward 1 day L5 tc{tc}", flush=True)
+    mesh, phys = case(5, tc)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        m = _model(5, tc, 300.0, precision=prec, graphs=True)
+        h, u = m.to_internal(phys["h"], "c"), m.to_internal(phys["u"], "e")
+        m.fwd(h, u, 288)
+        torch.cuda.synchronize()
+        out[prec] = (m.to_reference(h, "c"), m.to_reference(u, "e"))
+    assert rel_l2(out["fp32"][0], out["fp64"][0]) < 1e-5
+    assert rel_l2(out["fp32"][1], out["fp64"][1]) < 1e-5
+
+
+def test_level6_window_linearisation_dot_product_and_gradient():
+    """Level 6, TC5, dt 150 s, 48 steps, FP64: TLM linearisation ratio -> 1,
+    dot-product test <= 1e-12 over the window (2 members), and the gradient
+    of J = 1/2 sum A h(T)^2 through the checkpointed autograd window against
+    central differences."""
+    print("L6 tc5 window checks", flush=True)
+    from mpas_sw_amd.checkpoint import rk4_window
+    tc, dt, n, B = 5, 150.0, 48, 2
+    mesh, phys = case(6, tc)
+    m = _model(6, tc, dt, graphs=True, batch=B)
+    h0 = m.to_internal(phys["h"], "c", member=False)
+    u0 = m.to_internal(phys["u"], "e", member=False)
+    dh, du = perturbations(mesh, B)
+    # linearisation: (M(x + e dx) - M(x)) / e -> L dx as e -> 0
+    m1 = _model(6, tc, dt, graphs=True)
+    hb, ub = h0.clone(), u0.clone()
+    m1.base(hb, ub, n)
+    ht, ut = m1.to_internal(dh[:, 0], "c"), m1.to_internal(du[:, 0], "e")
+    hx, ux = h0.clone(), u0.clone()
+    m1.tlm(hx, ux, ht, ut, n)
+    errs, ratios = [], []
+    for eps in (1e-1, 1e-2, 1e-3):
+        hp = h0 + eps * m1.to_internal(dh[:, 0], "c", member=False)
+        up = u0 + eps * m1.to_internal(du[:, 0], "e", member=False)
+        m1.base(hp, up, n)
+        num = torch.cat([(hp - hb), (up - ub)]) / eps
+        lin = torch.cat([ht, ut])
+        errs.append(float(torch.linalg.norm(num - lin) / torch.linalg.norm(lin)))
+        ratios.append(float(torch.linalg.norm(num) / torch.linalg.norm(lin)))
+    # first-order convergence of the remainder and the ratio -> 1
+    assert 5 < errs[0] / errs[1] < 20 and 5 < errs[1] / errs[2] < 20
+    assert errs[2] < 1e-4 and abs(ratios[2] - 1) < 1e-6
+    # dot-product test over the window
+    xh, xu = m.to_internal(dh, "c"), m.to_internal(du, "e")
+    yh0, yu0 = perturbations(mesh, B, seed=77)
+    yh, yu = m.to_internal(yh0, "c"), m.to_internal(yu0, "e")
+    th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+    tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+    Lxh, Lxu = xh.clone(), xu.clone()
+    hb2, ub2 = h0.clone(), u0.clone()
+    m.tlm(hb2, ub2, Lxh, Lxu, n, th, tu)
+    LTyh, LTyu = yh.clone(), yu.clone()
+    m.adj(th, tu, LTyh, LTyu, n)
+    lhs = m.dot(Lxh, Lxu, yh, yu).cpu().numpy()
+    rhs = m.dot(xh, xu, LTyh, LTyu).cpu().numpy()
+    assert np.all(np.abs(lhs - rhs) <= 1e-12 * np.abs(lhs))
+    # checkpointed autograd gradient vs central differences
+    A = m1.to_internal(phys["areaCell"], "c", member=False)
+
+    def loss(hh, uu):
+        hT, _ = rk4_window(m1, hh, uu, n, 4)
+        return 0.5 * (A * hT ** 2).sum()
+
+    hg, ug = h0.clone().requires_grad_(True), u0.clone().requires_grad_(True)
+    loss(hg, ug).backward()
+    rng = np.random.default_rng(3)
+    vh = m1.to_internal(rng.standard_normal(mesh.nCells), "c", member=False)
+    vu = m1.to_internal(0.1 * rng.standard_normal(mesh.nEdges), "e", member=False)
+    eps = 1e-2
+    with torch.no_grad():
+        hp, _ = rk4_window(m1, h0 + eps * vh, u0 + eps * vu, n, 4)
+        hm, _ = rk4_window(m1, h0 - eps * vh, u0 - eps * vu, n, 4)
+        # J(+) - J(-) = 1/2 sum A (h+ - h-)(h+ + h-), without the cancellation
+        # of two sums of size ~ A h^2
+        fd = (0.5 * (A * (hp - hm) * (hp + hm)).sum()) / (2 * eps)
+    ad = (hg.grad * vh).sum() + (ug.grad * vu).sum()
+    assert abs(float(fd) - float(ad)) < 1e-7 * abs(float(ad))
