@@ -11,6 +11,10 @@ namespace msw {
 
 constexpr int kMaxEdges = 6;
 constexpr int kMaxEdges2 = 12;
+// edgesOnEdge of an edge = the other edges of its two cells, so with cells of
+// at most 6 edges no edge has more than 10; the member kernels gather only
+// these 10 slots (slots 10 and 11 of the 12-wide tables are always padding)
+constexpr int kEdgesOnEdge = 2 * (kMaxEdges - 1);
 constexpr int kVertexDegree = 3;
 constexpr double kGravity = 9.80616;   // src/mpas_sw_dyncore.f90:414
 constexpr double kRhoRef = 1000.0;     // src/mpas_sw_dyncore.f90:458

@@ -7,8 +7,9 @@
 // loads, one SGPR) and each neighbour gather is one contiguous B*sizeof(T)
 // row.  For B = 1 (the plain path) each lane is one entity.
 //
-// Neighbour lists are padded to fixed widths (6 per cell, 12 per edge, 3
-// per vertex) with zero weights, and every gather loop is fully unrolled:
+// Neighbour lists are padded to fixed widths (6 per cell, 10 edgesOnEdge in
+// tables of stride 12, 3 per vertex) with zero weights, and every gather
+// loop is fully unrolled:
 // all index loads and then all row loads of an entity are issued before any
 // arithmetic, so a wavefront keeps many loads in flight instead of walking a
 // chain of dependent loads.  Padded terms add exact zeros.
@@ -28,7 +29,8 @@ namespace msw {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kE2 = kMaxEdges2;
+constexpr int kE2 = kMaxEdges2;    // stride of the edgesOnEdge tables
+constexpr int kEoE = kEdgesOnEdge;  // slots the member gathers visit
 constexpr int kC6 = kMaxEdges;
 
 template <bool WU>
@@ -236,19 +238,19 @@ __global__ __launch_bounds__(kBlock) void k_fwd_edge(DMesh m, DConfig c, int lb,
         const int b = static_cast<int>(t & (B - 1));
         const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
         const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
-        int ix[kE2];
-        T ux[kE2];
+        int ix[kEoE];
+        T ux[kEoE];
 #pragma unroll
-        for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+        for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
 #pragma unroll
-        for (int j = 0; j < kE2; ++j) ux[j] = ld(u, ix[j], B, b);
+        for (int j = 0; j < kEoE; ++j) ux[j] = ld(u, ix[j], B, b);
         const T h1 = ld(h, c1, B, b), h2 = ld(h, c2, B, b);
         const T p1 = ld(pvv, v1, B, b), p2 = ld(pvv, v2, B, b);
         const T q1 = ld(pc, c1, B, b), q2 = ld(pc, c2, B, b);
         const T ue = ld(u, e, B, b);
         T vv = T(0);
 #pragma unroll
-        for (int j = 0; j < kE2; ++j) vv = vv + T(m.woe[kE2 * e + j]) * ux[j];
+        for (int j = 0; j < kEoE; ++j) vv = vv + T(m.woe[kE2 * e + j]) * ux[j];
         const T gt = (p2 - p1) / T(m.dv[e]);
         const T gn = (q2 - q1) / T(m.dc[e]);
         T p = T(0.5) * p1 + T(0.5) * p2;
@@ -424,12 +426,12 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
         if (is_edge) {
             const int e = entity_of<WU>(t, lb);
             const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
-            int ix[kE2];
-            T ux[kE2], px[kE2], hx[kE2];
+            int ix[kEoE];
+            T ux[kEoE], px[kEoE], hx[kEoE];
 #pragma unroll
-            for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+            for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
 #pragma unroll
-            for (int j = 0; j < kE2; ++j) {
+            for (int j = 0; j < kEoE; ++j) {
                 ux[j] = ld(u, ix[j], B, b);
                 px[j] = ld(pe, ix[j], B, b);
                 hx[j] = ld(he, ix[j], B, b);
@@ -439,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
             const T h1 = ld(h, c1, B, b), h2 = ld(h, c2, B, b);
             T q = T(0);
 #pragma unroll
-            for (int j = 0; j < kE2; ++j) {
+            for (int j = 0; j < kEoE; ++j) {
                 const T workpv = T(0.5) * (pee + px[j]);
                 q = q + T(m.woe[kE2 * e + j]) * ux[j] * workpv * hx[j];
             }
@@ -562,19 +564,19 @@ __global__ __launch_bounds__(kBlock) void k_tl_edge(DMesh m, DConfig c, int lb, 
         const int b = static_cast<int>(t & (B - 1));
         const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
         const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
-        int ix[kE2];
-        T ux[kE2];
+        int ix[kEoE];
+        T ux[kEoE];
 #pragma unroll
-        for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+        for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
 #pragma unroll
-        for (int j = 0; j < kE2; ++j) ux[j] = ld(u_tl, ix[j], B, b);
+        for (int j = 0; j < kEoE; ++j) ux[j] = ld(u_tl, ix[j], B, b);
         const T p1 = ld(pvv_tl, v1, B, b), p2 = ld(pvv_tl, v2, B, b);
         const T q1 = ld(pc_tl, c1, B, b), q2 = ld(pc_tl, c2, B, b);
         const T h1 = ld(h_tl, c1, B, b), h2 = ld(h_tl, c2, B, b);
         const T ue = ld(u_tl, e, B, b);
         T vt = T(0);
 #pragma unroll
-        for (int j = 0; j < kE2; ++j) vt = vt + T(m.woe[kE2 * e + j]) * ux[j];
+        for (int j = 0; j < kEoE; ++j) vt = vt + T(m.woe[kE2 * e + j]) * ux[j];
         const T gtt = (p2 - p1) / T(m.dv[e]);
         const T gnt = (q2 - q1) / T(m.dc[e]);
         T p = T(0.5) * p1 + T(0.5) * p2;
@@ -681,9 +683,9 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
         if (is_edge) {
             const int e = entity_of<WU>(t, lb);
             const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
-            int ix[kE2];
+            int ix[kEoE];
 #pragma unroll
-            for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+            for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
             const T pte = ld(pe_tl, e, B, b);
             const T k1 = ld(ke_tl, c1, B, b), k2 = ld(ke_tl, c2, B, b);
             const T h1 = ld(h_tl, c1, B, b), h2 = ld(h_tl, c2, B, b);
@@ -691,24 +693,24 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
             T qt = T(0);
             if constexpr (FUSED) {
                 // he_tl holds the TL flux F_tl = u_tl*he + u*he_tl
-                T fx[kE2], px[kE2];
+                T fx[kEoE], px[kEoE];
 #pragma unroll
-                for (int j = 0; j < kE2; ++j) {
+                for (int j = 0; j < kEoE; ++j) {
                     fx[j] = ld(he_tl, ix[j], B, b);
                     px[j] = ld(pe_tl, ix[j], B, b);
                 }
 #pragma unroll
-                for (int j = 0; j < kE2; ++j) qt = qt + T(r[j]) * fx[j] + T(r[12 + j]) * (pte + px[j]);
+                for (int j = 0; j < kEoE; ++j) qt = qt + T(r[j]) * fx[j] + T(r[12 + j]) * (pte + px[j]);
             } else {
-                T ux[kE2], px[kE2], hx[kE2];
+                T ux[kEoE], px[kEoE], hx[kEoE];
 #pragma unroll
-                for (int j = 0; j < kE2; ++j) {
+                for (int j = 0; j < kEoE; ++j) {
                     ux[j] = ld(u_tl, ix[j], B, b);
                     px[j] = ld(pe_tl, ix[j], B, b);
                     hx[j] = ld(he_tl, ix[j], B, b);
                 }
 #pragma unroll
-                for (int j = 0; j < kE2; ++j) {
+                for (int j = 0; j < kEoE; ++j) {
                     // w*u_tl*workpv*he + w*u*workpv_tl*he + w*u*workpv*h_edge_tl
                     const T wgt = T(m.woe[kE2 * e + j]);
                     const T wpv = T(r[j]), heb = T(r[12 + j]);
@@ -885,12 +887,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q
         const int e = entity_of<WU>(t, lb);
         const int b = static_cast<int>(t & (B - 1));
         const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
-        int ix[kE2];
-        T gx[kE2];
+        int ix[kEoE];
+        T gx[kEoE];
 #pragma unroll
-        for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+        for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
 #pragma unroll
-        for (int j = 0; j < kE2; ++j) gx[j] = ld(gu, ix[j], B, b);
+        for (int j = 0; j < kEoE; ++j) gx[j] = ld(gu, ix[j], B, b);
         const T gh1 = ld(gh, c1, B, b), gh2 = ld(gh, c2, B, b);
         const T ge = ld(gu, e, B, b);
         const double* r = q.e48 + 24L * e;
@@ -901,7 +903,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q
         // / d h_edge(e) = wt*workpv*u(e), / d pv_edge(e) = 0.5*wt*u*h_edge
         T s1 = T(0), s2 = T(0);
 #pragma unroll
-        for (int j = 0; j < kE2; ++j) {
+        for (int j = 0; j < kEoE; ++j) {
             s1 = s1 + T(r[j]) * gx[j];
             s2 = s2 + T(m.woeT[kE2 * e + j]) * gx[j];
         }
@@ -1119,19 +1121,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
         const int b = static_cast<int>(t & (B - 1));
         if (is_edge) {
             const int e = entity_of<WU>(t, lb);
-            int ix[kE2];
-            T px[kE2];
+            int ix[kEoE];
+            T px[kEoE];
 #pragma unroll
-            for (int j = 0; j < kE2; ++j) ix[j] = m.eoe[kE2 * e + j];
+            for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
 #pragma unroll
-            for (int j = 0; j < kE2; ++j) px[j] = ld(pe_ad, ix[j], B, b);
+            for (int j = 0; j < kEoE; ++j) px[j] = ld(pe_ad, ix[j], B, b);
             const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
             const T cv2 = ld(circ_ad, m.voe[2 * e + 1], B, b), cv1 = ld(circ_ad, m.voe[2 * e], B, b);
             const T k1 = ld(ke_ad, c1, B, b), k2 = ld(ke_ad, c2, B, b);
             const double* r = q.e48 + 24L * e + 12;
             T ua = ld(ua_p, e, B, b);
 #pragma unroll
-            for (int j = 0; j < kE2; ++j) ua = ua + T(r[j]) * px[j];
+            for (int j = 0; j < kEoE; ++j) ua = ua + T(r[j]) * px[j];
             ua = ua + T(m.dc[e]) * (cv2 - cv1);
             const T kc = T(q.e8[16L * e + 6]);
             ua = ua + kc * k1 + kc * k2;

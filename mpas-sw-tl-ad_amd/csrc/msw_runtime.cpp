@@ -204,6 +204,9 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
         if (n < 0 || n > std::min(mE2, kMaxEdges2))
             return fail(n > kMaxEdges2 ? MSW_EUNSUPPORTED : MSW_EINVAL,
                         "nEdgesOnEdge out of range at edge " + std::to_string(e + 1));
+        if (n > kEdgesOnEdge)
+            return fail(MSW_EINVAL, "edge " + std::to_string(e + 1) + " has " + std::to_string(n) +
+                                        " edgesOnEdge; cells of at most 6 edges give at most 10");
         for (int j = 0; j < n; ++j)
             if (!chk(d->edgesOnEdge[static_cast<long>(mE2) * e + j], E))
                 return fail(MSW_EINVAL, "edgesOnEdge out of range at edge " + std::to_string(e + 1));
