@@ -1,6 +1,6 @@
 """GPU runs of the reference's own cases and of the acceptance checks, at the
 sizes of the workloads they stand for (forward one day at level 5, TLM and
-adjoint windows against the reference Fortran's output, a level-6 window for
+adjoint windows against the reference Fortran's output, a one-day level-6 window for
 the linearisation, dot-product and gradient tests, FP32 accuracy after one
 day).
 
@@ -120,14 +120,15 @@ def test_fp32_forward_one_day_within_1e5_of_fp64(tc):
     assert rel_l2(out["fp32"][1], out["fp64"][1]) < 1e-5
 
 
-def test_level6_window_linearisation_dot_product_and_gradient():
-    """Level 6, TC5, dt 150 s, 48 steps, FP64: TLM linearisation ratio -> 1,
-    dot-product test <= 1e-12 over the window (2 members), and the gradient
-    of J = 1/2 sum A h(T)^2 through the checkpointed autograd window against
-    central differences."""
-    print("L6 tc5 window checks", flush=True)
+def test_level6_day_window_linearisation_dot_product_and_gradient():
+    """Level 6, TC5, dt 150 s, 576 steps (one day), FP64: TLM linearisation
+    ratio -> 1 with a first-order remainder, dot-product test <= 1e-12 over
+    the window (2 members), and the gradient of J = 1/2 sum A h(T)^2 through
+    the checkpointed autograd window (32 snapshots) against central
+    differences along 8 seeded directions."""
+    print("L6 tc5 one-day window checks", flush=True)
     from mpas_sw_amd.checkpoint import rk4_window
-    tc, dt, n, B = 5, 150.0, 48, 2
+    tc, dt, n, B = 5, 150.0, 576, 2
     mesh, phys = case(6, tc)
     m = _model(6, tc, dt, graphs=True, batch=B)
     h0 = m.to_internal(phys["h"], "c", member=False)
@@ -170,20 +171,108 @@ def test_level6_window_linearisation_dot_product_and_gradient():
     A = m1.to_internal(phys["areaCell"], "c", member=False)
 
     def loss(hh, uu):
-        hT, _ = rk4_window(m1, hh, uu, n, 4)
+        hT, _ = rk4_window(m1, hh, uu, n, 32)
         return 0.5 * (A * hT ** 2).sum()
 
     hg, ug = h0.clone().requires_grad_(True), u0.clone().requires_grad_(True)
     loss(hg, ug).backward()
     rng = np.random.default_rng(3)
-    vh = m1.to_internal(rng.standard_normal(mesh.nCells), "c", member=False)
-    vu = m1.to_internal(0.1 * rng.standard_normal(mesh.nEdges), "e", member=False)
     eps = 1e-2
-    with torch.no_grad():
-        hp, _ = rk4_window(m1, h0 + eps * vh, u0 + eps * vu, n, 4)
-        hm, _ = rk4_window(m1, h0 - eps * vh, u0 - eps * vu, n, 4)
-        # J(+) - J(-) = 1/2 sum A (h+ - h-)(h+ + h-), without the cancellation
-        # of two sums of size ~ A h^2
-        fd = (0.5 * (A * (hp - hm) * (hp + hm)).sum()) / (2 * eps)
-    ad = (hg.grad * vh).sum() + (ug.grad * vu).sum()
-    assert abs(float(fd) - float(ad)) < 1e-7 * abs(float(ad))
+    for k in range(8):
+        vh = m1.to_internal(rng.standard_normal(mesh.nCells), "c", member=False)
+        vu = m1.to_internal(0.1 * rng.standard_normal(mesh.nEdges), "e", member=False)
+        with torch.no_grad():
+            hp, _ = rk4_window(m1, h0 + eps * vh, u0 + eps * vu, n, 32)
+            hm, _ = rk4_window(m1, h0 - eps * vh, u0 - eps * vu, n, 32)
+            # J(+) - J(-) = 1/2 sum A (h+ - h-)(h+ + h-), without the
+            # cancellation of two sums of size ~ A h^2
+            fd = (0.5 * (A * (hp - hm) * (hp + hm)).sum()) / (2 * eps)
+        ad = (hg.grad * vh).sum() + (ug.grad * vu).sum()
+        assert abs(float(fd) - float(ad)) < 1e-7 * abs(float(ad)), k
+
+
+def test_level8_fp32_adjoint_gradient_window():
+    """Level 8 (655362 cells), TC5, dt 37.5 s: J = 1/2 sum A (h(T) - h_obs)^2
+    with h_obs = the unperturbed forward h(T) + N(0, 10 m) noise (seed
+    20261203 + 1000), gradient wrt (h0, u0) through the checkpointed
+    autograd window with FP64 and with FP32 adjoint members; the FP64
+    gradient against central differences.  A 96-step window with 8
+    snapshots exercises the binomial recomputation at the full mesh size."""
+    print("L8 tc5 gradient window fp64/fp32", flush=True)
+    from mpas_sw_amd.checkpoint import rk4_window
+    from mpas_sw_amd.core import SWModel
+    from mpas_sw_amd.mesh import get_mesh
+    from mpas_sw_amd.testcases import initial_conditions
+    tc, dt, n, snaps = 5, 37.5, 96, 8
+    mesh = get_mesh(8)
+    phys = initial_conditions(mesh, tc)   # no reconstruction coefficients needed here
+    grads = {}
+    for prec in ("fp64", "fp32"):
+        m = SWModel(mesh, phys, dt, test_case=tc, precision=prec, graphs=True)
+        h0 = m.to_internal(phys["h"], "c", member=False)
+        u0 = m.to_internal(phys["u"], "e", member=False)
+        A = m.to_internal(phys["areaCell"], "c", member=False)
+        if prec == "fp64":
+            hT, uT = h0.clone(), u0.clone()
+            m.base(hT, uT, n)
+            g = torch.Generator("cpu").manual_seed(20261203 + 1000)
+            noise = torch.randn(mesh.nCells, generator=g, dtype=torch.float64).numpy() * 10.0
+            hobs = hT + m.to_internal(noise, "c", member=False)
+        hg, ug = h0.clone().requires_grad_(True), u0.clone().requires_grad_(True)
+        hTg, _ = rk4_window(m, hg, ug, n, snaps)
+        (0.5 * (A * (hTg - hobs) ** 2).sum()).backward()
+        grads[prec] = (hg.grad.clone(), ug.grad.clone(), m)
+    gh, gu, m = grads["fp64"]
+    for a, b in zip(grads["fp32"][:2], (gh, gu)):
+        assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 1e-4
+    rng = np.random.default_rng(5)
+    eps = 1e-2
+    for k in range(2):
+        vh = m.to_internal(rng.standard_normal(mesh.nCells), "c", member=False)
+        vu = m.to_internal(0.1 * rng.standard_normal(mesh.nEdges), "e", member=False)
+        with torch.no_grad():
+            hp, _ = rk4_window(m, h0 + eps * vh, u0 + eps * vu, n, snaps)
+            hm, _ = rk4_window(m, h0 - eps * vh, u0 - eps * vu, n, snaps)
+            fd = (0.5 * (A * (hp - hm) * (hp + hm - 2 * hobs)).sum()) / (2 * eps)
+        ad = (gh * vh).sum() + (gu * vu).sum()
+        assert abs(float(fd) - float(ad)) < 1e-6 * abs(float(ad)), k
+
+
+def test_packed_uneven_instances_fp32_forward_and_tlm():
+    """The uneven-batch layout (64 small + 16 medium + 4 large + 1 largest
+    instance, alternating TC2/TC6, here levels 2/3/4/5 instead of 5/6/7/9):
+    FP32 forward and FP32 TLM members on the FP64 base, one launch per stage
+    for all instances, equal to the instances run alone, bit for bit."""
+    print("packed uneven instances fp32 fwd + tlm", flush=True)
+    from mpas_sw_amd.core import SWModel
+    from mpas_sw_amd.ensemble import packed_instances
+    levels = [2] * 64 + [3] * 16 + [4] * 4 + [5]
+    specs = [(lv, 2 if i % 2 == 0 else 6) for i, lv in enumerate(levels)]
+    dt, n = 18.75, 8
+    mesh, phys, offs, perm = packed_instances(specs)
+    dh, du = perturbations(mesh, 1)
+    m = SWModel(mesh, phys, dt, test_case=2, perm=perm, precision="fp32", graphs=True)
+    hf, uf = m.to_internal(phys["h"], "c"), m.to_internal(phys["u"], "e")
+    m.fwd(hf, uf, n)
+    h = m.to_internal(phys["h"], "c", member=False)
+    u = m.to_internal(phys["u"], "e", member=False)
+    ht, ut = m.to_internal(dh[:, 0], "c"), m.to_internal(du[:, 0], "e")
+    m.tlm(h, u, ht, ut, n)
+    torch.cuda.synchronize()
+    got = [m.to_reference(x, k) for x, k in ((hf, "c"), (uf, "e"), (ht, "c"), (ut, "e"))]
+    for i in (0, 1, 64, 81, 84):
+        (level, tc), (oc, oe, ov) = specs[i], offs[i]
+        mi, pi = case(level, tc)
+        s = SWModel(mi, pi, dt, test_case=tc, precision="fp32")
+        hf1, uf1 = s.to_internal(pi["h"], "c"), s.to_internal(pi["u"], "e")
+        s.fwd(hf1, uf1, n)
+        h1 = s.to_internal(pi["h"], "c", member=False)
+        u1 = s.to_internal(pi["u"], "e", member=False)
+        ht1 = s.to_internal(dh[oc:oc + mi.nCells, 0], "c")
+        ut1 = s.to_internal(du[oe:oe + mi.nEdges, 0], "e")
+        s.tlm(h1, u1, ht1, ut1, n)
+        torch.cuda.synchronize()
+        want = [s.to_reference(x, k) for x, k in ((hf1, "c"), (uf1, "e"), (ht1, "c"), (ut1, "e"))]
+        sl = [slice(oc, oc + mi.nCells), slice(oe, oe + mi.nEdges)] * 2
+        for g, w, sli in zip(got, want, sl):
+            np.testing.assert_array_equal(g[sli], w, err_msg=f"instance {i}")
