@@ -174,6 +174,8 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
 // Adjoint of one RK4 stage: pa = F'(p_s)^T g, then the epilogue
 //   s>0: accx = (first ? pa : accx + pa) ; g = w[s-1]*lam + sw[s-1]*pa
 //   s==0: lam = (accx + lam) + pa
+// The first stage of a step (first = true) reads g = w[3]*lam directly from
+// lam, scaling on load; later stages read g from (gh, gu).
 template <typename T>
 void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, const RKCoeffs& k,
                const BaseStage& b, T* gh, T* gu, T* pe_ad, T* he_ad, T* ua_p, T* pc_ad, T* ha_p,

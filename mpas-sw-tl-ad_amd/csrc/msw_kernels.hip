@@ -874,7 +874,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_coef_cell(DMesh m, DConfig c, Ba
 
 // edges: pv_edge, h_edge and the partial u adjoints
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q,
+__global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q, T gs,
                                                       const T* __restrict__ gh,
                                                       const T* __restrict__ gu,
                                                       T* __restrict__ pe_ad,
@@ -892,9 +892,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q
 #pragma unroll
         for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
 #pragma unroll
-        for (int j = 0; j < kEoE; ++j) gx[j] = ld(gu, ix[j], B, b);
-        const T gh1 = ld(gh, c1, B, b), gh2 = ld(gh, c2, B, b);
-        const T ge = ld(gu, e, B, b);
+        for (int j = 0; j < kEoE; ++j) gx[j] = gs * ld(gu, ix[j], B, b);
+        const T gh1 = gs * ld(gh, c1, B, b), gh2 = gs * ld(gh, c2, B, b);
+        const T ge = gs * ld(gu, e, B, b);
         const double* r = q.e48 + 24L * e;
         const double* s = q.e8 + 16L * e;
         const T flux_ad = gh2 * T(q.c32[32L * c2 + 24]) - gh1 * T(q.c32[32L * c1 + 24]);
@@ -919,7 +919,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q
 
 // cells: pv_cell, h and ke adjoints (+ divergence adjoint of del2/del4)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb, AdjCoef q,
+__global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb, AdjCoef q, T gs,
                                                       const T* __restrict__ gu,
                                                       const T* __restrict__ pe_ad,
                                                       const T* __restrict__ he_ad,
@@ -940,7 +940,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
         for (int i = 0; i < kC6; ++i) ie[i] = m.eoc[kC6 * ic + i];
 #pragma unroll
         for (int i = 0; i < kC6; ++i) {
-            ge[i] = ld(gu, ie[i], B, b);
+            ge[i] = gs * ld(gu, ie[i], B, b);
             pa[i] = ld(pe_ad, ie[i], B, b);
             ha[i] = ld(he_ad, ie[i], B, b);
         }
@@ -974,7 +974,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
 // vertices: h_vertex and circulation adjoints (+ vorticity adjoint of
 // del2/del4, :664-665, :679-680)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int lb, AdjCoef q,
+__global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int lb, AdjCoef q, T gs,
                                                        const T* __restrict__ pe_ad,
                                                        const T* __restrict__ pc_ad,
                                                        T* __restrict__ hv_ad,
@@ -1003,7 +1003,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 T x = T(0);
-                if (c.visc2 > 0.0) x = T(c.visc2) * ld(gu, ie[i], B, b);
+                if (c.visc2 > 0.0) x = T(c.visc2) * (gs * ld(gu, ie[i], B, b));
                 if (c.visc4 > 0.0) x = x + ld(dsu_ad, ie[i], B, b);
                 vv = vv - T(m.sgnV[3 * iv + i]) * x / T(m.dv[ie[i]]);
             }
@@ -1018,7 +1018,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
 // tend_u_ad, the adjoints of the vorticity (times 1/areaTriangle: the
 // circulation adjoint) and of the divergence (times 1/areaCell) of nabla^2 u
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_del4_vertex(DMesh m, DConfig c, int lb,
+__global__ __launch_bounds__(kBlock) void k_adj_del4_vertex(DMesh m, DConfig c, int lb, T gs,
                                                             const T* __restrict__ gu,
                                                             T* __restrict__ dsc_ad)
 {
@@ -1032,7 +1032,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_del4_vertex(DMesh m, DConfig c, 
 #pragma unroll
         for (int i = 0; i < 3; ++i) ie[i] = m.eovR[3 * iv + i];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) g[i] = ld(gu, ie[i], B, b);
+        for (int i = 0; i < 3; ++i) g[i] = gs * ld(gu, ie[i], B, b);
         T a = T(0);
 #pragma unroll
         for (int i = 0; i < 3; ++i) a = a + T(m.sgnV[3 * iv + i]) * (T(c.visc4) * g[i]) / T(m.dv[ie[i]]);
@@ -1041,7 +1041,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_del4_vertex(DMesh m, DConfig c, 
 }
 
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_del4_cell(DMesh m, DConfig c, int lb,
+__global__ __launch_bounds__(kBlock) void k_adj_del4_cell(DMesh m, DConfig c, int lb, T gs,
                                                           const T* __restrict__ gu,
                                                           T* __restrict__ dsd_ad)
 {
@@ -1055,7 +1055,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_del4_cell(DMesh m, DConfig c, in
 #pragma unroll
         for (int i = 0; i < kC6; ++i) ie[i] = m.eoc[kC6 * ic + i];
 #pragma unroll
-        for (int i = 0; i < kC6; ++i) g[i] = ld(gu, ie[i], B, b);
+        for (int i = 0; i < kC6; ++i) g[i] = gs * ld(gu, ie[i], B, b);
         T a = T(0);
 #pragma unroll
         for (int i = 0; i < kC6; ++i) a = a + T(m.sgnC[kC6 * ic + i]) * (T(c.visc4) * g[i]) / T(m.dc[ie[i]]);
@@ -1459,23 +1459,29 @@ void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, 
                T* ke_ad, T* hv_ad, T* circ_ad, const DissipScratch<T>& ds, const AdjCoef& q,
                T* lamh, T* lamu, T* acch, T* accu, hipStream_t s)
 {
+    // the stage's tendency adjoint: w[3]*lam in the first stage (scaled as it
+    // is loaded, the same product a separate scaling pass would store), then
+    // what the previous stage's epilogue left in (gh, gu)
+    const T* rgh = first ? lamh : gh;
+    const T* rgu = first ? lamu : gu;
+    const T gs = first ? T(k.w[3]) : T(1);
     // ds.dsv: circulation adjoint of nabla^2 u [V]; ds.dsd: its divergence
     // adjoint [C]; ds.dsu: nabla^2 u adjoint [E]; ds.div: divergence
     // adjoint / area [C]
     if (c.visc4 > 0.0) {
         LAUNCH_WU(k_adj_del4_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, c, lb,
-                  gu, ds.dsv);
-        LAUNCH_WU(k_adj_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, c, lb, gu,
-                  ds.dsd);
+                  gs, rgu, ds.dsv);
+        LAUNCH_WU(k_adj_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, c, lb, gs,
+                  rgu, ds.dsd);
         LAUNCH_WU(k_adj_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, lb,
                   ds.dsd, ds.dsv, ds.dsu);
     }
-    LAUNCH_WU(k_adj_edge1, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, lb, q, gh, gu,
-              pe_ad, he_ad, ua_p);
-    LAUNCH_WU(k_adj_cell1, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, c, lb, q, gu,
-              pe_ad, he_ad, pc_ad, ha_p, ke_ad, ds.dsu, ds.div);
-    LAUNCH_WU(k_adj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, c, lb, q, pe_ad,
-              pc_ad, hv_ad, circ_ad, gu, ds.dsu);
+    LAUNCH_WU(k_adj_edge1, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, lb, q, gs, rgh,
+              rgu, pe_ad, he_ad, ua_p);
+    LAUNCH_WU(k_adj_cell1, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, c, lb, q, gs,
+              rgu, pe_ad, he_ad, pc_ad, ha_p, ke_ad, ds.dsu, ds.div);
+    LAUNCH_WU(k_adj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, c, lb, q, gs,
+              pe_ad, pc_ad, hv_ad, circ_ad, rgu, ds.dsu);
     const double w = stage > 0 ? k.w[stage - 1] : 0.0;
     const double sw = stage > 0 ? k.sw[stage - 1] : 0.0;
     LAUNCH_WU(k_adj_final, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2), s, m, c, lb, stage, first, T(w),

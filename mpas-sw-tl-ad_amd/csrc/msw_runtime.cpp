@@ -628,8 +628,6 @@ int enqueue_adj_step(msw_ctx* c, T* lamh, T* lamu, bool want_end)
     }
     T* g = static_cast<T*>(c->mpA);
     T* acc = static_cast<T*>(c->macc);
-    adj_seed<T>(static_cast<int>(C * B), g, lamh, k.w[3], c->stream);
-    adj_seed<T>(static_cast<int>(E * B), g + C * B, lamu, k.w[3], c->stream);
     for (int s = 3; s >= 0; --s) {
         HIP_TRY(hipStreamWaitEvent(c->stream, c->ev[1 + s], 0));
         adj_stage<T>(m, c->dcfg, c->lb, s, s == 3, k, c->st[s], g, g + C * B,

@@ -493,3 +493,71 @@ def test_launch_stripe_does_not_change_results_and_autotune():
     u = m.to_internal(phys["u"], "e", member=False)
     times = m.autotune(h, u, nsteps=1, candidates=(0, 2))
     assert set(times) == {0, 2} and m.stripe_log2 in times
+
+
+def test_wave_per_entity_batch_matches_oracle():
+    """B = 64 (a wavefront per entity: connectivity and coefficients are
+    scalar loads shared by the members): fused TLM within 1e-12 and adjoint
+    within 1e-11 of the oracle for sampled members, FP64."""
+    print("B=64 fused TLM + adjoint L3 tc5 vs oracle", flush=True)
+    tc, dt, B, n = 5, 900.0, 64, 4
+    mesh, phys = case(LEVEL, tc)
+    dh, du = perturbations(mesh, B)
+    m = model(LEVEL, tc, dt, batch=B, tl_fused_flux=True)
+    h = m.to_internal(phys["h"], "c", member=False)
+    u = m.to_internal(phys["u"], "e", member=False)
+    ht, ut = m.to_internal(dh, "c"), m.to_internal(du, "e")
+    th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+    tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+    m.tlm(h, u, ht, ut, n, th, tu)
+    torch.cuda.synchronize()
+    HT, UT = m.to_reference(ht, "c"), m.to_reference(ut, "e")
+    ha, ua = m.to_internal(dh, "c"), m.to_internal(du, "e")
+    m.adj(th, tu, ha, ua, n)
+    torch.cuda.synchronize()
+    HA, UA = m.to_reference(ha, "c"), m.to_reference(ua, "e")
+    for b in (0, 31, 63):
+        o = make_oracle(LEVEL, tc, dt)
+        o["h_tl1"][:] = dh[:, b]
+        o["u_tl1"][:] = du[:, b]
+        o.rk4_tlm(n)
+        assert rel_l2(HT[:, b], o["h_tl1"]) < 1e-12
+        assert rel_l2(UT[:, b], o["u_tl1"]) < 1e-12
+        o = make_oracle(LEVEL, tc, dt)
+        traj = []
+        for _ in range(n):
+            traj.append((o["h1"].copy(), o["u1"].copy()))
+            o.rk4()
+        o["h_ad1"][:] = dh[:, b]
+        o["u_ad1"][:] = du[:, b]
+        for s in range(n - 1, -1, -1):
+            o["h1"][:] = traj[s][0]
+            o["u1"][:] = traj[s][1]
+            o.rk4_adj()
+        assert rel_l2(HA[:, b], o["h_ad1"]) < 1e-11
+        assert rel_l2(UA[:, b], o["u_ad1"]) < 1e-11
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-3)])
+def test_wave_per_entity_dot_product_with_dissipation(prec, tol):
+    """<L x, y> = <x, L^T y> for 64 members (fused TL flux, all dissipation
+    and forcing terms on); FP64 inner products of FP32 members hold to FP32
+    rounding."""
+    print(f"B=64 dot-product L3 tc5 {prec}", flush=True)
+    dt, B = 900.0, 64
+    mesh, phys = case(LEVEL, 5)
+    xh, xu = perturbations(mesh, B, seed=9)
+    yh, yu = perturbations(mesh, B, seed=900)
+    m = _dissip_model(dt, batch=B, tl_fused_flux=True, precision=prec)
+    h0 = m.to_internal(phys["h"], "c", member=False)
+    u0 = m.to_internal(phys["u"], "e", member=False)
+    Lxh, Lxu = m.to_internal(xh, "c"), m.to_internal(xu, "e")
+    h, u = h0.clone(), u0.clone()
+    m.tlm(h, u, Lxh, Lxu, 1)
+    LTyh, LTyu = m.to_internal(yh, "c"), m.to_internal(yu, "e")
+    yh_d, yu_d = LTyh.clone(), LTyu.clone()
+    m.adj_step(h0, u0, LTyh, LTyu)
+    xh_d, xu_d = m.to_internal(xh, "c"), m.to_internal(xu, "e")
+    lhs = m.dot(Lxh, Lxu, yh_d, yu_d).cpu().numpy()
+    rhs = m.dot(xh_d, xu_d, LTyh, LTyu).cpu().numpy()
+    assert np.all(np.abs(lhs - rhs) <= tol * np.abs(lhs))
