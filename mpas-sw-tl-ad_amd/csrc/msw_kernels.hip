@@ -149,6 +149,74 @@ __device__ __forceinline__ T ld(const T* p, int row, int B, int b)
     return rowp(p, row, B)[b];
 }
 
+// Lane -> member mapping.  Plain form: lane L of a wavefront handles member
+// L of its 64-member chunk.  Pair form (FP64 with a wavefront per entity):
+// lane L handles member 2*(L & 31) + (L >> 5).  A gather then brings two
+// rows with one 16-byte load per lane -- lanes 0-31 read members 2l, 2l+1 of
+// one row, lanes 32-63 the same members of another row -- and a half-wave
+// swap (v_permlane32_swap) leaves every lane with its own member of both
+// rows.  Half the load instructions for the same bytes: the gathers of the
+// member kernels are bound by load instructions issued per row, not by HBM.
+// Only the lane assignment changes, so every sum keeps its order and the
+// results are the same bits as in the plain form.
+template <typename T, bool WU>
+struct PairForm {
+    static constexpr bool value = WU && sizeof(T) == 8;
+};
+
+struct Lane {
+    int b;      // member this lane computes and stores
+    int b2;     // even member of the pair a 16-byte load brings (pair form)
+    int half;   // 0 for lanes 0-31, 1 for lanes 32-63 (pair form)
+};
+
+template <bool PF>
+__device__ __forceinline__ Lane lane_of(unsigned t, int B)
+{
+    const int raw = static_cast<int>(t & static_cast<unsigned>(B - 1));
+    if constexpr (PF) {
+        const int l = raw & 63, base = raw - l;
+        return {base + 2 * (l & 31) + (l >> 5), base + 2 * (l & 31), l >> 5};
+    } else {
+        return {raw, raw, 0};
+    }
+}
+
+// row r0 of p0 and row r1 of p1 at this lane's member
+template <typename T, bool PF>
+__device__ __forceinline__ void ld2(const T* p0, int r0, const T* p1, int r1, int B, const Lane& ln,
+                                    T& v0, T& v1)
+{
+    if constexpr (PF) {
+        const T* p = ln.half ? rowp(p1, r1, B) : rowp(p0, r0, B);
+        const double2 q = *reinterpret_cast<const double2*>(p + ln.b2);
+        const unsigned xl = __double2loint(q.x), xh = __double2hiint(q.x);
+        const unsigned yl = __double2loint(q.y), yh = __double2hiint(q.y);
+        // lanes 32-63 of x trade with lanes 0-31 of y: x -> row r0, y -> row r1
+        const auto lo = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+        v0 = __hiloint2double(static_cast<int>(hi[0]), static_cast<int>(lo[0]));
+        v1 = __hiloint2double(static_cast<int>(hi[1]), static_cast<int>(lo[1]));
+    } else {
+        v0 = ld(p0, r0, B, ln.b);
+        v1 = ld(p1, r1, B, ln.b);
+    }
+}
+
+// rows r[0..N) of one array at this lane's member, two per load
+template <typename T, bool PF, int N>
+__device__ __forceinline__ void ld_rows(const T* p, const int (&r)[N], int B, const Lane& ln, T (&v)[N])
+{
+#pragma unroll
+    for (int k = 0; k + 1 < N; k += 2) ld2<T, PF>(p, r[k], p, r[k + 1], B, ln, v[k], v[k + 1]);
+    if constexpr (N % 2 == 1) v[N - 1] = ld(p, r[N - 1], B, ln.b);
+}
+
+#define MEMBER_LANE(T, WU)                                                     \
+    constexpr bool PF = PairForm<T, WU>::value;                                \
+    const Lane ln = lane_of<PF>(t, B);                                         \
+    const int b = ln.b;
+
 // ---------------------------------------------------------------------------
 // Nonlinear diagnostics (src/mpas_sw_dyncore.f90:1-361)
 
@@ -489,13 +557,14 @@ __global__ __launch_bounds__(kBlock) void k_tl_vertex(DMesh m, int lb, const dou
     ENTITY_ITEM(m.nV)
     {
         const int iv = entity_of<WU>(t, lb);
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         int ie[3], ic[3];
         T ue[3], hc[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) { ie[i] = m.eovR[3 * iv + i]; ic[i] = m.cov[3 * iv + i]; }
-#pragma unroll
-        for (int i = 0; i < 3; ++i) { ue[i] = ld(u_tl, ie[i], B, b); hc[i] = ld(h_tl, ic[i], B, b); }
+        ld2<T, PF>(u_tl, ie[0], u_tl, ie[1], B, ln, ue[0], ue[1]);
+        ld2<T, PF>(u_tl, ie[2], h_tl, ic[0], B, ln, ue[2], hc[0]);
+        ld2<T, PF>(h_tl, ic[1], h_tl, ic[2], B, ln, hc[1], hc[2]);
         T ct = T(0);
 #pragma unroll
         for (int i = 0; i < 3; ++i) ct = ct + T(m.sgnV[3 * iv + i]) * (T(m.dc[ie[i]]) * ue[i]);
@@ -522,13 +591,13 @@ __global__ __launch_bounds__(kBlock) void k_tl_cell(DMesh m, int lb, const doubl
     ENTITY_ITEM(m.nC)
     {
         const int ic = entity_of<WU>(t, lb);
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         int ie[kC6], iv[kC6];
         T ue[kC6], pv[kC6];
 #pragma unroll
         for (int i = 0; i < kC6; ++i) { ie[i] = m.eoc[kC6 * ic + i]; iv[i] = m.vocR[kC6 * ic + i]; }
-#pragma unroll
-        for (int i = 0; i < kC6; ++i) { ue[i] = ld(u_tl, ie[i], B, b); pv[i] = ld(pvv_tl, iv[i], B, b); }
+        ld_rows<T, PF>(u_tl, ie, B, ln, ue);
+        ld_rows<T, PF>(pvv_tl, iv, B, ln, pv);
         const double* r = c32 + 32L * ic;
         const T area = T(m.areaC[ic]);
         T k = T(0);
@@ -561,18 +630,18 @@ __global__ __launch_bounds__(kBlock) void k_tl_edge(DMesh m, DConfig c, int lb, 
     ENTITY_ITEM(m.nE)
     {
         const int e = entity_of<WU>(t, lb);
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
         const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
         int ix[kEoE];
         T ux[kEoE];
 #pragma unroll
         for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
-#pragma unroll
-        for (int j = 0; j < kEoE; ++j) ux[j] = ld(u_tl, ix[j], B, b);
-        const T p1 = ld(pvv_tl, v1, B, b), p2 = ld(pvv_tl, v2, B, b);
-        const T q1 = ld(pc_tl, c1, B, b), q2 = ld(pc_tl, c2, B, b);
-        const T h1 = ld(h_tl, c1, B, b), h2 = ld(h_tl, c2, B, b);
+        ld_rows<T, PF>(u_tl, ix, B, ln, ux);
+        T p1, p2, q1, q2, h1, h2;
+        ld2<T, PF>(pvv_tl, v1, pvv_tl, v2, B, ln, p1, p2);
+        ld2<T, PF>(pc_tl, c1, pc_tl, c2, B, ln, q1, q2);
+        ld2<T, PF>(h_tl, c1, h_tl, c2, B, ln, h1, h2);
         const T ue = ld(u_tl, e, B, b);
         T vt = T(0);
 #pragma unroll
@@ -679,7 +748,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
     const int B = 1 << lb;
     EDGE_CELL_ITEM(is_edge, t)
     {
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         if (is_edge) {
             const int e = entity_of<WU>(t, lb);
             const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
@@ -687,28 +756,23 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
 #pragma unroll
             for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
             const T pte = ld(pe_tl, e, B, b);
-            const T k1 = ld(ke_tl, c1, B, b), k2 = ld(ke_tl, c2, B, b);
-            const T h1 = ld(h_tl, c1, B, b), h2 = ld(h_tl, c2, B, b);
+            T k1, k2, h1, h2;
+            ld2<T, PF>(ke_tl, c1, ke_tl, c2, B, ln, k1, k2);
+            ld2<T, PF>(h_tl, c1, h_tl, c2, B, ln, h1, h2);
             const double* r = e48 + 48L * e;
             T qt = T(0);
             if constexpr (FUSED) {
                 // he_tl holds the TL flux F_tl = u_tl*he + u*he_tl
                 T fx[kEoE], px[kEoE];
-#pragma unroll
-                for (int j = 0; j < kEoE; ++j) {
-                    fx[j] = ld(he_tl, ix[j], B, b);
-                    px[j] = ld(pe_tl, ix[j], B, b);
-                }
+                ld_rows<T, PF>(he_tl, ix, B, ln, fx);
+                ld_rows<T, PF>(pe_tl, ix, B, ln, px);
 #pragma unroll
                 for (int j = 0; j < kEoE; ++j) qt = qt + T(r[j]) * fx[j] + T(r[12 + j]) * (pte + px[j]);
             } else {
                 T ux[kEoE], px[kEoE], hx[kEoE];
-#pragma unroll
-                for (int j = 0; j < kEoE; ++j) {
-                    ux[j] = ld(u_tl, ix[j], B, b);
-                    px[j] = ld(pe_tl, ix[j], B, b);
-                    hx[j] = ld(he_tl, ix[j], B, b);
-                }
+                ld_rows<T, PF>(u_tl, ix, B, ln, ux);
+                ld_rows<T, PF>(pe_tl, ix, B, ln, px);
+                ld_rows<T, PF>(he_tl, ix, B, ln, hx);
 #pragma unroll
                 for (int j = 0; j < kEoE; ++j) {
                     // w*u_tl*workpv*he + w*u*workpv_tl*he + w*u*workpv*h_edge_tl
@@ -733,14 +797,13 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
             T th = T(0);
             if constexpr (FUSED) {
                 T fx[kC6];
-#pragma unroll
-                for (int k = 0; k < kC6; ++k) fx[k] = ld(he_tl, ie[k], B, b);
+                ld_rows<T, PF>(he_tl, ie, B, ln, fx);
 #pragma unroll
                 for (int k = 0; k < kC6; ++k) th = th + T(-m.sgnCR[kC6 * ic + k]) * (T(r[18 + k]) * fx[k]);
             } else {
                 T ue[kC6], hh[kC6];
-#pragma unroll
-                for (int k = 0; k < kC6; ++k) { ue[k] = ld(u_tl, ie[k], B, b); hh[k] = ld(he_tl, ie[k], B, b); }
+                ld_rows<T, PF>(u_tl, ie, B, ln, ue);
+                ld_rows<T, PF>(he_tl, ie, B, ln, hh);
 #pragma unroll
                 for (int k = 0; k < kC6; ++k) {
                     // u_tl*dv*h_edge + u*dv*h_edge_tl
@@ -885,15 +948,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q
     ENTITY_ITEM(m.nE)
     {
         const int e = entity_of<WU>(t, lb);
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
         int ix[kEoE];
         T gx[kEoE];
 #pragma unroll
         for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
+        ld_rows<T, PF>(gu, ix, B, ln, gx);
 #pragma unroll
-        for (int j = 0; j < kEoE; ++j) gx[j] = gs * ld(gu, ix[j], B, b);
-        const T gh1 = gs * ld(gh, c1, B, b), gh2 = gs * ld(gh, c2, B, b);
+        for (int j = 0; j < kEoE; ++j) gx[j] = gs * gx[j];
+        T gh1, gh2;
+        ld2<T, PF>(gh, c1, gh, c2, B, ln, gh1, gh2);
+        gh1 = gs * gh1;
+        gh2 = gs * gh2;
         const T ge = gs * ld(gu, e, B, b);
         const double* r = q.e48 + 24L * e;
         const double* s = q.e8 + 16L * e;
@@ -933,17 +1000,16 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
     ENTITY_ITEM(m.nC)
     {
         const int ic = entity_of<WU>(t, lb);
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         int ie[kC6];
         T ge[kC6], pa[kC6], ha[kC6];
 #pragma unroll
         for (int i = 0; i < kC6; ++i) ie[i] = m.eoc[kC6 * ic + i];
+        ld_rows<T, PF>(gu, ie, B, ln, ge);
+        ld_rows<T, PF>(pe_ad, ie, B, ln, pa);
+        ld_rows<T, PF>(he_ad, ie, B, ln, ha);
 #pragma unroll
-        for (int i = 0; i < kC6; ++i) {
-            ge[i] = gs * ld(gu, ie[i], B, b);
-            pa[i] = ld(pe_ad, ie[i], B, b);
-            ha[i] = ld(he_ad, ie[i], B, b);
-        }
+        for (int i = 0; i < kC6; ++i) ge[i] = gs * ge[i];
         const double* r = q.c32 + 32L * ic;
         T pcad = T(0), had = T(0), kad = T(0);
 #pragma unroll
@@ -986,13 +1052,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
     ENTITY_ITEM(m.nV)
     {
         const int iv = entity_of<WU>(t, lb);
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         int ic[3], ie[3];
         T pc[3], pe[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) { ic[i] = m.cov[3 * iv + i]; ie[i] = m.eovR[3 * iv + i]; }
-#pragma unroll
-        for (int i = 0; i < 3; ++i) { pc[i] = ld(pc_ad, ic[i], B, b); pe[i] = ld(pe_ad, ie[i], B, b); }
+        ld2<T, PF>(pc_ad, ic[0], pc_ad, ic[1], B, ln, pc[0], pc[1]);
+        ld2<T, PF>(pc_ad, ic[2], pe_ad, ie[0], B, ln, pc[2], pe[0]);
+        ld2<T, PF>(pe_ad, ie[1], pe_ad, ie[2], B, ln, pe[1], pe[2]);
         const double* r = q.v8 + 8L * iv;
         T pa = T(0);
 #pragma unroll
@@ -1118,18 +1185,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
     const int B = 1 << lb;
     EDGE_CELL_ITEM(is_edge, t)
     {
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         if (is_edge) {
             const int e = entity_of<WU>(t, lb);
             int ix[kEoE];
             T px[kEoE];
 #pragma unroll
             for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
-#pragma unroll
-            for (int j = 0; j < kEoE; ++j) px[j] = ld(pe_ad, ix[j], B, b);
+            ld_rows<T, PF>(pe_ad, ix, B, ln, px);
             const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
-            const T cv2 = ld(circ_ad, m.voe[2 * e + 1], B, b), cv1 = ld(circ_ad, m.voe[2 * e], B, b);
-            const T k1 = ld(ke_ad, c1, B, b), k2 = ld(ke_ad, c2, B, b);
+            T cv1, cv2, k1, k2;
+            ld2<T, PF>(circ_ad, m.voe[2 * e + 1], circ_ad, m.voe[2 * e], B, ln, cv2, cv1);
+            ld2<T, PF>(ke_ad, c1, ke_ad, c2, B, ln, k1, k2);
             const double* r = q.e48 + 24L * e + 12;
             T ua = ld(ua_p, e, B, b);
 #pragma unroll
@@ -1147,8 +1214,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
             T hx[kC6];
 #pragma unroll
             for (int k = 0; k < kC6; ++k) iv[k] = m.vocR[kC6 * ic + k];
-#pragma unroll
-            for (int k = 0; k < kC6; ++k) hx[k] = ld(hv_ad, iv[k], B, b);
+            ld_rows<T, PF>(hv_ad, iv, B, ln, hx);
             T ha = ld(ha_p, ic, B, b);
 #pragma unroll
             for (int k = 0; k < kC6; ++k) ha = ha + T(m.kiteCR[kC6 * ic + k]) * hx[k];

@@ -561,3 +561,28 @@ def test_wave_per_entity_dot_product_with_dissipation(prec, tol):
     lhs = m.dot(Lxh, Lxu, yh_d, yu_d).cpu().numpy()
     rhs = m.dot(xh_d, xu_d, LTyh, LTyu).cpu().numpy()
     assert np.all(np.abs(lhs - rhs) <= tol * np.abs(lhs))
+
+
+@pytest.mark.parametrize("tc,dt", CASES)
+def test_wave_per_entity_reference_order_tlm_bitwise(tc, dt):
+    """B = 64 in the reference term order: every member's TLM is the oracle's
+    to the bit (the lane assignment of the 64-member kernels changes which
+    lane computes a member, never the order of a sum)."""
+    print(f"B=64 reference-order TLM L{LEVEL} tc{tc} bitwise", flush=True)
+    B, n = 64, 3
+    mesh, phys = case(LEVEL, tc)
+    dh, du = perturbations(mesh, B, seed=41)
+    m = model(LEVEL, tc, dt, batch=B)
+    h = m.to_internal(phys["h"], "c", member=False)
+    u = m.to_internal(phys["u"], "e", member=False)
+    ht, ut = m.to_internal(dh, "c"), m.to_internal(du, "e")
+    m.tlm(h, u, ht, ut, n)
+    torch.cuda.synchronize()
+    HT, UT = m.to_reference(ht, "c"), m.to_reference(ut, "e")
+    for b in (0, 1, 32, 33, 62, 63):
+        o = make_oracle(LEVEL, tc, dt)
+        o["h_tl1"][:] = dh[:, b]
+        o["u_tl1"][:] = du[:, b]
+        o.rk4_tlm(n)
+        np.testing.assert_array_equal(HT[:, b], o["h_tl1"], err_msg=f"member {b}")
+        np.testing.assert_array_equal(UT[:, b], o["u_tl1"], err_msg=f"member {b}")
