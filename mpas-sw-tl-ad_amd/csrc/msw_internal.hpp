@@ -35,8 +35,10 @@ struct DMesh {
     const double* sgnC;   // [C][6]  +1 if the cell is cell1 of eoc[slot]
     const int* eocR;      // [C][6]  edges in ascending reference edge index
     const double* sgnCR;  // [C][6]  +1 if the cell is cell1 of eocR[k]
+    const double* dvCR;   // [C][6]  dvEdge of eocR[k]
     const int* vocR;      // [C][6]  vertices in ascending reference vertex index
     const double* kiteCR; // [C][6]  kite area of the cell at vocR[k]
+    const double* keC;    // [C][6]  0.25*dcEdge*dvEdge of eoc[k] (0 on padded slots)
     const double* areaC;  // [C]
     const double* hs;     // [C]
     // edges
@@ -54,6 +56,7 @@ struct DMesh {
     // vertices
     const int* eovR;      // [V][3]  edges in ascending reference edge index
     const double* sgnV;   // [V][3]  +1 if the vertex is vertex2 of eovR[k]
+    const double* sdcV;   // [V][3]  sgnV * dcEdge(eovR[k]) (a sign flip: exact)
     const int* cov;       // [V][3]  cellsOnVertex, slot order
     const double* kite;   // [V][3]
     const double* areaT;  // [V]
@@ -113,7 +116,8 @@ struct AdjCoef {
     double* e48;   // adjoint [E][24]: woeT*workpv, woeT*(-apvm*dt*gradPVt(eo)) per
                    //   edgesOnEdge slot; the TL stage uses [E][48]
     double* e8;    // [E][16]: u*dv, dv*he, sum_j 0.5*w*u*he, apvm*dt*gradPVn, drag/wind
-                   //   h_edge and u terms, ke coefficient, u, he, 0.5*u*he, 0...
+                   //   h_edge and u terms, ke coefficient, u, he, 0.5*u*he,
+                   //   1/areaCell of cell 1 and cell 2, 0...
     double* v8;    // [V][8]:  kite/areaCell[3], pv_edge terms[3], h_vertex, circulation
     double* c32;   // [C][32]: pv_cell[6], gravity[6], ke[6], h_edge[6], 1/areaCell, 0...
 };

@@ -239,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_vertex(DMesh m, int lb, const T*
         for (int i = 0; i < 3; ++i) { ue[i] = ld(u, ie[i], B, b); hc[i] = ld(h, ic[i], B, b); }
         T circ = T(0);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) circ = circ + T(m.sgnV[3 * iv + i]) * (T(m.dc[ie[i]]) * ue[i]);
+        for (int i = 0; i < 3; ++i) circ = circ + T(m.sdcV[3 * iv + i]) * ue[i];
         const T at = T(m.areaT[iv]);
         const T vo = circ / at;
         T hh = T(0);
@@ -272,10 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_cell(DMesh m, int lb, const T* _
         const T area = T(m.areaC[ic]);
         T k = T(0);
 #pragma unroll
-        for (int i = 0; i < kC6; ++i) {
-            const T mask = T(fabs(m.sgnC[kC6 * ic + i]));   // 1, or 0 on a padded slot
-            k = k + mask * (T(0.25) * T(m.dc[ie[i]]) * T(m.dv[ie[i]]) * (ue[i] * ue[i]));
-        }
+        for (int i = 0; i < kC6; ++i) k = k + T(m.keC[kC6 * ic + i]) * (ue[i] * ue[i]);   // 0 on a padded slot
         k = k / area;
         T p = T(0);
 #pragma unroll
@@ -407,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_div(DMesh m, int lb, const T* __rest
         for (int k = 0; k < kC6; ++k) ue[k] = ld(u, ie[k], B, b);
         T d = T(0);
 #pragma unroll
-        for (int k = 0; k < kC6; ++k) d = d + T(m.sgnCR[kC6 * ic + k]) * (ue[k] * T(m.dv[ie[k]]));
+        for (int k = 0; k < kC6; ++k) d = d + T(m.sgnCR[kC6 * ic + k]) * (ue[k] * T(m.dvCR[kC6 * ic + k]));
         const T r = T(1.0) / T(m.areaC[ic]);
         stn(&rowp(div, ic, B)[b], d * r);
     }
@@ -447,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_del4_vertex(DMesh m, int lb, const T
         for (int i = 0; i < 3; ++i) x[i] = ld(dsu, ie[i], B, b);
         T sc = T(0);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) sc = sc + T(m.sgnV[3 * iv + i]) * (T(m.dc[ie[i]]) * x[i]);
+        for (int i = 0; i < 3; ++i) sc = sc + T(m.sdcV[3 * iv + i]) * x[i];
         const T r = T(1.0) / T(m.areaT[iv]);
         stn(&rowp(dsv, iv, B)[b], sc * r);
     }
@@ -471,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_del4_cell(DMesh m, int lb, const T* 
         for (int k = 0; k < kC6; ++k) x[k] = ld(dsu, ie[k], B, b);
         T d = T(0);
 #pragma unroll
-        for (int k = 0; k < kC6; ++k) d = d + T(m.sgnCR[kC6 * ic + k]) * (x[k] * T(m.dv[ie[k]]));
+        for (int k = 0; k < kC6; ++k) d = d + T(m.sgnCR[kC6 * ic + k]) * (x[k] * T(m.dvCR[kC6 * ic + k]));
         const T r = T(1.0) / T(m.areaC[ic]);
         stn(&rowp(dsd, ic, B)[b], d * r);
     }
@@ -531,7 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
             T th = T(0);
 #pragma unroll
             for (int k = 0; k < kC6; ++k) {
-                const T flux = ue[k] * T(m.dv[ie[k]]) * hh[k];
+                const T flux = ue[k] * T(m.dvCR[kC6 * ic + k]) * hh[k];
                 th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
             }
             th = th / T(m.areaC[ic]);
@@ -567,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_vertex(DMesh m, int lb, const dou
         ld2<T, PF>(h_tl, ic[1], h_tl, ic[2], B, ln, hc[1], hc[2]);
         T ct = T(0);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) ct = ct + T(m.sgnV[3 * iv + i]) * (T(m.dc[ie[i]]) * ue[i]);
+        for (int i = 0; i < 3; ++i) ct = ct + T(m.sdcV[3 * iv + i]) * ue[i];
         const T at = T(m.areaT[iv]);
         const T vt = ct / at;
         T ht = T(0);
@@ -786,8 +783,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
             if (c.visc2 > 0.0 || c.visc4 > 0.0 || c.wind || c.drag)
                 tend = tend_forcing_tl(m, c, dp, bs, e, c1, c2, B, b, tend, ld(u_tl, e, B, b),
                                        FUSED ? T(0.5) * (h1 + h2) : ld(he_tl, e, B, b), k1, k2);
-            rk_update(stage, w, sw, tend, xu, accu, pu, e, B, b,
-                      c.frozen_u != 0);
+            rk_update(stage, w, sw, tend, xu, accu, pu, e, B, b, c.frozen_u != 0);
         } else {
             const int ic = entity_of<WU>(t, lb);
             int ie[kC6];
@@ -876,7 +872,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_coef_edge(DMesh m, DConfig c, Ba
             q[5] = uge;
         } else {
             q[6] = 0.25 * m.dc[e] * dv * 2.0 * ub;
-            for (int i = 10; i < 16; ++i) q[i] = 0.0;
+            q[10] = 1.0 / m.areaC[m.coe[2 * e]];
+            q[11] = 1.0 / m.areaC[m.coe[2 * e + 1]];
+            for (int i = 12; i < 16; ++i) q[i] = 0.0;
         }
     }
 }
@@ -964,7 +962,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q
         const T ge = gs * ld(gu, e, B, b);
         const double* r = q.e48 + 24L * e;
         const double* s = q.e8 + 16L * e;
-        const T flux_ad = gh2 * T(q.c32[32L * c2 + 24]) - gh1 * T(q.c32[32L * c1 + 24]);
+        const T flux_ad = gh2 * T(s[11]) - gh1 * T(s[10]);
         // edge eo's q contains this edge as a neighbour: with wt the weight of
         // this edge in eo's list, d q(eo) / d u(e) = wt*workpv*h_edge(e) and
         // / d h_edge(e) = wt*workpv*u(e), / d pv_edge(e) = 0.5*wt*u*h_edge

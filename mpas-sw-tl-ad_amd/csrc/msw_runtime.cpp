@@ -236,7 +236,8 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
     std::vector<int> nEoC(C), eoc(static_cast<size_t>(C) * kMaxEdges, 0);
     std::vector<double> sgnC(static_cast<size_t>(C) * kMaxEdges, 0.0);
     std::vector<int> eocR(static_cast<size_t>(C) * kMaxEdges, 0), vocR(static_cast<size_t>(C) * kMaxEdges, 0);
-    std::vector<double> sgnCR(static_cast<size_t>(C) * kMaxEdges, 0.0), kiteCR(static_cast<size_t>(C) * kMaxEdges, 0.0);
+    std::vector<double> sgnCR(static_cast<size_t>(C) * kMaxEdges, 0.0), kiteCR(static_cast<size_t>(C) * kMaxEdges, 0.0),
+        keC(static_cast<size_t>(C) * kMaxEdges, 0.0), dvCR(static_cast<size_t>(C) * kMaxEdges, 0.0);
     std::vector<double> areaC(C), hs(C), trig(static_cast<size_t>(C) * 4),
         coeffs(static_cast<size_t>(C) * kMaxEdges * 3, 0.0);
     // edges of each reference cell in ascending reference edge index
@@ -262,14 +263,17 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
             // padding: a valid neighbour with a zero sign / kite
             eoc[kMaxEdges * k + i] = ie[d->edgesOnCell[static_cast<long>(mE) * o] - 1];
             eocR[kMaxEdges * k + i] = ie[cellEdges[o][0]];
+            dvCR[kMaxEdges * k + i] = d->dvEdge[cellEdges[o][0]];
             vocR[kMaxEdges * k + i] = iv[cellVerts[o][0]];
         }
         for (int i = 0; i < n; ++i) {
             int e = d->edgesOnCell[static_cast<long>(mE) * o + i] - 1;
             eoc[kMaxEdges * k + i] = ie[e];
             sgnC[kMaxEdges * k + i] = (d->cellsOnEdge[2 * e] - 1 == o) ? 1.0 : -1.0;
+            keC[kMaxEdges * k + i] = 0.25 * d->dcEdge[e] * d->dvEdge[e];
             int er = cellEdges[o][i];
             eocR[kMaxEdges * k + i] = ie[er];
+            dvCR[kMaxEdges * k + i] = d->dvEdge[er];
             sgnCR[kMaxEdges * k + i] = (d->cellsOnEdge[2 * er] - 1 == o) ? 1.0 : -1.0;
             int vr = cellVerts[o][i];
             vocR[kMaxEdges * k + i] = iv[vr];
@@ -343,7 +347,8 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
         if (v2 != v1) vertEdges[v2].push_back(e);
     }
     std::vector<int> eovR(3 * static_cast<size_t>(V)), cov(3 * static_cast<size_t>(V));
-    std::vector<double> sgnV(3 * static_cast<size_t>(V)), kite(3 * static_cast<size_t>(V)), areaT(V), fV(V);
+    std::vector<double> sgnV(3 * static_cast<size_t>(V)), sdcV(3 * static_cast<size_t>(V)),
+        kite(3 * static_cast<size_t>(V)), areaT(V), fV(V);
     for (int k = 0; k < V; ++k) {
         const int o = pv[k];
         if (vertEdges[o].size() != 3)
@@ -352,6 +357,7 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
             const int e = vertEdges[o][i];
             eovR[3 * k + i] = ie[e];
             sgnV[3 * k + i] = (d->verticesOnEdge[2 * e + 1] - 1 == o) ? 1.0 : -1.0;
+            sdcV[3 * k + i] = sgnV[3 * k + i] * d->dcEdge[e];
             cov[3 * k + i] = ic[d->cellsOnVertex[3 * o + i] - 1];
             kite[3 * k + i] = d->kiteAreasOnVertex[3 * o + i];
         }
@@ -364,11 +370,11 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
     m.stripe_log2 = 2;
     int rc = 0;
 #define UP(dst, vec) if ((rc = c->upload(&(dst), vec))) return rc
-    UP(m.nEoC, nEoC); UP(m.eoc, eoc); UP(m.sgnC, sgnC); UP(m.eocR, eocR); UP(m.sgnCR, sgnCR);
-    UP(m.vocR, vocR); UP(m.kiteCR, kiteCR); UP(m.areaC, areaC); UP(m.hs, hs);
+    UP(m.nEoC, nEoC); UP(m.eoc, eoc); UP(m.sgnC, sgnC); UP(m.eocR, eocR); UP(m.sgnCR, sgnCR); UP(m.dvCR, dvCR);
+    UP(m.vocR, vocR); UP(m.kiteCR, kiteCR); UP(m.keC, keC); UP(m.areaC, areaC); UP(m.hs, hs);
     UP(m.coe, coe); UP(m.voe, voe); UP(m.nEoE, nEoE); UP(m.eoe, eoe); UP(m.woe, woe);
     UP(m.eoeT, eoeT); UP(m.woeT, woeT); UP(m.eSlot, eSlot); UP(m.dc, dc); UP(m.dv, dv);
-    UP(m.eovR, eovR); UP(m.sgnV, sgnV); UP(m.cov, cov); UP(m.kite, kite); UP(m.areaT, areaT);
+    UP(m.eovR, eovR); UP(m.sgnV, sgnV); UP(m.sdcV, sdcV); UP(m.cov, cov); UP(m.kite, kite); UP(m.areaT, areaT);
     UP(m.fV, fV);
     UP(c->coeffs, coeffs); UP(c->trig, trig);
     std::vector<int> vpc(pc.begin(), pc.end()), vpe(pe.begin(), pe.end()), vpv(pv.begin(), pv.end());
@@ -826,11 +832,19 @@ int msw_base_steps(msw_ctx* ctx, double* h, double* u, int nsteps, double* traj_
                      traj_h ? &pre : nullptr);
 }
 
+// FP64 member arrays of a 64-member-or-larger context move as 16-byte pairs
+static bool members_aligned(const msw_ctx* ctx, const void* a, const void* b)
+{
+    if (ctx->prec != MSW_FP64 || ctx->B < 64) return true;
+    return (reinterpret_cast<uintptr_t>(a) % 16 == 0) && (reinterpret_cast<uintptr_t>(b) % 16 == 0);
+}
+
 int msw_tlm_steps(msw_ctx* ctx, double* h, double* u, void* h_tl, void* u_tl, int nsteps,
                   double* traj_h, double* traj_u)
 {
     if (!ctx || !h || !u || !h_tl || !u_tl || nsteps < 0 || (!traj_h) != (!traj_u))
         return fail(MSW_EINVAL, "bad argument");
+    if (!members_aligned(ctx, h_tl, u_tl)) return fail(MSW_EINVAL, "h_tl/u_tl must be 16-byte aligned");
     int rc = check_cfg(ctx);
     if (rc) return rc;
     const size_t C = ctx->nC, E = ctx->nE;
@@ -852,6 +866,7 @@ static int adj_common(msw_ctx* ctx, void* h_ad, void* u_ad, int nsteps, bool wan
                       const std::function<void(int)>& pre)
 {
     if (ctx->dcfg.frozen_u) return fail(MSW_EUNSUPPORTED, "adjoint of test case 1 is not implemented");
+    if (!members_aligned(ctx, h_ad, u_ad)) return fail(MSW_EINVAL, "h_ad/u_ad must be 16-byte aligned");
     if (ctx->prec == MSW_FP64)
         return run_steps(ctx, want_end ? 6 : 7, h_ad, u_ad, nullptr, nullptr, nsteps, [&] {
             return enqueue_adj_step<double>(ctx, static_cast<double*>(h_ad), static_cast<double*>(u_ad), want_end);
