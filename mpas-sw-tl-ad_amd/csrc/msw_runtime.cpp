@@ -97,6 +97,11 @@ struct msw_ctx {
     double* usrc = nullptr;           // [E]  wind-stress source, internal order
     AdjCoef adjq{};                   // stage coefficients (B = 1) of the eager single-stream path
     AdjCoef qs[4]{};                  // per-stage coefficients of the two-stream steps
+    // second base-stage / coefficient set and step-start state: a multi-step
+    // adjoint sweep forms the next step's set while the current one is used
+    BaseStage stB[4]{};
+    AdjCoef qsB[4]{};
+    double* bxB = nullptr;            // [C+E]
     double* st0 = nullptr;            // [C+E] copy of the TLM step-start state (stage 0)
     hipStream_t side = nullptr;       // base trajectory stream (default priority)
     hipEvent_t ev[8]{};               // fork, stage-ready x4, join
@@ -387,21 +392,23 @@ int alloc_buffers(msw_ctx* c)
 {
     const size_t C = c->nC, E = c->nE, V = c->nV, B = c->B;
     int rc;
-    for (int s = 0; s < 4; ++s) {
-        BaseStage& b = c->st[s];
-        if (s > 0) {
-            if ((rc = c->alloc(&b.h, C + E))) return rc;
-            b.u = b.h + C;
+    for (BaseStage* set : {c->st, c->stB})
+        for (int s = 0; s < 4; ++s) {
+            BaseStage& b = set[s];
+            if (s > 0) {
+                if ((rc = c->alloc(&b.h, C + E))) return rc;
+                b.u = b.h + C;
+            }
+            double* eb = nullptr;
+            if ((rc = c->alloc(&eb, 5 * E + 2 * C + 3 * V))) return rc;
+            b.he = eb; b.pe = eb + E; b.v = eb + 2 * E; b.gpt = eb + 3 * E; b.gpn = eb + 4 * E;
+            b.ke = eb + 5 * E; b.pc = b.ke + C;
+            b.pvv = b.pc + C; b.hv = b.pvv + V; b.vort = b.hv + V;
+            if ((rc = c->alloc(&b.rec, 4 * E))) return rc;
         }
-        double* eb = nullptr;
-        if ((rc = c->alloc(&eb, 5 * E + 2 * C + 3 * V))) return rc;
-        b.he = eb; b.pe = eb + E; b.v = eb + 2 * E; b.gpt = eb + 3 * E; b.gpn = eb + 4 * E;
-        b.ke = eb + 5 * E; b.pc = b.ke + C;
-        b.pvv = b.pc + C; b.hv = b.pvv + V; b.vort = b.hv + V;
-        if ((rc = c->alloc(&b.rec, 4 * E))) return rc;
-    }
     if ((rc = c->alloc(&c->bacc, C + E))) return rc;
     if ((rc = c->alloc(&c->bx, C + E))) return rc;
+    if ((rc = c->alloc(&c->bxB, C + E))) return rc;
     if ((rc = c->alloc(&c->bend, C + E))) return rc;
     if ((rc = c->alloc(&c->dextra, V + 2 * C))) return rc;
     const size_t es = c->esz();
@@ -419,14 +426,15 @@ int alloc_buffers(msw_ctx* c)
     c->adjq.e8 = q + 48 * E;
     c->adjq.v8 = q + 64 * E;
     c->adjq.c32 = q + 64 * E + 8 * V;
-    for (int st = 0; st < 4; ++st) {
-        double* qq = nullptr;
-        if ((rc = c->alloc(&qq, 64 * E + 8 * V + 32 * C))) return rc;
-        c->qs[st].e48 = qq;
-        c->qs[st].e8 = qq + 48 * E;
-        c->qs[st].v8 = qq + 64 * E;
-        c->qs[st].c32 = qq + 64 * E + 8 * V;
-    }
+    for (AdjCoef* set : {c->qs, c->qsB})
+        for (int st = 0; st < 4; ++st) {
+            double* qq = nullptr;
+            if ((rc = c->alloc(&qq, 64 * E + 8 * V + 32 * C))) return rc;
+            set[st].e48 = qq;
+            set[st].e8 = qq + 48 * E;
+            set[st].v8 = qq + 64 * E;
+            set[st].c32 = qq + 64 * E + 8 * V;
+        }
     if ((rc = c->alloc(&c->st0, C + E))) return rc;
     c->dot_nblk = dot_blocks(std::max<long>(C, E));
     if ((rc = c->alloc(&c->dpartial, static_cast<size_t>(c->dot_nblk) * B + 2 * B))) return rc;
@@ -599,6 +607,59 @@ int enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
     return join_side(c);
 }
 
+// Side stream: the four base stages of an adjoint step whose start state is
+// in x (FP64, B = 1) into `st`, then the coefficient rows of stages 3, 2, 1, 0
+// into `qs`; with `events`, ev[1+s] is recorded once stage s's rows exist.
+int side_adj_base(msw_ctx* c, BaseStage* st, AdjCoef* qs, double* x, bool want_end, bool events)
+{
+    const DMesh& m = c->dm;
+    const RKCoeffs k = rk_coeffs(c->cfg.dt);
+    const size_t C = c->nC;
+    hipStream_t ss = c->side;
+    st[0].h = x;
+    st[0].u = x + C;
+    for (int s = 0; s < 4; ++s) {
+        BaseStage& b = st[s];
+        fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
+                         b.gpt, b.gpn, b.rec, ss);
+        if (s < 3 || want_end) {
+            // stage 3 writes x_new into its "x" argument: give it the end buffer
+            double* nh = s < 3 ? st[s + 1].h : c->bend;
+            double* wxh = s < 3 ? x : c->bend;
+            double* wxu = s < 3 ? x + C : c->bend + C;
+            base_tend(c, b, s, k, wxh, wxu, nh, ss);
+        }
+    }
+    for (int s = 3; s >= 0; --s) {
+        adj_coef(m, c->dcfg, st[s], qs[s], ss);
+        if (events) HIP_TRY(hipEventRecord(c->ev[1 + s], ss));
+    }
+    return MSW_OK;
+}
+
+// Main stream: the member adjoint stages 3, 2, 1, 0 with coefficient set
+// `qs`; with `wait`, stage s first waits for ev[1+s].  The first stage reads
+// the tendency adjoint w[3]*lam from lam itself.
+template <typename T>
+int main_adj_stages(msw_ctx* c, const BaseStage* st, const AdjCoef* qs, T* lamh, T* lamu, bool wait)
+{
+    const DMesh& m = c->dm;
+    const RKCoeffs k = rk_coeffs(c->cfg.dt);
+    const size_t C = c->nC, B = c->B;
+    T* g = static_cast<T*>(c->mpA);
+    T* acc = static_cast<T*>(c->macc);
+    for (int s = 3; s >= 0; --s) {
+        if (wait) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev[1 + s], 0));
+        adj_stage<T>(m, c->dcfg, c->lb, s, s == 3, k, st[s], g, g + C * B,
+                     static_cast<T*>(c->me1), static_cast<T*>(c->me2), static_cast<T*>(c->me3),
+                     static_cast<T*>(c->mc1), static_cast<T*>(c->mc2), static_cast<T*>(c->mc3),
+                     static_cast<T*>(c->mv1),
+                     static_cast<T*>(c->mv2), member_ds<T>(c), qs[s], lamh, lamu, acc,
+                     acc + C * B, c->stream);
+    }
+    return MSW_OK;
+}
+
 // One adjoint step; the base step-start state must already be in c->bx.
 // The side stream recomputes the four base stages, then forms the
 // coefficient rows of stages 3, 2, 1, 0 while the main stream runs the
@@ -606,43 +667,31 @@ int enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
 template <typename T>
 int enqueue_adj_step(msw_ctx* c, T* lamh, T* lamu, bool want_end)
 {
-    const DMesh& m = c->dm;
-    const RKCoeffs k = rk_coeffs(c->cfg.dt);
-    const size_t C = c->nC, E = c->nE, B = c->B;
-    double* xh = c->bx;
-    double* xu = c->bx + C;
-    hipStream_t ss = c->side;
     int rc = fork_side(c);
     if (rc) return rc;
-    c->st[0].h = xh;
-    c->st[0].u = xu;
-    for (int s = 0; s < 4; ++s) {
-        BaseStage& b = c->st[s];
-        fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
-                         b.gpt, b.gpn, b.rec, ss);
-        if (s < 3 || want_end) {
-            // stage 3 writes x_new into its "x" argument: give it the end buffer
-            double* nh = s < 3 ? c->st[s + 1].h : c->bend;
-            double* wxh = s < 3 ? xh : c->bend;
-            double* wxu = s < 3 ? xu : c->bend + C;
-            base_tend(c, b, s, k, wxh, wxu, nh, ss);
-        }
+    if ((rc = side_adj_base(c, c->st, c->qs, c->bx, want_end, true))) return rc;
+    if ((rc = main_adj_stages<T>(c, c->st, c->qs, lamh, lamu, true))) return rc;
+    return join_side(c);
+}
+
+// One step of a multi-step adjoint sweep: the member stages use base set
+// `par` (formed during the previous step), while the side stream forms the
+// other set for the next step from the start state already copied into that
+// set's buffer (when `next`).
+template <typename T>
+int enqueue_adj_step_piped(msw_ctx* c, int par, bool next, T* lamh, T* lamu)
+{
+    BaseStage* cur = par ? c->stB : c->st;
+    AdjCoef* qcur = par ? c->qsB : c->qs;
+    int rc = fork_side(c);
+    if (rc) return rc;
+    if (next) {
+        BaseStage* nst = par ? c->st : c->stB;
+        AdjCoef* nq = par ? c->qs : c->qsB;
+        double* nx = par ? c->bx : c->bxB;
+        if ((rc = side_adj_base(c, nst, nq, nx, false, false))) return rc;
     }
-    for (int s = 3; s >= 0; --s) {
-        adj_coef(m, c->dcfg, c->st[s], c->qs[s], ss);
-        HIP_TRY(hipEventRecord(c->ev[1 + s], ss));
-    }
-    T* g = static_cast<T*>(c->mpA);
-    T* acc = static_cast<T*>(c->macc);
-    for (int s = 3; s >= 0; --s) {
-        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev[1 + s], 0));
-        adj_stage<T>(m, c->dcfg, c->lb, s, s == 3, k, c->st[s], g, g + C * B,
-                     static_cast<T*>(c->me1), static_cast<T*>(c->me2), static_cast<T*>(c->me3),
-                     static_cast<T*>(c->mc1), static_cast<T*>(c->mc2), static_cast<T*>(c->mc3),
-                     static_cast<T*>(c->mv1),
-                     static_cast<T*>(c->mv2), member_ds<T>(c), c->qs[s], lamh, lamu, acc,
-                     acc + C * B, c->stream);
-    }
+    if ((rc = main_adj_stages<T>(c, cur, qcur, lamh, lamu, false))) return rc;
     return join_side(c);
 }
 
@@ -654,52 +703,58 @@ int check_cfg(msw_ctx* c)
     return MSW_OK;
 }
 
-// Run `body` once per step, either directly or through a captured graph
-// keyed by (kind, pointers).
+// Run `body(v)` once per step, either directly or through a captured graph
+// keyed by (kind, variant, pointers); step i uses variant `variant(i)`
+// (0 when no variant function is given).
 template <typename F>
-int run_steps(msw_ctx* c, int kind, const void* p0, const void* p1, const void* p2, const void* p3,
-              int nsteps, F&& body, const std::function<void(int)>* pre = nullptr)
+int run_steps_var(msw_ctx* c, int kind, const void* p0, const void* p1, const void* p2, const void* p3,
+                  int nsteps, F&& body, const std::function<void(int)>* pre,
+                  const std::function<int(int)>* variant)
 {
-    if (!c->use_graphs) {
-        for (int i = 0; i < nsteps; ++i) {
-            if (pre) (*pre)(i);
-            const int brc = body();
-            if (brc) return brc;
-        }
-        HIP_TRY(hipGetLastError());
-        return MSW_OK;
-    }
-    auto key = std::make_tuple(kind, p0, p1, p2, p3);
-    auto it = c->graphs.find(key);
-    if (it == c->graphs.end()) {
-        if (!c->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
-        hipStream_t user = c->stream;
-        c->stream = c->cap_stream;
-        hipGraph_t graph = nullptr;
-        int brc = MSW_OK;
-        hipError_t e = hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-            brc = body();
-            e = hipStreamEndCapture(c->cap_stream, &graph);
-        }
-        c->stream = user;
-        if (brc) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return brc;
-        }
-        if (e != hipSuccess) return fail(MSW_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-        hipGraphExec_t exec = nullptr;
-        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return fail(MSW_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-        it = c->graphs.emplace(key, exec).first;
-    }
     for (int i = 0; i < nsteps; ++i) {
         if (pre) (*pre)(i);
+        const int v = variant ? (*variant)(i) : 0;
+        if (!c->use_graphs) {
+            const int brc = body(v);
+            if (brc) return brc;
+            continue;
+        }
+        auto key = std::make_tuple(kind * 16 + v, p0, p1, p2, p3);
+        auto it = c->graphs.find(key);
+        if (it == c->graphs.end()) {
+            if (!c->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+            hipStream_t user = c->stream;
+            c->stream = c->cap_stream;
+            hipGraph_t graph = nullptr;
+            int brc = MSW_OK;
+            hipError_t e = hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                brc = body(v);
+                e = hipStreamEndCapture(c->cap_stream, &graph);
+            }
+            c->stream = user;
+            if (brc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return brc;
+            }
+            if (e != hipSuccess) return fail(MSW_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+            hipGraphExec_t exec = nullptr;
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(MSW_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            it = c->graphs.emplace(key, exec).first;
+        }
         HIP_TRY(hipGraphLaunch(it->second, c->stream));
     }
     HIP_TRY(hipGetLastError());
     return MSW_OK;
+}
+
+template <typename F>
+int run_steps(msw_ctx* c, int kind, const void* p0, const void* p1, const void* p2, const void* p3,
+              int nsteps, F&& body, const std::function<void(int)>* pre = nullptr)
+{
+    return run_steps_var(c, kind, p0, p1, p2, p3, nsteps, [&](int) { return body(); }, pre, nullptr);
 }
 
 }  // namespace
@@ -901,13 +956,34 @@ int msw_adj_steps(msw_ctx* ctx, const double* traj_h, const double* traj_u, void
     if (!ctx || !traj_h || !traj_u || !h_ad || !u_ad || nsteps < 0) return fail(MSW_EINVAL, "bad argument");
     int rc = check_cfg(ctx);
     if (rc) return rc;
+    if (ctx->dcfg.frozen_u) return fail(MSW_EUNSUPPORTED, "adjoint of test case 1 is not implemented");
+    if (!members_aligned(ctx, h_ad, u_ad)) return fail(MSW_EINVAL, "h_ad/u_ad must be 16-byte aligned");
+    if (nsteps == 0) return MSW_OK;
     const size_t C = ctx->nC, E = ctx->nE;
-    std::function<void(int)> pre = [&](int i) {
-        const size_t k = static_cast<size_t>(nsteps - 1 - i);
-        (void)hipMemcpyAsync(ctx->bx, traj_h + k * C, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
-        (void)hipMemcpyAsync(ctx->bx + C, traj_u + k * E, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+    // step i (reverse order) runs the adjoint of step k = nsteps-1-i with base
+    // set i % 2, formed while step i-1 ran; before it, the start state of step
+    // k-1 goes into the other set's buffer for the side stream to work on
+    auto copy_start = [&](size_t k, double* dst) {
+        (void)hipMemcpyAsync(dst, traj_h + k * C, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+        (void)hipMemcpyAsync(dst + C, traj_u + k * E, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
     };
-    return adj_common(ctx, h_ad, u_ad, nsteps, false, pre);
+    copy_start(static_cast<size_t>(nsteps - 1), ctx->bx);
+    if ((rc = fork_side(ctx))) return rc;
+    if ((rc = side_adj_base(ctx, ctx->st, ctx->qs, ctx->bx, false, false))) return rc;
+    if ((rc = join_side(ctx))) return rc;
+    std::function<void(int)> pre = [&](int i) {
+        if (i + 1 < nsteps) copy_start(static_cast<size_t>(nsteps - 2 - i), (i % 2) ? ctx->bx : ctx->bxB);
+    };
+    std::function<int(int)> variant = [&](int i) { return (i % 2) * 2 + (i + 1 < nsteps ? 1 : 0); };
+    if (ctx->prec == MSW_FP64)
+        return run_steps_var(ctx, 10, h_ad, u_ad, nullptr, nullptr, nsteps, [&](int v) {
+            return enqueue_adj_step_piped<double>(ctx, v >> 1, (v & 1) != 0, static_cast<double*>(h_ad),
+                                                  static_cast<double*>(u_ad));
+        }, &pre, &variant);
+    return run_steps_var(ctx, 11, h_ad, u_ad, nullptr, nullptr, nsteps, [&](int v) {
+        return enqueue_adj_step_piped<float>(ctx, v >> 1, (v & 1) != 0, static_cast<float*>(h_ad),
+                                             static_cast<float*>(u_ad));
+    }, &pre, &variant);
 }
 
 static const char* kDiagNames =
