@@ -80,7 +80,7 @@ def run_window(m, bufs, h0, u0, ht0, ut0, nsteps):
     return wall, ev[0].elapsed_time(ev[1]) / 1e3, ev[1].elapsed_time(ev[2]) / 1e3
 
 
-def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True, fused=True):
+def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True, fused=True, tune=True):
     import torch
     from mpas_sw_amd.core import SWModel
     m = SWModel(mesh, phys, DT, test_case=TC, batch=batch, precision=precision, graphs=graphs,
@@ -90,8 +90,11 @@ def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True, f
     sel = slice(0, batch)
     ht0 = m.to_internal(dh[:, sel] if batch > 1 else dh[:, 0], "c")
     ut0 = m.to_internal(du[:, sel] if batch > 1 else du[:, 0], "e")
-    tune = m.autotune(h0, u0)     # launch stripe per XCD, chosen by timing (not timed here)
-    log(f"stripe autotune B={batch} {precision}: {tune}")
+    if tune:
+        tune = m.autotune(h0, u0)     # launch stripe per XCD, chosen by timing (not timed here)
+        log(f"stripe autotune B={batch} {precision}: {tune}")
+    else:
+        tune = {m.stripe_log2: 0.0}
     n = max(steps, warmup)
     bufs = (torch.empty_like(h0), torch.empty_like(u0), torch.empty_like(ht0), torch.empty_like(ut0),
             torch.empty((n, m.nC), dtype=torch.float64, device=m.device),
@@ -124,7 +127,7 @@ def calibration_copy(nbytes=1 << 30, reps=3):
 
 def traffic_from_counters(fetch_dir, write_dir, nsteps, out_json, out_md, command):
     """HBM bytes per TLM+ADJ step from two rocprofv3 --pmc runs of
-    `bench.py --steps S --warmup W --no-cpu --no-plain --no-fp32 --calibrate`
+    `bench.py --steps S --warmup W --no-cpu --no-plain --no-fp32 --no-tune --calibrate`
     (FETCH_SIZE, then WRITE_SIZE; counters in KiB per dispatch).  Each counter
     is scaled by what it reads for the 1 GiB calibration copy; the library's
     kernels over the whole run are summed and divided by the S + W window
@@ -206,6 +209,8 @@ def main():
     ap.add_argument("--level", type=int, default=LEVEL)
     ap.add_argument("--no-graphs", action="store_true", help="launch stage kernels directly")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch stripe (no autotuning runs before the timed window)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 1 GiB device copy (reference for HBM counters)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -214,9 +219,9 @@ def main():
     args = ap.parse_args()
     if args.traffic_from:
         cmd = ("rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc1 -o run -- python3 bench.py "
-               f"--steps {args.steps} --warmup {args.warmup} --no-cpu --no-plain --no-fp32 --calibrate\n"
+               f"--steps {args.steps} --warmup {args.warmup} --no-cpu --no-plain --no-fp32 --no-tune --calibrate\n"
                "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc2 -o run -- python3 bench.py "
-               f"--steps {args.steps} --warmup {args.warmup} --no-cpu --no-plain --no-fp32 --calibrate\n"
+               f"--steps {args.steps} --warmup {args.warmup} --no-cpu --no-plain --no-fp32 --no-tune --calibrate\n"
                f"python3 bench.py --steps {args.steps} --warmup {args.warmup} --traffic-from "
                "gpurun_out/pmc1 gpurun_out/pmc2")
         r = traffic_from_counters(args.traffic_from[0], args.traffic_from[1], args.steps + args.warmup,
@@ -242,7 +247,7 @@ def main():
     if args.calibrate:
         calibration_copy()
     g = not args.no_graphs
-    res = gpu_rate(mesh, phys, dh, du, BATCH, "fp64", args.steps, args.warmup, g)
+    res = gpu_rate(mesh, phys, dh, du, BATCH, "fp64", args.steps, args.warmup, g, tune=not args.no_tune)
     log(f"batched fp64: {res}")
     exact = None
     if not args.no_plain:
