@@ -672,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_coef(DMesh m, BaseStage bs, int f
     {
         if (t < ne) {
             const int e = static_cast<int>(t >> 4), j = static_cast<int>(t & 15);
-            if (j >= kE2) return;
+            if (j >= kEoE) return;   // the member kernels read the 10 real slots only
             const int eo = m.eoe[kE2 * e + j];
             const double4 ro = *reinterpret_cast<const double4*>(bs.rec + 4L * eo);  // pe u he gpt
             const double wpv = 0.5 * (bs.pe[e] + ro.x);
@@ -829,13 +829,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_coef_edge(DMesh m, DConfig c, Ba
                                                           double* __restrict__ e16)
 {
     // 16 lanes per edge (all of them active through the reduction below):
-    // lanes 0-11 fill the edgesOnEdge slots, lanes 12-15 the per-edge values
+    // lanes 0-9 fill the edgesOnEdge slots, lanes 12-15 the per-edge values
     const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * kBlock + threadIdx.x;
     const unsigned total = 16u * static_cast<unsigned>(m.nE);
     if (t >= total) return;   // whole 16-lane groups only, so the shuffles stay inside live lanes
     const int e = static_cast<int>(t >> 4), j = static_cast<int>(t & 15);
     double term = 0.0;   // 0.5*w*u(eo)*he(eo), summed over the slots
-    if (j < kE2) {
+    if (j < kEoE) {      // the member kernels read the 10 real slots only
         const int eo = m.eoe[kE2 * e + j];
         const double4 ro = *reinterpret_cast<const double4*>(bs.rec + 4L * eo);  // pe u he gpt
         const double wt = m.woeT[kE2 * e + j];

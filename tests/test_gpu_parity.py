@@ -563,13 +563,14 @@ def test_wave_per_entity_dot_product_with_dissipation(prec, tol):
     assert np.all(np.abs(lhs - rhs) <= tol * np.abs(lhs))
 
 
-@pytest.mark.parametrize("tc,dt", CASES)
-def test_wave_per_entity_reference_order_tlm_bitwise(tc, dt):
-    """B = 64 in the reference term order: every member's TLM is the oracle's
-    to the bit (the lane assignment of the 64-member kernels changes which
-    lane computes a member, never the order of a sum)."""
-    print(f"B=64 reference-order TLM L{LEVEL} tc{tc} bitwise", flush=True)
-    B, n = 64, 3
+@pytest.mark.parametrize("tc,dt,B", [(2, 1200.0, 64), (5, 900.0, 64), (6, 600.0, 64), (5, 900.0, 128)])
+def test_wave_per_entity_reference_order_tlm_bitwise(tc, dt, B):
+    """B = 64 and 128 (two wavefronts per entity) in the reference term order:
+    every sampled member's TLM is the oracle's to the bit (the lane
+    assignment of the 64-member kernels changes which lane computes a member,
+    never the order of a sum)."""
+    print(f"B={B} reference-order TLM L{LEVEL} tc{tc} bitwise", flush=True)
+    n = 3
     mesh, phys = case(LEVEL, tc)
     dh, du = perturbations(mesh, B, seed=41)
     m = model(LEVEL, tc, dt, batch=B)
@@ -579,7 +580,7 @@ def test_wave_per_entity_reference_order_tlm_bitwise(tc, dt):
     m.tlm(h, u, ht, ut, n)
     torch.cuda.synchronize()
     HT, UT = m.to_reference(ht, "c"), m.to_reference(ut, "e")
-    for b in (0, 1, 32, 33, 62, 63):
+    for b in sorted({0, 1, 32, 33, 62, 63, B - 64, B - 33, B - 1}):
         o = make_oracle(LEVEL, tc, dt)
         o["h_tl1"][:] = dh[:, b]
         o["u_tl1"][:] = du[:, b]
