@@ -104,10 +104,11 @@ def test_dissipation_window_against_reference_fortran():
 
 
 @pytest.mark.parametrize("tc", [2, 5])
-def test_fp32_forward_one_day_within_1e5_of_fp64(tc):
+def test_fp32_forward_one_and_five_days(tc):
     """FP32 members after one day (level 5, dt 300 s, 288 steps) within 1e-5
-    relative L2 of the FP64 state."""
-    print(f"fp32 forward 1 day L5 tc{tc}", flush=True)
+    relative L2 of the FP64 state; after five days (1440 steps) within 1e-4
+    (the drift grows with the flow's own error growth)."""
+    print(f"fp32 forward 1 and 5 days L5 tc{tc}", flush=True)
     mesh, phys = case(5, tc)
     out = {}
     for prec in ("fp64", "fp32"):
@@ -115,9 +116,16 @@ def test_fp32_forward_one_day_within_1e5_of_fp64(tc):
         h, u = m.to_internal(phys["h"], "c"), m.to_internal(phys["u"], "e")
         m.fwd(h, u, 288)
         torch.cuda.synchronize()
-        out[prec] = (m.to_reference(h, "c"), m.to_reference(u, "e"))
-    assert rel_l2(out["fp32"][0], out["fp64"][0]) < 1e-5
-    assert rel_l2(out["fp32"][1], out["fp64"][1]) < 1e-5
+        day1 = (m.to_reference(h, "c"), m.to_reference(u, "e"))
+        m.fwd(h, u, 4 * 288)
+        torch.cuda.synchronize()
+        out[prec] = (day1, (m.to_reference(h, "c"), m.to_reference(u, "e")))
+    (h1, u1), (h5, u5) = out["fp32"]
+    (g1, v1), (g5, v5) = out["fp64"]
+    err = [rel_l2(h1, g1), rel_l2(u1, v1), rel_l2(h5, g5), rel_l2(u5, v5)]
+    print("fp32 vs fp64 rel L2 h, u after 1 day and 5 days:", ["%.2e" % x for x in err], flush=True)
+    assert err[0] < 1e-5 and err[1] < 1e-5
+    assert err[2] < 1e-4 and err[3] < 1e-4
 
 
 def test_level6_day_window_linearisation_dot_product_and_gradient():
