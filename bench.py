@@ -183,6 +183,103 @@ def traffic_from_counters(fetch_dir, write_dir, nsteps, out_json, out_md, comman
     return res
 
 
+def packed_workload(steps, warmup):
+    """Uneven batch: 64 level-5 + 16 level-6 + 4 level-7 + 1 level-9 instances
+    (4.59 M cells, alternating TC2 / TC6) in one packed index space, so every
+    stage kernel covers all of them in one launch; FP32 forward and FP32 TLM
+    members (FP64 base), dt 18.75 s.  Returns cell-steps/s of each."""
+    import torch
+    from mpas_sw_amd.core import SWModel
+    from mpas_sw_amd.ensemble import packed_instances
+    from mpas_sw_amd.mesh import get_mesh
+    t = time.time()
+    for lv in (5, 6, 7, 9):
+        get_mesh(lv)
+        log(f"mesh level {lv}: {time.time() - t:.1f}s")
+    levels = [5] * 64 + [6] * 16 + [7] * 4 + [9]
+    specs = [(lv, 2 if i % 2 == 0 else 6) for i, lv in enumerate(levels)]
+    mesh, phys, offs, perm = packed_instances(specs)
+    log(f"packed set-up: {time.time() - t:.1f}s; C={mesh.nCells} E={mesh.nEdges} V={mesh.nVertices}")
+    m = SWModel(mesh, phys, 18.75, test_case=2, perm=perm, precision="fp32", graphs=True)
+    h = m.to_internal(phys["h"], "c")
+    u = m.to_internal(phys["u"], "e")
+    m.fwd(h, u, warmup)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    m.fwd(h, u, steps)
+    e1.record()
+    torch.cuda.synchronize()
+    t_fwd = e0.elapsed_time(e1) / 1e3
+    log(f"packed forward: {1e3 * t_fwd / steps:.3f} ms/step")
+    hb = m.to_internal(phys["h"], "c", member=False)
+    ub = m.to_internal(phys["u"], "e", member=False)
+    g = torch.Generator("cpu").manual_seed(SEED)
+    ht = m.to_internal(torch.randn(mesh.nCells, generator=g, dtype=torch.float64).numpy(), "c")
+    ut = m.to_internal(0.1 * torch.randn(mesh.nEdges, generator=g, dtype=torch.float64).numpy(), "e")
+    m.tlm(hb, ub, ht, ut, warmup)
+    torch.cuda.synchronize()
+    e0.record()
+    m.tlm(hb, ub, ht, ut, steps)
+    e1.record()
+    torch.cuda.synchronize()
+    t_tlm = e0.elapsed_time(e1) / 1e3
+    ok = int(m.nonfinite(h, u).sum().item()) == 0 and int(m.nonfinite(ht, ut).sum().item()) == 0
+    return {"workload": "uneven batch 64xL5 + 16xL6 + 4xL7 + 1xL9, TC2/TC6, dt 18.75 s, FP32 members",
+            "cells": mesh.nCells, "edges": mesh.nEdges, "vertices": mesh.nVertices, "steps": steps,
+            "fwd_cell_steps_per_s": mesh.nCells * steps / t_fwd, "fwd_ms_per_step": 1e3 * t_fwd / steps,
+            "tlm_cell_steps_per_s": mesh.nCells * steps / t_tlm, "tlm_ms_per_step": 1e3 * t_tlm / steps,
+            "finite": ok}
+
+
+def gradient_workload(steps, snapshots):
+    """Level 8 (655362 cells), TC5, dt 37.5 s: J = 1/2 sum A (h(T) - h_obs)^2,
+    h_obs = the unperturbed forward h(T) + N(0, 10 m) (seed 20261203 + 1000);
+    the gradient wrt (h0, u0) through the binomially checkpointed autograd
+    window, with FP64 and with FP32 adjoint members.  Returns the wall time
+    of each gradient and their relative difference."""
+    import torch
+    from mpas_sw_amd.checkpoint import forward_cost, rk4_window
+    from mpas_sw_amd.core import SWModel
+    from mpas_sw_amd.mesh import get_mesh
+    from mpas_sw_amd.testcases import initial_conditions
+    t = time.time()
+    mesh = get_mesh(8)
+    phys = initial_conditions(mesh, 5)
+    log(f"level-8 set-up: {time.time() - t:.1f}s")
+    res = {"workload": f"level-8 TC5 dt 37.5 s, {steps} steps, {snapshots} binomial snapshots, "
+                       "J = 1/2 sum A (h(T) - h_obs)^2",
+           "cells": mesh.nCells, "steps": steps, "snapshots": snapshots,
+           "recomputed_forward_steps": forward_cost(steps, snapshots)}
+    grads, hobs = {}, None
+    for prec in ("fp64", "fp32"):
+        m = SWModel(mesh, phys, 37.5, test_case=5, precision=prec, graphs=True)
+        h0 = m.to_internal(phys["h"], "c", member=False)
+        u0 = m.to_internal(phys["u"], "e", member=False)
+        A = m.to_internal(phys["areaCell"], "c", member=False)
+        if hobs is None:
+            hT, uT = h0.clone(), u0.clone()
+            m.base(hT, uT, steps)
+            g = torch.Generator("cpu").manual_seed(SEED + 1000)
+            noise = torch.randn(mesh.nCells, generator=g, dtype=torch.float64).numpy() * 10.0
+            hobs = hT + m.to_internal(noise, "c", member=False)
+            log("level-8 observation forward done")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hg, ug = h0.clone().requires_grad_(True), u0.clone().requires_grad_(True)
+        hTg, _ = rk4_window(m, hg, ug, steps, snapshots)
+        (0.5 * (A * (hTg - hobs) ** 2).sum()).backward()
+        torch.cuda.synchronize()
+        res[f"{prec}_gradient_s"] = time.perf_counter() - t0
+        log(f"level-8 gradient {prec}: {res[f'{prec}_gradient_s']:.1f}s")
+        grads[prec] = torch.cat([hg.grad, ug.grad])
+        del m
+    d = grads["fp32"] - grads["fp64"]
+    res["fp32_vs_fp64_gradient_rel_l2"] = float(torch.linalg.norm(d) / torch.linalg.norm(grads["fp64"]))
+    res["finite"] = bool(torch.isfinite(grads["fp64"]).all() and torch.isfinite(grads["fp32"]).all())
+    return res
+
+
 def cpu_baseline(mesh, phys, dh, du, steps=2):
     """Reference Fortran (oracle/_ref/msw_ref, one core) TLM and adjoint
     steps on the same mesh and case; member cell-steps/s of TLM+ADJ."""
@@ -213,6 +310,10 @@ def main():
                     help="keep the default launch stripe (no autotuning runs before the timed window)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 1 GiB device copy (reference for HBM counters)")
+    ap.add_argument("--packed", action="store_true",
+                    help="time the uneven-batch workload (64+16+4+1 instances, FP32) and exit")
+    ap.add_argument("--gradient", action="store_true",
+                    help="time the level-8 checkpointed gradient (FP64 and FP32 adjoint) and exit")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
                          "`--no-cpu --no-plain --no-fp32 --calibrate` into profiles/ and exit")
@@ -231,8 +332,14 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the batched members would shard "
-              "across GPUs with no per-step exchange (see DESIGN.md)", file=sys.stderr)
+              "across GPUs with no per-step exchange", file=sys.stderr)
         sys.exit(2)
+    if args.packed:
+        print(json.dumps(packed_workload(args.steps, args.warmup)), flush=True)
+        return
+    if args.gradient:
+        print(json.dumps(gradient_workload(args.steps, 32)), flush=True)
+        return
     import numpy as np
     import torch
     from mpas_sw_amd.mesh import get_mesh
