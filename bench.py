@@ -319,12 +319,12 @@ def main():
                          "`--no-cpu --no-plain --no-fp32 --calibrate` into profiles/ and exit")
     args = ap.parse_args()
     if args.traffic_from:
-        cmd = ("rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc1 -o run -- python3 bench.py "
+        cmd = ("rocprofv3 --pmc FETCH_SIZE --output-format csv -d prof/fetch -o run -- python3 bench.py "
                f"--steps {args.steps} --warmup {args.warmup} --no-cpu --no-plain --no-fp32 --no-tune --calibrate\n"
-               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc2 -o run -- python3 bench.py "
+               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d prof/write -o run -- python3 bench.py "
                f"--steps {args.steps} --warmup {args.warmup} --no-cpu --no-plain --no-fp32 --no-tune --calibrate\n"
                f"python3 bench.py --steps {args.steps} --warmup {args.warmup} --traffic-from "
-               "gpurun_out/pmc1 gpurun_out/pmc2")
+               "prof/fetch prof/write")
         r = traffic_from_counters(args.traffic_from[0], args.traffic_from[1], args.steps + args.warmup,
                                   os.path.join(ROOT, "profiles", "traffic_tlm_adj_l7_b64_fp64.json"),
                                   os.path.join(ROOT, "profiles", "traffic_tlm_adj_l7_b64_fp64.md"), cmd)
