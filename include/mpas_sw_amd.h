@@ -16,9 +16,9 @@
  *    members stored innermost: element (entity i, member b) at [i*B + b].
  *  - `prec` selects the member arithmetic: MSW_FP64 (double*) or MSW_FP32
  *    (float*).  Nonlinear base states of the TLM/adjoint are always double.
- *  - FP64 member arrays of a context with B >= 64 passed to msw_tlm_steps and
- *    msw_adj_step(s) must be 16-byte aligned (MSW_EINVAL otherwise); device
- *    allocations and torch tensors are.
+ *  - FP64 member arrays of a context with B >= 64 passed to msw_fwd_steps,
+ *    msw_tlm_steps and msw_adj_step(s) must be 16-byte aligned (MSW_EINVAL
+ *    otherwise); device allocations and torch tensors are.
  *  - Every call is ordered on the stream given at creation (e.g. the caller's
  *    torch stream) and returns 0 or a negative MSW_E* code; the library never
  *    aborts.  msw_last_error() describes the last failure of this thread.

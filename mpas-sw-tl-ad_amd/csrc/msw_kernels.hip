@@ -230,13 +230,14 @@ __global__ __launch_bounds__(kBlock) void k_fwd_vertex(DMesh m, int lb, const T*
     ENTITY_ITEM(m.nV)
     {
         const int iv = entity_of<WU>(t, lb);
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         int ie[3], ic[3];
         T ue[3], hc[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) { ie[i] = m.eovR[3 * iv + i]; ic[i] = m.cov[3 * iv + i]; }
-#pragma unroll
-        for (int i = 0; i < 3; ++i) { ue[i] = ld(u, ie[i], B, b); hc[i] = ld(h, ic[i], B, b); }
+        ld2<T, PF>(u, ie[0], u, ie[1], B, ln, ue[0], ue[1]);
+        ld2<T, PF>(u, ie[2], h, ic[0], B, ln, ue[2], hc[0]);
+        ld2<T, PF>(h, ic[1], h, ic[2], B, ln, hc[1], hc[2]);
         T circ = T(0);
 #pragma unroll
         for (int i = 0; i < 3; ++i) circ = circ + T(m.sdcV[3 * iv + i]) * ue[i];
@@ -262,13 +263,13 @@ __global__ __launch_bounds__(kBlock) void k_fwd_cell(DMesh m, int lb, const T* _
     ENTITY_ITEM(m.nC)
     {
         const int ic = entity_of<WU>(t, lb);
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         int ie[kC6], iv[kC6];
         T ue[kC6], pv[kC6];
 #pragma unroll
         for (int i = 0; i < kC6; ++i) { ie[i] = m.eoc[kC6 * ic + i]; iv[i] = m.vocR[kC6 * ic + i]; }
-#pragma unroll
-        for (int i = 0; i < kC6; ++i) { ue[i] = ld(u, ie[i], B, b); pv[i] = ld(pvv, iv[i], B, b); }
+        ld_rows<T, PF>(u, ie, B, ln, ue);
+        ld_rows<T, PF>(pvv, iv, B, ln, pv);
         const T area = T(m.areaC[ic]);
         T k = T(0);
 #pragma unroll
@@ -300,18 +301,18 @@ __global__ __launch_bounds__(kBlock) void k_fwd_edge(DMesh m, DConfig c, int lb,
     ENTITY_ITEM(m.nE)
     {
         const int e = entity_of<WU>(t, lb);
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
         const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
         int ix[kEoE];
         T ux[kEoE];
 #pragma unroll
         for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
-#pragma unroll
-        for (int j = 0; j < kEoE; ++j) ux[j] = ld(u, ix[j], B, b);
-        const T h1 = ld(h, c1, B, b), h2 = ld(h, c2, B, b);
-        const T p1 = ld(pvv, v1, B, b), p2 = ld(pvv, v2, B, b);
-        const T q1 = ld(pc, c1, B, b), q2 = ld(pc, c2, B, b);
+        ld_rows<T, PF>(u, ix, B, ln, ux);
+        T h1, h2, p1, p2, q1, q2;
+        ld2<T, PF>(h, c1, h, c2, B, ln, h1, h2);
+        ld2<T, PF>(pvv, v1, pvv, v2, B, ln, p1, p2);
+        ld2<T, PF>(pc, c1, pc, c2, B, ln, q1, q2);
         const T ue = ld(u, e, B, b);
         T vv = T(0);
 #pragma unroll
@@ -487,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
     const int B = 1 << lb;
     EDGE_CELL_ITEM(is_edge, t)
     {
-        const int b = static_cast<int>(t & (B - 1));
+        MEMBER_LANE(T, WU)
         if (is_edge) {
             const int e = entity_of<WU>(t, lb);
             const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
@@ -495,15 +496,13 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
             T ux[kEoE], px[kEoE], hx[kEoE];
 #pragma unroll
             for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
-#pragma unroll
-            for (int j = 0; j < kEoE; ++j) {
-                ux[j] = ld(u, ix[j], B, b);
-                px[j] = ld(pe, ix[j], B, b);
-                hx[j] = ld(he, ix[j], B, b);
-            }
+            ld_rows<T, PF>(u, ix, B, ln, ux);
+            ld_rows<T, PF>(pe, ix, B, ln, px);
+            ld_rows<T, PF>(he, ix, B, ln, hx);
             const T pee = ld(pe, e, B, b);
-            const T k1 = ld(ke, c1, B, b), k2 = ld(ke, c2, B, b);
-            const T h1 = ld(h, c1, B, b), h2 = ld(h, c2, B, b);
+            T k1, k2, h1, h2;
+            ld2<T, PF>(ke, c1, ke, c2, B, ln, k1, k2);
+            ld2<T, PF>(h, c1, h, c2, B, ln, h1, h2);
             T q = T(0);
 #pragma unroll
             for (int j = 0; j < kEoE; ++j) {
@@ -523,8 +522,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
             T ue[kC6], hh[kC6];
 #pragma unroll
             for (int k = 0; k < kC6; ++k) ie[k] = m.eocR[kC6 * ic + k];
-#pragma unroll
-            for (int k = 0; k < kC6; ++k) { ue[k] = ld(u, ie[k], B, b); hh[k] = ld(he, ie[k], B, b); }
+            ld_rows<T, PF>(u, ie, B, ln, ue);
+            ld_rows<T, PF>(he, ie, B, ln, hh);
             T th = T(0);
 #pragma unroll
             for (int k = 0; k < kC6; ++k) {

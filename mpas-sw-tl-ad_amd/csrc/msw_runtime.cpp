@@ -861,9 +861,17 @@ int msw_set_graphs(msw_ctx* ctx, int enable)
     return MSW_OK;
 }
 
+// FP64 member arrays of a 64-member-or-larger context move as 16-byte pairs
+static bool members_aligned(const msw_ctx* ctx, const void* a, const void* b)
+{
+    if (ctx->prec != MSW_FP64 || ctx->B < 64) return true;
+    return (reinterpret_cast<uintptr_t>(a) % 16 == 0) && (reinterpret_cast<uintptr_t>(b) % 16 == 0);
+}
+
 int msw_fwd_steps(msw_ctx* ctx, void* h, void* u, int nsteps)
 {
     if (!ctx || !h || !u || nsteps < 0) return fail(MSW_EINVAL, "bad argument");
+    if (!members_aligned(ctx, h, u)) return fail(MSW_EINVAL, "h/u must be 16-byte aligned");
     int rc = check_cfg(ctx);
     if (rc) return rc;
     if (ctx->prec == MSW_FP64)
@@ -885,13 +893,6 @@ int msw_base_steps(msw_ctx* ctx, double* h, double* u, int nsteps, double* traj_
     };
     return run_steps(ctx, 3, h, u, nullptr, nullptr, nsteps, [&] { enqueue_base_step(ctx, h, u); return MSW_OK; },
                      traj_h ? &pre : nullptr);
-}
-
-// FP64 member arrays of a 64-member-or-larger context move as 16-byte pairs
-static bool members_aligned(const msw_ctx* ctx, const void* a, const void* b)
-{
-    if (ctx->prec != MSW_FP64 || ctx->B < 64) return true;
-    return (reinterpret_cast<uintptr_t>(a) % 16 == 0) && (reinterpret_cast<uintptr_t>(b) % 16 == 0);
 }
 
 int msw_tlm_steps(msw_ctx* ctx, double* h, double* u, void* h_tl, void* u_tl, int nsteps,

@@ -587,3 +587,30 @@ def test_wave_per_entity_reference_order_tlm_bitwise(tc, dt, B):
         o.rk4_tlm(n)
         np.testing.assert_array_equal(HT[:, b], o["h_tl1"], err_msg=f"member {b}")
         np.testing.assert_array_equal(UT[:, b], o["u_tl1"], err_msg=f"member {b}")
+
+
+@pytest.mark.parametrize("tc,dt", [(5, 900.0), (6, 600.0)])
+def test_wave_per_entity_forward_bitwise(tc, dt):
+    """64 perturbed FP64 forward members (the pair-form forward kernels):
+    sampled members equal the oracle's forward of the same state, bit for bit."""
+    print(f"B=64 forward L{LEVEL} tc{tc} bitwise", flush=True)
+    from oracle.oracle import Oracle
+    from helpers import oracle_phys
+    B, n = 64, 4
+    mesh, phys = case(LEVEL, tc)
+    dh, du = perturbations(mesh, B, seed=61)
+    h0 = phys["h"][:, None] + dh
+    u0 = phys["u"][:, None] + du
+    m = model(LEVEL, tc, dt, batch=B)
+    h, u = m.to_internal(h0, "c"), m.to_internal(u0, "e")
+    m.fwd(h, u, n)
+    torch.cuda.synchronize()
+    H, U = m.to_reference(h, "c"), m.to_reference(u, "e")
+    for b in (0, 1, 32, 33, 63):
+        o = Oracle(mesh, oracle_phys(phys), dt, test_case=tc)
+        o["h1"][:] = h0[:, b]
+        o["u1"][:] = u0[:, b]
+        o.init_block()
+        o.rk4(n)
+        np.testing.assert_array_equal(H[:, b], o["h1"], err_msg=f"member {b}")
+        np.testing.assert_array_equal(U[:, b], o["u1"], err_msg=f"member {b}")
