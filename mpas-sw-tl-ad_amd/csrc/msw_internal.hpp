@@ -144,7 +144,7 @@ template <typename T>
 void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
                         const T* h, const T* u, const T* he, const T* pe, const T* ke,
                         const Dissip<T>& dp, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
-                        hipStream_t s);
+                        const double* rec, hipStream_t s);
 
 // Scratch of the del2/del4 terms for one field set (members or base):
 // div [C], dsu [E], dsd [C], dsv [V]; vort [V] comes from the vertex pass.

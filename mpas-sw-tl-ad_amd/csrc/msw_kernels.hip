@@ -483,7 +483,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
                                                      const T* __restrict__ he,
                                                      const T* __restrict__ pe,
                                                      const T* __restrict__ ke, Dissip<T> dp,
-                                                     T* xh, T* xu, T* acch, T* accu, T* ph, T* pu)
+                                                     T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
+                                                     const double* __restrict__ rec)
 {
     const int B = 1 << lb;
     EDGE_CELL_ITEM(is_edge, t)
@@ -496,9 +497,21 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
             T ux[kEoE], px[kEoE], hx[kEoE];
 #pragma unroll
             for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
-            ld_rows<T, PF>(u, ix, B, ln, ux);
-            ld_rows<T, PF>(pe, ix, B, ln, px);
-            ld_rows<T, PF>(he, ix, B, ln, hx);
+            if (!WU && rec) {
+                // one member (the FP64 base): pv_edge, u, h_edge of a neighbour
+                // come in one packed record
+#pragma unroll
+                for (int j = 0; j < kEoE; ++j) {
+                    const double4 r4 = *reinterpret_cast<const double4*>(rec + 4L * ix[j]);
+                    px[j] = T(r4.x);
+                    ux[j] = T(r4.y);
+                    hx[j] = T(r4.z);
+                }
+            } else {
+                ld_rows<T, PF>(u, ix, B, ln, ux);
+                ld_rows<T, PF>(pe, ix, B, ln, px);
+                ld_rows<T, PF>(he, ix, B, ln, hx);
+            }
             const T pee = ld(pe, e, B, b);
             T k1, k2, h1, h2;
             ld2<T, PF>(ke, c1, ke, c2, B, ln, k1, k2);
@@ -1439,10 +1452,10 @@ template <typename T>
 void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
                         const T* h, const T* u, const T* he, const T* pe, const T* ke,
                         const Dissip<T>& dp, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
-                        hipStream_t s)
+                        const double* rec, hipStream_t s)
 {
     LAUNCH_WU(k_fwd_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2), s, m, c, lb, stage, T(k.w[stage]),
-              T(k.sw[stage]), h, u, he, pe, ke, dp, xh, xu, acch, accu, ph, pu);
+              T(k.sw[stage]), h, u, he, pe, ke, dp, xh, xu, acch, accu, ph, pu, lb == 0 ? rec : nullptr);
 }
 
 template <typename T>
@@ -1634,7 +1647,8 @@ void nonfinite_flags(long n, int lb, const T* x, int* flags, hipStream_t s)
                               T*, T*, T*, T*, T*, T*, double*, hipStream_t);                        \
     template void fwd_tend_update_he<T>(const DMesh&, const DConfig&, int, int, const RKCoeffs&,    \
                                         const T*, const T*, const T*, const T*, const T*,           \
-                                        const Dissip<T>&, T*, T*, T*, T*, T*, T*, hipStream_t);     \
+                                        const Dissip<T>&, T*, T*, T*, T*, T*, T*, const double*,    \
+                                        hipStream_t);                                               \
     template Dissip<T> dissip_fields<T>(const DMesh&, const DConfig&, int, const T*, const T*,       \
                                         const DissipScratch<T>&, hipStream_t);                      \
     template void tl_stage<T>(const DMesh&, const DConfig&, int, int, const RKCoeffs&,              \

@@ -488,7 +488,7 @@ void base_tend(msw_ctx* c, const BaseStage& b, int s, const RKCoeffs& k, double*
     const Dissip<double> dp = dissip_fields<double>(c->dm, c->dcfg, 0, b.u, b.vort, base_ds(c), st);
     const size_t C = c->nC;
     fwd_tend_update_he<double>(c->dm, c->dcfg, 0, s, k, b.h, b.u, b.he, b.pe, b.ke, dp, xh, xu,
-                               c->bacc, c->bacc + C, nh, nh + C, st);
+                               c->bacc, c->bacc + C, nh, nh + C, b.rec, st);
 }
 
 // The side stream joins the main stream's work (and a capture in progress).
@@ -532,7 +532,7 @@ void enqueue_fwd_step(msw_ctx* c, T* h, T* u)
         const Dissip<T> dp = dissip_fields<T>(m, c->dcfg, c->lb, ru, vort, member_ds<T>(c), c->stream);
         T* nb = (s % 2 == 0) ? pA : pB;
         fwd_tend_update_he<T>(m, c->dcfg, c->lb, s, k, rh, ru, he, pe, ke, dp, h, u, acc,
-                              acc + C * B, nb, nb + C * B, c->stream);
+                              acc + C * B, nb, nb + C * B, nullptr, c->stream);
         rh = nb;
         ru = nb + C * B;
     }
