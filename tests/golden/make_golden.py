@@ -82,11 +82,36 @@ def recon_adjoint_case():
          seeds=seeds, h_ad=a["h_ad"], u_ad=a["u_ad"])
 
 
+def sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def level6_day_digest():
+    """Level 6, TC5, dt 150 s, 576 steps (one day): SHA-256 digests and norms
+    of the reference's forward state and of its TLM (perturbation seeded
+    SEED + 4) -- a bit-for-bit pin of a one-day window without storing the
+    arrays (a test regenerates the inputs and compares digests)."""
+    m6 = get_mesh(6)
+    tc, dt, n = 5, 150.0, 576
+    dh, du = perturb(m6, SEED + 4)
+    ic = ref.run(m6, "ic", test_case=tc, dt=dt)
+    f = ref.run(m6, "fwd", nsteps=n, test_case=tc, dt=dt)
+    t = ref.run(m6, "tlm", nsteps=n, test_case=tc, dt=dt, tl=(dh, du))
+    cfg = dict(test_case=tc, dt=dt, nsteps=n, perturbation_seed=SEED + 4,
+               sha_h0=sha(ic["h"]), sha_u0=sha(ic["u"]), sha_dh=sha(dh), sha_du=sha(du),
+               sha_h=sha(f["h"]), sha_u=sha(f["u"]), sha_h_tl=sha(t["h_tl"]), sha_u_tl=sha(t["u_tl"]),
+               sha_h_base=sha(t["h"]), sha_u_base=sha(t["u"]))
+    norms = np.array([np.linalg.norm(x) for x in (f["h"], f["u"], t["h_tl"], t["u_tl"])])
+    save("tc5_l6_1day_digest", m6, cfg, norms=norms)
+
+
 def main(names):
     assert ref.available(), "build the reference first: make -C oracle ref"
     if names:
         for n in names:
-            {"tc5_l3_dissipation": dissipation_case, "tc5_l3_recon_adjoint": recon_adjoint_case}[n]()
+            {"tc5_l3_dissipation": dissipation_case, "tc5_l3_recon_adjoint": recon_adjoint_case,
+             "tc5_l6_1day_digest": level6_day_digest}[n]()
         return
 
     # 1. Williamson TC2, level 5, dt = 300 s, 288 steps (one day)
@@ -142,6 +167,9 @@ def main(names):
 
     # 6. reconstructed-velocity adjoint seeds
     recon_adjoint_case()
+
+    # 7. one-day level-6 window, as digests
+    level6_day_digest()
 
 
 if __name__ == "__main__":

@@ -284,3 +284,46 @@ def test_packed_uneven_instances_fp32_forward_and_tlm():
         sl = [slice(oc, oc + mi.nCells), slice(oe, oe + mi.nEdges)] * 2
         for g, w, sli in zip(got, want, sl):
             np.testing.assert_array_equal(g[sli], w, err_msg=f"instance {i}")
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("B", [1, 64])
+def test_level6_one_day_digests_equal_reference_fortran(B):
+    """TC5 level 6, dt 150 s, 576 steps in the reference term order: the
+    forward state and the TLM (perturbation seeded as the reference run's)
+    after one day hash to the reference Fortran's SHA-256, for one member
+    (lane per item) and as member 0 of 64 (pair form)."""
+    print(f"L6 tc5 one day vs reference digests, B={B}", flush=True)
+    g, meta = golden("tc5_l6_1day_digest")
+    n, dt = meta["nsteps"], meta["dt"]
+    mesh, phys = case(6, 5)
+    rng = np.random.default_rng(meta["perturbation_seed"])
+    dh, du = rng.standard_normal(mesh.nCells), 0.1 * rng.standard_normal(mesh.nEdges)
+    ex_h, ex_u = perturbations(mesh, B)
+    ex_h[:, 0], ex_u[:, 0] = dh, du
+    m = _model(6, 5, dt, graphs=True, batch=B)
+    h = m.to_internal(phys["h"][:, None] + 0 * ex_h, "c")
+    u = m.to_internal(phys["u"][:, None] + 0 * ex_u, "e")
+    if B == 1:
+        h, u = h.reshape(-1).contiguous(), u.reshape(-1).contiguous()
+    m.fwd(h, u, n)
+    torch.cuda.synchronize()
+    H, U = m.to_reference(h, "c"), m.to_reference(u, "e")
+    H0, U0 = (H, U) if B == 1 else (H[:, 0], U[:, 0])
+    assert _sha(H0) == meta["sha_h"] and _sha(U0) == meta["sha_u"]
+    hb = m.to_internal(phys["h"], "c", member=False)
+    ub = m.to_internal(phys["u"], "e", member=False)
+    ht = m.to_internal(ex_h if B > 1 else dh, "c")
+    ut = m.to_internal(ex_u if B > 1 else du, "e")
+    m.tlm(hb, ub, ht, ut, n)
+    torch.cuda.synchronize()
+    HT, UT = m.to_reference(ht, "c"), m.to_reference(ut, "e")
+    if B > 1:
+        HT, UT = HT[:, 0], UT[:, 0]
+    assert _sha(m.to_reference(hb, "c")) == meta["sha_h_base"]
+    assert _sha(m.to_reference(ub, "e")) == meta["sha_u_base"]
+    assert _sha(HT) == meta["sha_h_tl"] and _sha(UT) == meta["sha_u_tl"]

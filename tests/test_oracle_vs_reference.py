@@ -151,3 +151,24 @@ def test_reconstruction_adjoint_seeds_bitwise():
     np.testing.assert_array_equal(o["h_ad1"], g["h_ad"])
     np.testing.assert_array_equal(o["u_ad1"], g["u_ad"])
     assert not np.any(o["uReconstructZonal_ad"])
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def test_level6_one_day_forward_digest():
+    """TC5 level 6, dt 150 s, 576 steps: the oracle's state after one day has
+    the reference's SHA-256 (the initial state and the seeded TLM
+    perturbation regenerated here match the reference's inputs too)."""
+    g, meta = golden("tc5_l6_1day_digest")
+    mesh, phys = case(6, 5)
+    assert _sha(phys["h"]) == meta["sha_h0"] and _sha(phys["u"]) == meta["sha_u0"]
+    rng = np.random.default_rng(meta["perturbation_seed"])
+    dh, du = rng.standard_normal(mesh.nCells), 0.1 * rng.standard_normal(mesh.nEdges)
+    assert _sha(dh) == meta["sha_dh"] and _sha(du) == meta["sha_du"]
+    o = _oracle_from(6, 5, meta["dt"], phys["h"], phys["u"])
+    o.rk4(meta["nsteps"])
+    assert _sha(o["h1"]) == meta["sha_h"]
+    assert _sha(o["u1"]) == meta["sha_u"]
