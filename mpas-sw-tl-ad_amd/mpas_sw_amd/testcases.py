@@ -4,7 +4,10 @@ Restates ``sw_test_case_1/2/5/6`` of ``src/mpas_sw_testcases.f90`` in the
 reference's expression order.  Transcendentals go through Python's ``math``
 module (the C library's libm, the same the reference Fortran links) element by
 element, so the initial conditions are the reference's to the bit rather than
-depending on NumPy's vectorised kernels.
+depending on NumPy's vectorised kernels.  Squares (``x**2.0`` and ``x**2``
+in the Fortran) are written as products, since the reference's compiler emits
+``x*x`` for them and libm's ``pow(x, 2.0)`` can differ from that product in the
+last bit; every other real exponent stays a ``pow`` call, as it does there.
 
 Constants are the reference's (``src/mpas_sw_module.f90:8-11``); with the
 reference's ``-fdefault-real-8`` build every literal is a double.
@@ -68,7 +71,7 @@ def scale_mesh(mesh):
 
     Returns a dict; the mesh object itself is not modified.
     """
-    a2 = A ** 2.0
+    a2 = A * A
     return {
         "xCell": mesh.xCell * A, "yCell": mesh.yCell * A, "zCell": mesh.zCell * A,
         "xVertex": mesh.xVertex * A, "yVertex": mesh.yVertex * A, "zVertex": mesh.zVertex * A,
@@ -99,13 +102,14 @@ def _zonal_psi(mesh, u0, alpha):
 def _geostrophic_h(lat, lon, u0, gh0, alpha):
     slat, clat = _sincos(lat)
     s = (-_cos(lon) * clat * math.sin(alpha) + slat * math.cos(alpha))
-    return (gh0 - (A * OMEGA * u0 + 0.5 * u0 ** 2.0) * _pow(s, 2.0)) / GRAVITY
+    return (gh0 - (A * OMEGA * u0 + 0.5 * (u0 * u0)) * (s * s)) / GRAVITY
 
 
 def sphere_distance(lat1, lon1, lat2, lon2, radius):
     """``src/mpas_sw_testcases.f90:321-339``."""
-    arg1 = math.sqrt(math.sin(0.5 * (lat2 - lat1)) ** 2 +
-                     math.cos(lat1) * math.cos(lat2) * math.sin(0.5 * (lon2 - lon1)) ** 2)
+    s1 = math.sin(0.5 * (lat2 - lat1))
+    s2 = math.sin(0.5 * (lon2 - lon1))
+    arg1 = math.sqrt(s1 * s1 + math.cos(lat1) * math.cos(lat2) * (s2 * s2))
     return 2.0 * radius * math.asin(arg1)
 
 
@@ -114,10 +118,10 @@ def aa(theta):
     w = K = 7.848e-6
     R = 4.0
     c = math.cos(theta)
-    return (0.5 * w * (2.0 * OMEGA + w) * math.pow(c, 2.0) +
-            0.25 * math.pow(K, 2.0) * math.pow(c, 2.0 * R) *
-            ((R + 1.0) * math.pow(c, 2.0) + 2.0 * math.pow(R, 2.0) - R - 2.0 -
-             2.0 * math.pow(R, 2.0) * math.pow(c, -2.0)))
+    return (0.5 * w * (2.0 * OMEGA + w) * (c * c) +
+            0.25 * (K * K) * math.pow(c, 2.0 * R) *
+            ((R + 1.0) * (c * c) + 2.0 * (R * R) - R - 2.0 -
+             2.0 * (R * R) * math.pow(c, -2.0)))
 
 
 def bb(theta):
@@ -126,7 +130,7 @@ def bb(theta):
     R = 4.0
     c = math.cos(theta)
     return ((2.0 * (OMEGA + w) * K / ((R + 1.0) * (R + 2.0))) * math.pow(c, R) *
-            ((math.pow(R, 2.0) + 2.0 * R + 2.0) - math.pow((R + 1.0) * c, 2.0)))
+            (((R * R) + 2.0 * R + 2.0) - ((R + 1.0) * c) * ((R + 1.0) * c)))
 
 
 def cc(theta):
@@ -134,7 +138,7 @@ def cc(theta):
     K = 7.848e-6
     R = 4.0
     c = math.cos(theta)
-    return 0.25 * math.pow(K, 2.0) * math.pow(c, 2.0 * R) * ((R + 1.0) * math.pow(c, 2.0) - R - 2.0)
+    return 0.25 * (K * K) * math.pow(c, 2.0 * R) * ((R + 1.0) * (c * c) - R - 2.0)
 
 
 def initial_conditions(mesh, case):
@@ -178,9 +182,11 @@ def initial_conditions(mesh, case):
         fE = _coriolis(mesh.latEdge, mesh.lonEdge, alpha)
         fV = _coriolis(mesh.latVertex, mesh.lonVertex, alpha)
         lonCell = np.where(lonCell < 0.0, lonCell + 2.0 * PII, lonCell)
-        rr2 = rr ** 2.0
+        rr2 = rr * rr
         for i in range(C):
-            d2 = math.pow(lonCell[i] - lambda_c, 2.0) + math.pow(mesh.latCell[i] - theta_c, 2.0)
+            dl = float(lonCell[i]) - lambda_c
+            dt = float(mesh.latCell[i]) - theta_c
+            d2 = dl * dl + dt * dt
             r = math.sqrt(min(rr2, d2))
             h_s[i] = hs0 * (1.0 - r / rr)
             tracer[i] = 1.0 - r / rr

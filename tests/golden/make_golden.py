@@ -106,12 +106,34 @@ def level6_day_digest():
     save("tc5_l6_1day_digest", m6, cfg, norms=norms)
 
 
+def level7_window_digest():
+    """The benchmark's configuration: level 7, TC6, dt 75 s, 48 steps; the
+    perturbation is member 0 of the benchmark's ensemble (torch generator
+    seeded SEED: dh ~ N(0, 1 m) then du ~ N(0, 0.1 m/s)).  SHA-256 digests
+    and norms of the reference's forward state and TLM after the window."""
+    import torch
+    m7 = get_mesh(7)
+    tc, dt, n = 6, 75.0, 48
+    gen = torch.Generator("cpu").manual_seed(SEED)
+    dh = torch.randn(m7.nCells, generator=gen, dtype=torch.float64).numpy()
+    du = torch.randn(m7.nEdges, generator=gen, dtype=torch.float64).numpy() * 0.1
+    ic = ref.run(m7, "ic", test_case=tc, dt=dt)
+    f = ref.run(m7, "fwd", nsteps=n, test_case=tc, dt=dt)
+    t = ref.run(m7, "tlm", nsteps=n, test_case=tc, dt=dt, tl=(dh, du))
+    cfg = dict(test_case=tc, dt=dt, nsteps=n, perturbation_seed=SEED,
+               sha_h0=sha(ic["h"]), sha_u0=sha(ic["u"]), sha_dh=sha(dh), sha_du=sha(du),
+               sha_h=sha(f["h"]), sha_u=sha(f["u"]), sha_h_tl=sha(t["h_tl"]), sha_u_tl=sha(t["u_tl"]),
+               sha_h_base=sha(t["h"]), sha_u_base=sha(t["u"]))
+    norms = np.array([np.linalg.norm(x) for x in (f["h"], f["u"], t["h_tl"], t["u_tl"])])
+    save("tc6_l7_window_digest", m7, cfg, norms=norms)
+
+
 def main(names):
     assert ref.available(), "build the reference first: make -C oracle ref"
     if names:
         for n in names:
             {"tc5_l3_dissipation": dissipation_case, "tc5_l3_recon_adjoint": recon_adjoint_case,
-             "tc5_l6_1day_digest": level6_day_digest}[n]()
+             "tc5_l6_1day_digest": level6_day_digest, "tc6_l7_window_digest": level7_window_digest}[n]()
         return
 
     # 1. Williamson TC2, level 5, dt = 300 s, 288 steps (one day)
@@ -170,6 +192,9 @@ def main(names):
 
     # 7. one-day level-6 window, as digests
     level6_day_digest()
+
+    # 8. the benchmark's level-7 window, as digests
+    level7_window_digest()
 
 
 if __name__ == "__main__":

@@ -327,3 +327,46 @@ def test_level6_one_day_digests_equal_reference_fortran(B):
     assert _sha(m.to_reference(hb, "c")) == meta["sha_h_base"]
     assert _sha(m.to_reference(ub, "e")) == meta["sha_u_base"]
     assert _sha(HT) == meta["sha_h_tl"] and _sha(UT) == meta["sha_u_tl"]
+
+
+def test_benchmark_window_against_reference_fortran():
+    """The benchmark's own configuration (level 7, TC6, dt 75 s, 64 members,
+    48 steps): in the reference term order member 0 (the benchmark's member
+    0) hashes to the reference Fortran's forward base state and TLM after
+    the window; the fused-flux TLM of all 64 members agrees with that run
+    to 1e-12; and the dot-product test <L x, y> = <x, L^T y> of the fused
+    TLM sweep against the adjoint sweep holds to 1e-11 for every member."""
+    print("L7 tc6 benchmark window vs reference digests", flush=True)
+    g, meta = golden("tc6_l7_window_digest")
+    n, dt, B = meta["nsteps"], meta["dt"], 64
+    mesh, phys = case(7, 6)
+    dh, du = perturbations(mesh, B)
+    assert _sha(dh[:, 0]) == meta["sha_dh"] and _sha(du[:, 0]) == meta["sha_du"]
+    out = {}
+    for fused in (False, True):
+        m = _model(7, 6, dt, graphs=True, batch=B, tl_fused_flux=fused)
+        hb = m.to_internal(phys["h"], "c", member=False)
+        ub = m.to_internal(phys["u"], "e", member=False)
+        ht, ut = m.to_internal(dh, "c"), m.to_internal(du, "e")
+        th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+        tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+        m.tlm(hb, ub, ht, ut, n, th, tu)
+        torch.cuda.synchronize()
+        out[fused] = (m.to_reference(ht, "c"), m.to_reference(ut, "e"))
+        if not fused:
+            assert _sha(m.to_reference(hb, "c")) == meta["sha_h_base"]
+            assert _sha(m.to_reference(ub, "e")) == meta["sha_u_base"]
+            assert _sha(out[False][0][:, 0]) == meta["sha_h_tl"]
+            assert _sha(out[False][1][:, 0]) == meta["sha_u_tl"]
+        else:
+            yh0, yu0 = perturbations(mesh, B, seed=4242)
+            yh, yu = m.to_internal(yh0, "c"), m.to_internal(yu0, "e")
+            lhs = m.dot(ht, ut, yh, yu).cpu().numpy()
+            ah, au = yh.clone(), yu.clone()
+            m.adj(th, tu, ah, au, n)
+            xh, xu = m.to_internal(dh, "c"), m.to_internal(du, "e")
+            rhs = m.dot(xh, xu, ah, au).cpu().numpy()
+            assert np.all(np.abs(lhs - rhs) <= 1e-11 * np.abs(lhs))
+        del m
+    for a, b in zip(out[True], out[False]):
+        assert rel_l2(a, b) < 1e-12
