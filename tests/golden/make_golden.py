@@ -128,12 +128,30 @@ def level7_window_digest():
     save("tc6_l7_window_digest", m7, cfg, norms=norms)
 
 
+def level7_ic_digest():
+    """Initial conditions of all four test cases on the level-7 mesh, as
+    SHA-256 digests of the reference's h, u, h_s, fEdge, fVertex and lonCell
+    (in the mesh's own order): a bit-for-bit pin of the host set-up on a mesh
+    large enough to meet the last-bit cases of libm that level 3 misses."""
+    m7 = get_mesh(7)
+    cfg = {}
+    for tc in (1, 2, 5, 6):
+        ic = ref.run(m7, "ic", test_case=tc, dt=75.0)
+        for k in IC_FIELDS:
+            cfg[f"sha_tc{tc}_{k}"] = sha(ic[k])
+    save("l7_ic_digest", m7, cfg)
+
+
+IC_FIELDS = ("h", "u", "h_s", "fEdge", "fVertex", "lonCell")
+
+
 def main(names):
     assert ref.available(), "build the reference first: make -C oracle ref"
     if names:
         for n in names:
             {"tc5_l3_dissipation": dissipation_case, "tc5_l3_recon_adjoint": recon_adjoint_case,
-             "tc5_l6_1day_digest": level6_day_digest, "tc6_l7_window_digest": level7_window_digest}[n]()
+             "tc5_l6_1day_digest": level6_day_digest, "tc6_l7_window_digest": level7_window_digest,
+             "l7_ic_digest": level7_ic_digest}[n]()
         return
 
     # 1. Williamson TC2, level 5, dt = 300 s, 288 steps (one day)
@@ -195,6 +213,9 @@ def main(names):
 
     # 8. the benchmark's level-7 window, as digests
     level7_window_digest()
+
+    # 9. level-7 initial conditions of the four test cases, as digests
+    level7_ic_digest()
 
 
 if __name__ == "__main__":

@@ -22,3 +22,23 @@ def test_scaled_metrics_and_coefficients():
     np.testing.assert_array_equal(phys["kiteAreasOnVertex"].T.ravel(), g["kiteAreasOnVertex"])
     mine = np.transpose(phys["coeffs_reconstruct"], (2, 1, 0)).ravel()
     np.testing.assert_array_equal(mine, g["coeffs_reconstruct"])
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def test_level7_initial_conditions_digests():
+    """All four test cases on the level-7 mesh hash to the reference
+    Fortran's initial fields.  At this size the last-bit differences between
+    libm's pow(x, 2.0) and the product x*x the reference's compiler emits
+    for squares show up (10 TC6 cells), which level 3 does not reach."""
+    from mpas_sw_amd.mesh import get_mesh
+    from mpas_sw_amd.testcases import initial_conditions
+    _, meta = golden("l7_ic_digest")
+    mesh = get_mesh(7)
+    for tc in (1, 2, 5, 6):
+        phys = initial_conditions(mesh, tc)
+        for k in ("h", "u", "h_s", "fEdge", "fVertex", "lonCell"):
+            assert _sha(phys[k]) == meta[f"sha_tc{tc}_{k}"], (tc, k)
