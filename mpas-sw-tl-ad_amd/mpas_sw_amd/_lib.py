@@ -30,6 +30,13 @@ HIPCC_FLAGS = [
 MSW_FP64 = 0
 MSW_FP32 = 1
 
+# status codes (enum msw_status)
+MSW_OK = 0
+MSW_EINVAL = -1
+MSW_ENOMEM = -2
+MSW_EHIP = -3
+MSW_EUNSUPPORTED = -4
+
 
 def build(verbose: bool = False) -> str:
     """Compile the HIP library in-tree; returns its path."""

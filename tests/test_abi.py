@@ -45,3 +45,12 @@ def test_create_rejects_bad_arguments_without_gpu():
     d = _lib.MeshDesc()
     rc = lib.msw_create(ctypes.byref(d), ctypes.byref(cfg), 0, 3, None, ctypes.byref(out))
     assert rc == -1 and b"power of two" in lib.msw_last_error()
+
+
+def test_python_constants_match_the_header():
+    src = open(os.path.join(ROOT, "include", "mpas_sw_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    consts = {k: int(v) for k, v in re.findall(r"\b(MSW_[A-Z0-9]+)\s*=\s*(-?\d+)", src)}
+    for name in ("MSW_OK", "MSW_EINVAL", "MSW_ENOMEM", "MSW_EHIP", "MSW_EUNSUPPORTED",
+                 "MSW_FP64", "MSW_FP32"):
+        assert getattr(_lib, name) == consts[name], name

@@ -495,12 +495,14 @@ def test_launch_stripe_does_not_change_results_and_autotune():
     assert set(times) == {0, 2} and m.stripe_log2 in times
 
 
-def test_wave_per_entity_batch_matches_oracle():
+@pytest.mark.parametrize("B", [64, 256])
+def test_wave_per_entity_batch_matches_oracle(B):
     """B = 64 (a wavefront per entity: connectivity and coefficients are
-    scalar loads shared by the members): fused TLM within 1e-12 and adjoint
-    within 1e-11 of the oracle for sampled members, FP64."""
-    print("B=64 fused TLM + adjoint L3 tc5 vs oracle", flush=True)
-    tc, dt, B, n = 5, 900.0, 64, 4
+    scalar loads shared by the members) and B = 256, the largest batch (four
+    wavefronts per entity): fused TLM within 1e-12 and adjoint within 1e-11
+    of the oracle for sampled members, FP64."""
+    print(f"B={B} fused TLM + adjoint L3 tc5 vs oracle", flush=True)
+    tc, dt, n = 5, 900.0, 4
     mesh, phys = case(LEVEL, tc)
     dh, du = perturbations(mesh, B)
     m = model(LEVEL, tc, dt, batch=B, tl_fused_flux=True)
@@ -516,7 +518,7 @@ def test_wave_per_entity_batch_matches_oracle():
     m.adj(th, tu, ha, ua, n)
     torch.cuda.synchronize()
     HA, UA = m.to_reference(ha, "c"), m.to_reference(ua, "e")
-    for b in (0, 31, 63):
+    for b in sorted({0, 31, 63, B // 2 + 3, B - 1}):
         o = make_oracle(LEVEL, tc, dt)
         o["h_tl1"][:] = dh[:, b]
         o["u_tl1"][:] = du[:, b]
@@ -563,9 +565,10 @@ def test_wave_per_entity_dot_product_with_dissipation(prec, tol):
     assert np.all(np.abs(lhs - rhs) <= tol * np.abs(lhs))
 
 
-@pytest.mark.parametrize("tc,dt,B", [(2, 1200.0, 64), (5, 900.0, 64), (6, 600.0, 64), (5, 900.0, 128)])
+@pytest.mark.parametrize("tc,dt,B", [(2, 1200.0, 64), (5, 900.0, 64), (6, 600.0, 64), (5, 900.0, 128),
+                                     (6, 600.0, 256)])
 def test_wave_per_entity_reference_order_tlm_bitwise(tc, dt, B):
-    """B = 64 and 128 (two wavefronts per entity) in the reference term order:
+    """B = 64, 128 and 256 (one, two and four wavefronts per entity) in the reference term order:
     every sampled member's TLM is the oracle's to the bit (the lane
     assignment of the 64-member kernels changes which lane computes a member,
     never the order of a sum)."""
@@ -580,7 +583,7 @@ def test_wave_per_entity_reference_order_tlm_bitwise(tc, dt, B):
     m.tlm(h, u, ht, ut, n)
     torch.cuda.synchronize()
     HT, UT = m.to_reference(ht, "c"), m.to_reference(ut, "e")
-    for b in sorted({0, 1, 32, 33, 62, 63, B - 64, B - 33, B - 1}):
+    for b in sorted({0, 1, 32, 33, 62, 63, B // 2 + 1, B - 64, B - 33, B - 1}):
         o = make_oracle(LEVEL, tc, dt)
         o["h_tl1"][:] = dh[:, b]
         o["u_tl1"][:] = du[:, b]
@@ -589,14 +592,15 @@ def test_wave_per_entity_reference_order_tlm_bitwise(tc, dt, B):
         np.testing.assert_array_equal(UT[:, b], o["u_tl1"], err_msg=f"member {b}")
 
 
-@pytest.mark.parametrize("tc,dt", [(5, 900.0), (6, 600.0)])
-def test_wave_per_entity_forward_bitwise(tc, dt):
-    """64 perturbed FP64 forward members (the pair-form forward kernels):
-    sampled members equal the oracle's forward of the same state, bit for bit."""
-    print(f"B=64 forward L{LEVEL} tc{tc} bitwise", flush=True)
+@pytest.mark.parametrize("tc,dt,B", [(5, 900.0, 64), (6, 600.0, 64), (6, 600.0, 256)])
+def test_wave_per_entity_forward_bitwise(tc, dt, B):
+    """64 and 256 perturbed FP64 forward members (the pair-form forward
+    kernels): sampled members equal the oracle's forward of the same state,
+    bit for bit."""
+    print(f"B={B} forward L{LEVEL} tc{tc} bitwise", flush=True)
     from oracle.oracle import Oracle
     from helpers import oracle_phys
-    B, n = 64, 4
+    n = 4
     mesh, phys = case(LEVEL, tc)
     dh, du = perturbations(mesh, B, seed=61)
     h0 = phys["h"][:, None] + dh
@@ -606,7 +610,7 @@ def test_wave_per_entity_forward_bitwise(tc, dt):
     m.fwd(h, u, n)
     torch.cuda.synchronize()
     H, U = m.to_reference(h, "c"), m.to_reference(u, "e")
-    for b in (0, 1, 32, 33, 63):
+    for b in sorted({0, 1, 32, 33, 63, B // 2 + 5, B - 1}):
         o = Oracle(mesh, oracle_phys(phys), dt, test_case=tc)
         o["h1"][:] = h0[:, b]
         o["u1"][:] = u0[:, b]
@@ -614,3 +618,55 @@ def test_wave_per_entity_forward_bitwise(tc, dt):
         o.rk4(n)
         np.testing.assert_array_equal(H[:, b], o["h1"], err_msg=f"member {b}")
         np.testing.assert_array_equal(U[:, b], o["u1"], err_msg=f"member {b}")
+
+
+def test_step_arguments_zero_negative_and_misaligned():
+    """Edge cases of the step entry points at B = 64 (FP64): zero steps leave
+    every state untouched; a negative step count and a member array that is
+    contiguous but not 16-byte aligned are refused with MSW_EINVAL, before
+    any work, and the context keeps working afterwards."""
+    print("step argument edge cases L3 tc5 B=64", flush=True)
+    from mpas_sw_amd._lib import MSWError, MSW_EINVAL
+    tc, dt, B = 5, 900.0, 64
+    mesh, phys = case(LEVEL, tc)
+    dh, du = perturbations(mesh, B, seed=77)
+    m = model(LEVEL, tc, dt, batch=B)
+    h = m.to_internal(phys["h"][:, None] + dh, "c")
+    u = m.to_internal(phys["u"][:, None] + du, "e")
+    hb = m.to_internal(phys["h"], "c", member=False)
+    ub = m.to_internal(phys["u"], "e", member=False)
+    ht, ut = m.to_internal(dh, "c"), m.to_internal(du, "e")
+    th = torch.empty((1, m.nC), dtype=torch.float64, device=m.device)
+    tu = torch.empty((1, m.nE), dtype=torch.float64, device=m.device)
+    keep = [t.clone() for t in (h, u, hb, ub, ht, ut)]
+    m.fwd(h, u, 0)
+    m.tlm(hb, ub, ht, ut, 0)
+    m.adj(th, tu, ht, ut, 0)
+    torch.cuda.synchronize()
+    for a, b in zip((h, u, hb, ub, ht, ut), keep):
+        assert torch.equal(a, b)
+    for call in (lambda: m.fwd(h, u, -1), lambda: m.tlm(hb, ub, ht, ut, -1),
+                 lambda: m.adj(th, tu, ht, ut, -1)):
+        with pytest.raises(MSWError) as err:
+            call()
+        assert err.value.code == MSW_EINVAL
+    # contiguous [C, B] views that start 8 bytes into their storage
+    hx = torch.empty(m.nC * B + 1, dtype=torch.float64, device=m.device)[1:].view(m.nC, B)
+    ux = torch.empty(m.nE * B + 1, dtype=torch.float64, device=m.device)[1:].view(m.nE, B)
+    hx.copy_(h)
+    ux.copy_(u)
+    assert hx.data_ptr() % 16 == 8
+    for call in (lambda: m.fwd(hx, ux, 1), lambda: m.tlm(hb, ub, hx, ux, 1),
+                 lambda: m.adj(th, tu, hx, ux, 1)):
+        with pytest.raises(MSWError) as err:
+            call()
+        assert err.value.code == MSW_EINVAL and "aligned" in str(err.value)
+    torch.cuda.synchronize()
+    assert torch.equal(hx, h) and torch.equal(ux, u) and torch.equal(hb, keep[2])
+    # the context still steps, and as a fresh one does
+    m.fwd(h, u, 1)
+    m2 = model(LEVEL, tc, dt, batch=B)
+    h2, u2 = keep[0].clone(), keep[1].clone()
+    m2.fwd(h2, u2, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(h, h2) and torch.equal(u, u2)
