@@ -145,13 +145,43 @@ def level7_ic_digest():
 IC_FIELDS = ("h", "u", "h_s", "fEdge", "fVertex", "lonCell")
 
 
+def quirk_cases():
+    """Level 3, dt 900 s.  (1) TC1, the frozen-wind advection case: forward
+    h, u after 12 steps and TLM (base and h_tl, u_tl; u and u_tl stay at
+    their initial values, src/mpas_sw_module.f90:599-605,738-743).  (2)
+    Thickness advection orders 3 and 4, with and without config_monotonic:
+    SHA-256 digests of the TC5 forward, TLM and adjoint after 8 steps -- the
+    same as order 2's, because the reference never fills deriv_two
+    (src/mpas_sw_module.f90:285-286)."""
+    m3 = get_mesh(3)
+    dt = 900.0
+    n = 12
+    dh, du = perturb(m3, SEED + 7)
+    f = ref.run(m3, "fwd", nsteps=n, test_case=1, dt=dt)
+    t = ref.run(m3, "tlm", nsteps=n, test_case=1, dt=dt, tl=(dh, du))
+    save("tc1_l3_frozen_wind", m3, dict(test_case=1, dt=dt, nsteps=n, perturbation_seed=SEED + 7),
+         dh=dh, du=du, h=f["h"], u=f["u"], h_base=t["h"], u_base=t["u"], h_tl=t["h_tl"], u_tl=t["u_tl"])
+    n = 8
+    dh, du = perturb(m3, SEED + 8)
+    dig = {}
+    for order, mono in ((2, False), (3, False), (3, True), (4, False)):
+        kw = dict(nsteps=n, test_case=5, dt=dt, adv_order=order, monotonic=mono)
+        f = ref.run(m3, "fwd", **kw)
+        t = ref.run(m3, "tlm", tl=(dh, du), **kw)
+        a = ref.run(m3, "adj", ad=(dh, du), **kw)
+        dig[f"order{order}{'_monotonic' if mono else ''}"] = [
+            sha(x) for x in (f["h"], f["u"], t["h_tl"], t["u_tl"], a["h_ad"], a["u_ad"])]
+    save("tc5_l3_adv_order_digest", m3, dict(test_case=5, dt=dt, nsteps=n, perturbation_seed=SEED + 8,
+                                               digests=dig), dh=dh, du=du)
+
+
 def main(names):
     assert ref.available(), "build the reference first: make -C oracle ref"
     if names:
         for n in names:
             {"tc5_l3_dissipation": dissipation_case, "tc5_l3_recon_adjoint": recon_adjoint_case,
              "tc5_l6_1day_digest": level6_day_digest, "tc6_l7_window_digest": level7_window_digest,
-             "l7_ic_digest": level7_ic_digest}[n]()
+             "l7_ic_digest": level7_ic_digest, "quirks": quirk_cases}[n]()
         return
 
     # 1. Williamson TC2, level 5, dt = 300 s, 288 steps (one day)
@@ -216,6 +246,9 @@ def main(names):
 
     # 9. level-7 initial conditions of the four test cases, as digests
     level7_ic_digest()
+
+    # 10. TC1 (frozen wind) and thickness advection orders 3/4
+    quirk_cases()
 
 
 if __name__ == "__main__":

@@ -370,3 +370,104 @@ def test_benchmark_window_against_reference_fortran():
         del m
     for a, b in zip(out[True], out[False]):
         assert rel_l2(a, b) < 1e-12
+
+
+@pytest.mark.parametrize("B", [1, 64])
+def test_tc1_frozen_wind_against_reference_fortran(B):
+    """TC1 (cosine-bell advection with the wind held fixed), level 3, 12
+    steps of 900 s: every forward member and the reference-order TLM of
+    member 0 equal the reference Fortran's to the bit, with u and u_tl
+    unchanged for all members; the fused-flux TLM within 1e-12; the adjoint
+    of TC1, which this library does not implement, is refused with
+    MSW_EUNSUPPORTED."""
+    print(f"tc1 frozen wind L3 B={B} vs reference", flush=True)
+    from mpas_sw_amd._lib import MSWError, MSW_EUNSUPPORTED
+    g, meta = golden("tc1_l3_frozen_wind")
+    n, dt = meta["nsteps"], meta["dt"]
+    mesh, phys = case(3, 1)
+    dh, du = perturbations(mesh, B, seed=91)
+    dh[:, 0], du[:, 0] = g["dh"], g["du"]
+    col = (lambda a: a[:, 0]) if B > 1 else (lambda a: a)
+    mem = (lambda a: a) if B > 1 else (lambda a: a[:, 0])
+    for fused in (False, True):
+        m = _model(3, 1, dt, graphs=True, batch=B, tl_fused_flux=fused)
+        h = m.to_internal(mem(np.repeat(phys["h"][:, None], B, 1)), "c")
+        u = m.to_internal(mem(np.repeat(phys["u"][:, None], B, 1)), "e")
+        m.fwd(h, u, n)
+        hb = m.to_internal(phys["h"], "c", member=False)
+        ub = m.to_internal(phys["u"], "e", member=False)
+        ht, ut = m.to_internal(mem(dh), "c"), m.to_internal(mem(du), "e")
+        m.tlm(hb, ub, ht, ut, n)
+        torch.cuda.synchronize()
+        H = m.to_reference(h, "c").reshape(-1, B)
+        U = m.to_reference(u, "e").reshape(-1, B)
+        for b in range(B):
+            np.testing.assert_array_equal(H[:, b], g["h"])
+            np.testing.assert_array_equal(U[:, b], g["u"])
+        np.testing.assert_array_equal(m.to_reference(hb, "c"), g["h_base"])
+        np.testing.assert_array_equal(m.to_reference(ub, "e"), g["u_base"])
+        HT, UT = m.to_reference(ht, "c"), m.to_reference(ut, "e")
+        np.testing.assert_array_equal(UT.reshape(-1, B), du)
+        if fused:
+            assert rel_l2(col(HT), g["h_tl"]) < 1e-12
+        else:
+            np.testing.assert_array_equal(col(HT), g["h_tl"])
+            np.testing.assert_array_equal(col(UT), g["u_tl"])
+    th = torch.empty((1, m.nC), dtype=torch.float64, device=m.device)
+    tu = torch.empty((1, m.nE), dtype=torch.float64, device=m.device)
+    with pytest.raises(MSWError) as err:
+        m.adj(th, tu, ht, ut, 1)
+    assert err.value.code == MSW_EUNSUPPORTED
+
+
+def test_thickness_advection_orders_against_reference_fortran():
+    """Orders 3 and 4 of config_thickness_adv_order (3 also with
+    config_monotonic) run as order 2, as the reference's do (its deriv_two
+    stays zero): the TC5 forward and reference-order TLM after 8 steps hash
+    to the reference Fortran's digests, the adjoint is bit-identical to the
+    order-2 adjoint, and the order-2 adjoint is within 1e-11 of the oracle."""
+    print("thickness advection orders 3/4 L3 tc5 vs reference", flush=True)
+    import hashlib
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+    g, meta = golden("tc5_l3_adv_order_digest")
+    n, dt, dig = meta["nsteps"], meta["dt"], meta["digests"]
+    mesh, phys = case(3, 5)
+    adj = {}
+    for key, order, mono in (("order2", 2, False), ("order3", 3, False), ("order3_monotonic", 3, True),
+                             ("order4", 4, False)):
+        m = _model(3, 5, dt, thickness_adv_order=order, monotonic=mono)
+        h = m.to_internal(phys["h"], "c")
+        u = m.to_internal(phys["u"], "e")
+        m.fwd(h, u, n)
+        hb = m.to_internal(phys["h"], "c", member=False)
+        ub = m.to_internal(phys["u"], "e", member=False)
+        ht, ut = m.to_internal(g["dh"], "c"), m.to_internal(g["du"], "e")
+        th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+        tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+        m.tlm(hb, ub, ht, ut, n, th, tu)
+        torch.cuda.synchronize()
+        got = [sha(m.to_reference(h, "c")), sha(m.to_reference(u, "e")),
+               sha(m.to_reference(ht, "c")), sha(m.to_reference(ut, "e"))]
+        assert got == dig[key][:4], key
+        ha, ua = m.to_internal(g["dh"], "c"), m.to_internal(g["du"], "e")
+        m.adj(th, tu, ha, ua, n)
+        torch.cuda.synchronize()
+        adj[key] = (m.to_reference(ha, "c"), m.to_reference(ua, "e"))
+    for key in adj:
+        np.testing.assert_array_equal(adj[key][0], adj["order2"][0], err_msg=key)
+        np.testing.assert_array_equal(adj[key][1], adj["order2"][1], err_msg=key)
+    from helpers import make_oracle
+    o = make_oracle(3, 5, dt)
+    traj = []
+    for _ in range(n):
+        traj.append((o["h1"].copy(), o["u1"].copy()))
+        o.rk4()
+    o["h_ad1"][:] = g["dh"]
+    o["u_ad1"][:] = g["du"]
+    for s in range(n - 1, -1, -1):
+        o["h1"][:] = traj[s][0]
+        o["u1"][:] = traj[s][1]
+        o.rk4_adj()
+    assert sha(o["h_ad1"]) == dig["order2"][4] and sha(o["u_ad1"]) == dig["order2"][5]
+    assert rel_l2(adj["order2"][0], o["h_ad1"]) < 1e-11
+    assert rel_l2(adj["order2"][1], o["u_ad1"]) < 1e-11

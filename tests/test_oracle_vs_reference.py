@@ -172,3 +172,57 @@ def test_level6_one_day_forward_digest():
     o.rk4(meta["nsteps"])
     assert _sha(o["h1"]) == meta["sha_h"]
     assert _sha(o["u1"]) == meta["sha_u"]
+
+
+def test_tc1_frozen_wind_bitwise():
+    """TC1 (cosine-bell advection, src/mpas_sw_testcases.f90:1-66): the
+    reference keeps u, and in the TLM u_tl, at their initial values
+    (src/mpas_sw_module.f90:599-605,738-743) while h is advected."""
+    g, meta = golden("tc1_l3_frozen_wind")
+    n, dt = meta["nsteps"], meta["dt"]
+    mesh, phys = case(3, 1)
+    o = make_oracle(3, 1, dt)
+    o.rk4(n)
+    np.testing.assert_array_equal(o["h1"], g["h"])
+    np.testing.assert_array_equal(o["u1"], g["u"])
+    np.testing.assert_array_equal(g["u"], phys["u"])
+    o = make_oracle(3, 1, dt)
+    o["h_tl1"][:] = g["dh"]
+    o["u_tl1"][:] = g["du"]
+    o.rk4_tlm(n)
+    for k, ref in (("h1", "h_base"), ("u1", "u_base"), ("h_tl1", "h_tl"), ("u_tl1", "u_tl")):
+        np.testing.assert_array_equal(o[k], g[ref], err_msg=k)
+    np.testing.assert_array_equal(g["u_tl"], g["du"])
+
+
+def test_thickness_advection_orders_are_order_two():
+    """The reference's orders 3 and 4 (with and without config_monotonic)
+    hash to the same TC5 forward, TLM and adjoint as order 2, because
+    deriv_two stays zero (src/mpas_sw_module.f90:285-286); the oracle's
+    order-2 run gives those digests."""
+    import hashlib
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+    g, meta = golden("tc5_l3_adv_order_digest")
+    n, dt = meta["nsteps"], meta["dt"]
+    dig = meta["digests"]
+    assert set(dig) == {"order2", "order3", "order3_monotonic", "order4"}
+    assert all(v == dig["order2"] for v in dig.values())
+    o = make_oracle(3, 5, dt)
+    traj = []
+    for _ in range(n):
+        traj.append((o["h1"].copy(), o["u1"].copy()))
+        o.rk4()
+    fwd = (sha(o["h1"]), sha(o["u1"]))
+    o = make_oracle(3, 5, dt)
+    o["h_tl1"][:] = g["dh"]
+    o["u_tl1"][:] = g["du"]
+    o.rk4_tlm(n)
+    tlm = (sha(o["h_tl1"]), sha(o["u_tl1"]))
+    o["h_ad1"][:] = g["dh"]
+    o["u_ad1"][:] = g["du"]
+    for s in range(n - 1, -1, -1):
+        o["h1"][:] = traj[s][0]
+        o["u1"][:] = traj[s][1]
+        o.rk4_adj()
+    adj = (sha(o["h_ad1"]), sha(o["u_ad1"]))
+    assert list(fwd + tlm + adj) == dig["order2"]
