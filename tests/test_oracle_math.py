@@ -155,3 +155,57 @@ def test_rk4_stage_weights_on_a_uniform_tendency():
     o.rk4(3)
     assert np.all(o["h1"] == 1000.0) and np.all(o["u1"] == 0.0)
     assert rel_l2(o["h1"], np.full(mesh.nCells, 1000.0)) == 0.0
+
+
+def test_dense_jacobians_on_the_12_cell_mesh():
+    """Level 0 (12 cells, 30 edges): the 3-step TLM matrix, built column by
+    column from unit perturbations, equals central differences of the
+    forward, and the adjoint applied to unit seeds gives its rows (L^T e_i
+    is row i of L, so the adjoint outputs stacked as rows rebuild the same
+    matrix) -- the brute-force form of the tangent-linear and adjoint
+    relations, on a mesh small enough to form them whole."""
+    level, tc, dt, n = 0, 5, 1800.0, 3
+    mesh, phys = case(level, tc)
+    C, E = mesh.nCells, mesh.nEdges
+    N = C + E
+
+    def unit(k):
+        x = np.zeros(N)
+        x[k] = 1.0
+        return x[:C], x[C:]
+
+    M = np.empty((N, N))
+    for k in range(N):
+        o = make_oracle(level, tc, dt)
+        o["h_tl1"][:], o["u_tl1"][:] = unit(k)
+        o.rk4_tlm(n)
+        M[:, k] = np.concatenate([o["h_tl1"], o["u_tl1"]])
+    o = make_oracle(level, tc, dt)
+    traj = []
+    for _ in range(n):
+        traj.append((o["h1"].copy(), o["u1"].copy()))
+        o.rk4()
+    A = np.empty((N, N))
+    for i in range(N):
+        o = make_oracle(level, tc, dt)
+        o["h_ad1"][:], o["u_ad1"][:] = unit(i)
+        for s in range(n - 1, -1, -1):
+            o["h1"][:] = traj[s][0]
+            o["u1"][:] = traj[s][1]
+            o.rk4_adj()
+        A[i, :] = np.concatenate([o["h_ad1"], o["u_ad1"]])
+    assert np.abs(A - M).max() < 1e-12 * np.abs(M).max()
+    assert np.abs(A - M.T).max() > 1e-3 * np.abs(M).max()   # L is far from symmetric
+    # central differences, step 1e-3 m in h and 1e-4 m/s in u
+    x0 = np.concatenate([phys["h"], phys["u"]])
+    steps = np.concatenate([np.full(C, 1e-3), np.full(E, 1e-4)])
+    FD = np.empty((N, N))
+    for k in range(N):
+        d = np.zeros(N)
+        d[k] = steps[k]
+        hp, up = _run_fwd(level, tc, dt, (x0 + d)[:C], (x0 + d)[C:], n)
+        hm, um = _run_fwd(level, tc, dt, (x0 - d)[:C], (x0 - d)[C:], n)
+        FD[:, k] = np.concatenate([hp - hm, up - um]) / (2 * steps[k])
+    assert np.abs(FD - M).max() < 1e-6 * np.abs(M).max()
+    # and the matrix is not trivial: every state variable reaches others
+    assert np.count_nonzero(np.abs(M - np.eye(N)) > 1e-12 * np.abs(M).max()) > N * N // 4
