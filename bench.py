@@ -17,8 +17,10 @@ The TL tendency is formed from the fused TL thickness flux
 (MSW_OPT_TL_FUSED_FLUX; same TLM to rounding).  Also reported: the same
 batch in the reference's term order (bit-identical TLM), the plain unbatched
 FP64 path (B = 1, same kernels, reference order) and its ratio, the
-FP32-member variant, the algorithmic-bytes roofline, and the reference
-Fortran timed on this host's CPU (one core) as cpu_baseline.
+FP32-member variant, the same batch on TC6 with the physical Coriolis
+parameter f = 2 Omega sin(lat) (the reference's TC6 leaves f = 0), the
+algorithmic-bytes roofline, and the reference Fortran timed on this host's
+CPU (one core) as cpu_baseline.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -306,6 +308,8 @@ def main():
     ap.add_argument("--level", type=int, default=LEVEL)
     ap.add_argument("--no-graphs", action="store_true", help="launch stage kernels directly")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-physical-f", action="store_true",
+                    help="skip the TC6 run with f = 2 Omega sin(lat)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch stripe (no autotuning runs before the timed window)")
     ap.add_argument("--calibrate", action="store_true",
@@ -368,6 +372,14 @@ def main():
     if not args.no_fp32:
         f32 = gpu_rate(mesh, phys, dh, du, BATCH, "fp32", args.steps, args.warmup, g)
         log(f"batched fp32: {f32}")
+    physf = None
+    if not args.no_physical_f:
+        from mpas_sw_amd.testcases import OMEGA
+        pf = dict(phys)
+        pf["fVertex"] = 2.0 * OMEGA * np.sin(mesh.latVertex)
+        pf["fEdge"] = 2.0 * OMEGA * np.sin(mesh.latEdge)
+        physf = gpu_rate(mesh, pf, dh, du, BATCH, "fp64", args.steps, args.warmup, g, tune=False)
+        log(f"batched fp64, TC6 with f = 2 Omega sin(lat): {physf}")
 
     K = args.steps
     bytes_step = BYTES_PER_CELL_STEP["f64"] * mesh.nCells * BATCH
@@ -418,6 +430,8 @@ def main():
             "value": f32["rate"], "ms_per_step": 1e3 * f32["seconds"] / K,
             "roofline_frac": round(BYTES_PER_CELL_STEP["f32"] * mesh.nCells * BATCH * K /
                                    f32["seconds"] / 1e9 / HBM_PEAK_GBS, 4)},
+        "tc6_physical_f": None if physf is None else {
+            "value": physf["rate"], "ms_per_step": 1e3 * physf["seconds"] / K, "finite": physf["finite"]},
         "tlm_s": res["t_tlm"], "adj_s": res["t_adj"],
         "finite": res["finite"] and (f32 is None or f32["finite"]),
     }
