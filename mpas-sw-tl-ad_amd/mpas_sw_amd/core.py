@@ -30,6 +30,52 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def mesh_desc(mesh, phys, perm):
+    """msw_mesh_desc of a mesh and its scaled metrics (``phys``, as from
+    :func:`mpas_sw_amd.testcases.initial_conditions`), with the internal
+    order ``perm`` = {"c", "e", "v"} (new -> old, int32).  Returns the
+    descriptor and the host arrays it points to, which must outlive the
+    msw_create call that reads them."""
+    keep = []
+
+    def f8(x):
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data
+
+    def i4(x):
+        a = np.ascontiguousarray(x, dtype=np.int32)
+        keep.append(a)
+        return a.ctypes.data
+
+    d = _lib.MeshDesc()
+    d.nCells, d.nEdges, d.nVertices = mesh.nCells, mesh.nEdges, mesh.nVertices
+    d.maxEdges, d.maxEdges2, d.vertexDegree = mesh.maxEdges, mesh.maxEdges2, mesh.vertexDegree
+    d.cellsOnEdge = i4(mesh.cellsOnEdge.T)
+    d.verticesOnEdge = i4(mesh.verticesOnEdge.T)
+    d.nEdgesOnCell = i4(mesh.nEdgesOnCell)
+    d.edgesOnCell = i4(mesh.edgesOnCell.T)
+    d.nEdgesOnEdge = i4(mesh.nEdgesOnEdge)
+    d.edgesOnEdge = i4(mesh.edgesOnEdge.T)
+    d.weightsOnEdge = f8(mesh.weightsOnEdge.T)
+    d.cellsOnVertex = i4(mesh.cellsOnVertex.T)
+    kite = np.asarray(phys["kiteAreasOnVertex"])
+    d.kiteAreasOnVertex = f8(kite.T if kite.ndim == 2 else kite)
+    for name in ("dcEdge", "dvEdge", "areaCell", "areaTriangle", "fVertex", "h_s"):
+        setattr(d, name, f8(phys[name]))
+    d.latCell = f8(mesh.latCell)
+    d.lonCell = f8(phys.get("lonCell", mesh.lonCell))
+    cr = phys.get("coeffs_reconstruct")
+    if cr is not None:
+        cr = np.asarray(cr, np.float64)
+        # (3, maxEdges, C) reference layout -> entity-major flat
+        d.coeffs_reconstruct = f8(np.transpose(cr, (2, 1, 0)) if cr.ndim == 3 else cr)
+    d.cellPerm = i4(perm["c"])
+    d.edgePerm = i4(perm["e"])
+    d.vertexPerm = i4(perm["v"])
+    return d, keep
+
+
 class SWModel:
     """One mesh + configuration on the GPU.
 
@@ -75,43 +121,7 @@ class SWModel:
                      "v": np.ascontiguousarray(pv, np.int32)}
         self.cfg = _lib.Config(dt, apvm, visc2, visc4, test_case, thickness_adv_order,
                                int(monotonic), int(wind_stress), int(bottom_drag))
-        keep = []
-
-        def f8(x):
-            a = np.ascontiguousarray(x, dtype=np.float64)
-            keep.append(a)
-            return a.ctypes.data
-
-        def i4(x):
-            a = np.ascontiguousarray(x, dtype=np.int32)
-            keep.append(a)
-            return a.ctypes.data
-
-        d = _lib.MeshDesc()
-        d.nCells, d.nEdges, d.nVertices = self.nC, self.nE, self.nV
-        d.maxEdges, d.maxEdges2, d.vertexDegree = mesh.maxEdges, mesh.maxEdges2, mesh.vertexDegree
-        d.cellsOnEdge = i4(mesh.cellsOnEdge.T)
-        d.verticesOnEdge = i4(mesh.verticesOnEdge.T)
-        d.nEdgesOnCell = i4(mesh.nEdgesOnCell)
-        d.edgesOnCell = i4(mesh.edgesOnCell.T)
-        d.nEdgesOnEdge = i4(mesh.nEdgesOnEdge)
-        d.edgesOnEdge = i4(mesh.edgesOnEdge.T)
-        d.weightsOnEdge = f8(mesh.weightsOnEdge.T)
-        d.cellsOnVertex = i4(mesh.cellsOnVertex.T)
-        kite = np.asarray(phys["kiteAreasOnVertex"])
-        d.kiteAreasOnVertex = f8(kite.T if kite.ndim == 2 else kite)
-        for name in ("dcEdge", "dvEdge", "areaCell", "areaTriangle", "fVertex", "h_s"):
-            setattr(d, name, f8(phys[name]))
-        d.latCell = f8(mesh.latCell)
-        d.lonCell = f8(phys.get("lonCell", mesh.lonCell))
-        cr = phys.get("coeffs_reconstruct")
-        if cr is not None:
-            cr = np.asarray(cr, np.float64)
-            # (3, maxEdges, C) reference layout -> entity-major flat
-            d.coeffs_reconstruct = f8(np.transpose(cr, (2, 1, 0)) if cr.ndim == 3 else cr)
-        d.cellPerm = self.perm["c"].ctypes.data
-        d.edgePerm = self.perm["e"].ctypes.data
-        d.vertexPerm = self.perm["v"].ctypes.data
+        d, keep = mesh_desc(mesh, phys, self.perm)   # keep: host arrays d points to
         with torch.cuda.device(self.device):
             self.stream = torch.cuda.current_stream(self.device)
             h = C.c_void_p()

@@ -704,3 +704,34 @@ def test_dense_jacobians_on_the_12_cell_mesh():
         o.rk4_tlm(n)
         np.testing.assert_array_equal(M[:, k], np.concatenate([o["h_tl1"], o["u_tl1"]]), err_msg=f"column {k}")
     assert np.abs(A - M).max() < 1e-12 * np.abs(M).max()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_nonfinite_flags_only_the_broken_members(prec):
+    """A NaN planted in one member's h and an Inf in another member's u,
+    then two TLM steps: the per-member detector flags exactly those two
+    members, and the other members' states stay finite and equal to a run
+    without the faults."""
+    print(f"nonfinite detector L3 tc5 B=64 {prec}", flush=True)
+    tc, dt, B, n = 5, 900.0, 64, 2
+    mesh, phys = case(LEVEL, tc)
+    dh, du = perturbations(mesh, B, seed=5)
+    runs = []
+    for plant in (False, True):
+        m = model(LEVEL, tc, dt, batch=B, precision=prec)
+        h = m.to_internal(phys["h"], "c", member=False)
+        u = m.to_internal(phys["u"], "e", member=False)
+        ddh, ddu = dh.copy(), du.copy()
+        if plant:
+            ddh[100, 17] = np.nan
+            ddu[333, 40] = np.inf
+        ht, ut = m.to_internal(ddh, "c"), m.to_internal(ddu, "e")
+        assert set(np.nonzero(m.nonfinite(ht, ut).cpu().numpy())[0]) == ({17, 40} if plant else set())
+        m.tlm(h, u, ht, ut, n)
+        torch.cuda.synchronize()
+        flags = m.nonfinite(ht, ut).cpu().numpy()
+        assert set(np.nonzero(flags)[0]) == ({17, 40} if plant else set())
+        runs.append((m.to_reference(ht, "c"), m.to_reference(ut, "e")))
+    keep = [b for b in range(B) if b not in (17, 40)]
+    for a, b in zip(runs[0], runs[1]):
+        np.testing.assert_array_equal(a[:, keep], b[:, keep])
