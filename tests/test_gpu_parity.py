@@ -735,3 +735,38 @@ def test_nonfinite_flags_only_the_broken_members(prec):
     keep = [b for b in range(B) if b not in (17, 40)]
     for a, b in zip(runs[0], runs[1]):
         np.testing.assert_array_equal(a[:, keep], b[:, keep])
+
+
+@pytest.mark.parametrize("level,prec", [(LEVEL, "fp64"), (LEVEL, "fp32"), (7, "fp64")])
+def test_pipelined_adjoint_sweep_equals_single_steps(level, prec):
+    """64 members: the multi-step adjoint sweep (each step's base stages
+    formed on the side stream while the previous step runs, alternating two
+    buffer sets) gives the same bits as the single-step entry point called
+    step by step, eagerly and through replayed hipGraphs, twice in a row;
+    level 7 is large enough for the two streams to overlap for real."""
+    print(f"pipelined adjoint sweep vs single steps L{level} B=64 {prec}", flush=True)
+    tc, B = 5, 64
+    dt, n = (900.0, 7) if level == LEVEL else (75.0, 4)
+    mesh, phys = case(level, tc)
+    dh, du = perturbations(mesh, B, seed=23)
+    outs = []
+    for graphs in (False, True):
+        m = model(level, tc, dt, batch=B, precision=prec, graphs=graphs, tl_fused_flux=True)
+        h = m.to_internal(phys["h"], "c", member=False)
+        u = m.to_internal(phys["u"], "e", member=False)
+        ht, ut = m.to_internal(dh, "c"), m.to_internal(du, "e")
+        th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+        tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+        m.tlm(h, u, ht, ut, n, th, tu)
+        for rep in range(2):
+            ha, ua = ht.clone(), ut.clone()
+            m.adj(th, tu, ha, ua, n)
+            torch.cuda.synchronize()
+            outs.append((ha.cpu(), ua.cpu()))
+        hs, us = ht.clone(), ut.clone()
+        for k in range(n - 1, -1, -1):
+            m.adj_step(th[k].contiguous(), tu[k].contiguous(), hs, us)
+        torch.cuda.synchronize()
+        outs.append((hs.cpu(), us.cpu()))
+    for a, b in outs[1:]:
+        assert torch.equal(a, outs[0][0]) and torch.equal(b, outs[0][1])
