@@ -1164,14 +1164,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_del4_edge(DMesh m, int lb,
 // Epilogue of an adjoint stage for one value
 template <typename T>
 __device__ __forceinline__ void adj_update(int stage, bool first, T w, T sw, T pa, T* lam, T* acc,
-                                           T* g, int row, int B, int o)
+                                           T* g, int row, int B, int o, bool frozen = false)
 {
     lam = rowp(lam, row, B);
     acc = rowp(acc, row, B);
     g = rowp(g, row, B);
     if (stage > 0) {
         stn(acc + o, first ? pa : ldn(acc + o) + pa);
-        stn(g + o, w * ldn(lam + o) + sw * pa);
+        // a frozen field has no tendency: its tendency adjoint stays zero and
+        // every stage's adjoint goes to the step-start state
+        // (src/mpas_sw_module.f90:907-910, :930-933)
+        if (!frozen) stn(g + o, w * ldn(lam + o) + sw * pa);
     } else {
         stn(lam + o, (ldn(acc + o) + ldn(lam + o)) + pa);
     }
@@ -1217,7 +1220,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
             // divergence(c) = sum(+-u dv) / area (:221-242)
             if (c.visc2 > 0.0 || c.visc4 > 0.0)
                 ua = ua + T(m.dv[e]) * (ld(dva, c1, B, b) - ld(dva, c2, B, b));
-            adj_update(stage, first, w, sw, ua, lamu, accu, gu, e, B, b);
+            adj_update(stage, first, w, sw, ua, lamu, accu, gu, e, B, b, c.frozen_u != 0);
         } else {
             const int ic = entity_of<WU>(t, lb);
             int iv[kC6];
@@ -1539,7 +1542,9 @@ void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, 
     // is loaded, the same product a separate scaling pass would store), then
     // what the previous stage's epilogue left in (gh, gu)
     const T* rgh = first ? lamh : gh;
-    const T* rgu = first ? lamu : gu;
+    // with the wind held fixed (test case 1) the u tendency adjoint is zero
+    // in every stage: gu then holds zeros the caller put there
+    const T* rgu = (first && !c.frozen_u) ? lamu : gu;
     const T gs = first ? T(k.w[3]) : T(1);
     // ds.dsv: circulation adjoint of nabla^2 u [V]; ds.dsd: its divergence
     // adjoint [C]; ds.dsu: nabla^2 u adjoint [E]; ds.div: divergence

@@ -648,6 +648,9 @@ int main_adj_stages(msw_ctx* c, const BaseStage* st, const AdjCoef* qs, T* lamh,
     const size_t C = c->nC, B = c->B;
     T* g = static_cast<T*>(c->mpA);
     T* acc = static_cast<T*>(c->macc);
+    // frozen wind: the u tendency adjoint (g + C*B) is read as zero by every
+    // stage and never written (a fill kernel, so graph capture stays valid)
+    if (c->dcfg.frozen_u) fill<T>(static_cast<long>(c->nE) * static_cast<long>(B), g + C * B, T(0), c->stream);
     for (int s = 3; s >= 0; --s) {
         if (wait) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev[1 + s], 0));
         adj_stage<T>(m, c->dcfg, c->lb, s, s == 3, k, st[s], g, g + C * B,
@@ -921,7 +924,6 @@ int msw_tlm_steps(msw_ctx* ctx, double* h, double* u, void* h_tl, void* u_tl, in
 static int adj_common(msw_ctx* ctx, void* h_ad, void* u_ad, int nsteps, bool want_end,
                       const std::function<void(int)>& pre)
 {
-    if (ctx->dcfg.frozen_u) return fail(MSW_EUNSUPPORTED, "adjoint of test case 1 is not implemented");
     if (!members_aligned(ctx, h_ad, u_ad)) return fail(MSW_EINVAL, "h_ad/u_ad must be 16-byte aligned");
     if (ctx->prec == MSW_FP64)
         return run_steps(ctx, want_end ? 6 : 7, h_ad, u_ad, nullptr, nullptr, nsteps, [&] {
@@ -957,7 +959,6 @@ int msw_adj_steps(msw_ctx* ctx, const double* traj_h, const double* traj_u, void
     if (!ctx || !traj_h || !traj_u || !h_ad || !u_ad || nsteps < 0) return fail(MSW_EINVAL, "bad argument");
     int rc = check_cfg(ctx);
     if (rc) return rc;
-    if (ctx->dcfg.frozen_u) return fail(MSW_EUNSUPPORTED, "adjoint of test case 1 is not implemented");
     if (!members_aligned(ctx, h_ad, u_ad)) return fail(MSW_EINVAL, "h_ad/u_ad must be 16-byte aligned");
     if (nsteps == 0) return MSW_OK;
     const size_t C = ctx->nC, E = ctx->nE;

@@ -147,8 +147,9 @@ IC_FIELDS = ("h", "u", "h_s", "fEdge", "fVertex", "lonCell")
 
 def quirk_cases():
     """Level 3, dt 900 s.  (1) TC1, the frozen-wind advection case: forward
-    h, u after 12 steps and TLM (base and h_tl, u_tl; u and u_tl stay at
-    their initial values, src/mpas_sw_module.f90:599-605,738-743).  (2)
+    h, u after 12 steps, TLM (base and h_tl, u_tl; u and u_tl stay at their
+    initial values, src/mpas_sw_module.f90:596-598,724-726) and the adjoint
+    window seeded with the same vectors (:907-910,930-933).  (2)
     Thickness advection orders 3 and 4, with and without config_monotonic:
     SHA-256 digests of the TC5 forward, TLM and adjoint after 8 steps -- the
     same as order 2's, because the reference never fills deriv_two
@@ -159,8 +160,10 @@ def quirk_cases():
     dh, du = perturb(m3, SEED + 7)
     f = ref.run(m3, "fwd", nsteps=n, test_case=1, dt=dt)
     t = ref.run(m3, "tlm", nsteps=n, test_case=1, dt=dt, tl=(dh, du))
+    a = ref.run(m3, "adj", nsteps=n, test_case=1, dt=dt, ad=(dh, du))
     save("tc1_l3_frozen_wind", m3, dict(test_case=1, dt=dt, nsteps=n, perturbation_seed=SEED + 7),
-         dh=dh, du=du, h=f["h"], u=f["u"], h_base=t["h"], u_base=t["u"], h_tl=t["h_tl"], u_tl=t["u_tl"])
+         dh=dh, du=du, h=f["h"], u=f["u"], h_base=t["h"], u_base=t["u"], h_tl=t["h_tl"], u_tl=t["u_tl"],
+         h_ad=a["h_ad"], u_ad=a["u_ad"])
     n = 8
     dh, du = perturb(m3, SEED + 8)
     dig = {}

@@ -177,7 +177,9 @@ def test_level6_one_day_forward_digest():
 def test_tc1_frozen_wind_bitwise():
     """TC1 (cosine-bell advection, src/mpas_sw_testcases.f90:1-66): the
     reference keeps u, and in the TLM u_tl, at their initial values
-    (src/mpas_sw_module.f90:599-605,738-743) while h is advected."""
+    (src/mpas_sw_module.f90:596-598,615,724-726,749-752) while h is advected;
+    its adjoint sends every stage's u adjoint to the step-start state
+    (:907-910,930-933)."""
     g, meta = golden("tc1_l3_frozen_wind")
     n, dt = meta["nsteps"], meta["dt"]
     mesh, phys = case(3, 1)
@@ -193,6 +195,19 @@ def test_tc1_frozen_wind_bitwise():
     for k, ref in (("h1", "h_base"), ("u1", "u_base"), ("h_tl1", "h_tl"), ("u_tl1", "u_tl")):
         np.testing.assert_array_equal(o[k], g[ref], err_msg=k)
     np.testing.assert_array_equal(g["u_tl"], g["du"])
+    o = make_oracle(3, 1, dt)
+    traj = []
+    for _ in range(n):
+        traj.append((o["h1"].copy(), o["u1"].copy()))
+        o.rk4()
+    o["h_ad1"][:] = g["dh"]
+    o["u_ad1"][:] = g["du"]
+    for s in range(n - 1, -1, -1):
+        o["h1"][:] = traj[s][0]
+        o["u1"][:] = traj[s][1]
+        o.rk4_adj()
+    np.testing.assert_array_equal(o["h_ad1"], g["h_ad"])
+    np.testing.assert_array_equal(o["u_ad1"], g["u_ad"])
 
 
 def test_thickness_advection_orders_are_order_two():
