@@ -145,6 +145,16 @@ def level7_ic_digest():
 IC_FIELDS = ("h", "u", "h_s", "fEdge", "fVertex", "lonCell")
 
 
+def level7_day_digest():
+    """The benchmark's mesh and case for one day: level 7, TC6, dt 75 s,
+    1152 steps.  SHA-256 digests and norms of the reference's forward h, u."""
+    m7 = get_mesh(7)
+    tc, dt, n = 6, 75.0, 1152
+    f = ref.run(m7, "fwd", nsteps=n, test_case=tc, dt=dt)
+    cfg = dict(test_case=tc, dt=dt, nsteps=n, sha_h=sha(f["h"]), sha_u=sha(f["u"]))
+    save("tc6_l7_1day_digest", m7, cfg, norms=np.array([np.linalg.norm(f["h"]), np.linalg.norm(f["u"])]))
+
+
 def quirk_cases():
     """Level 3, dt 900 s.  (1) TC1, the frozen-wind advection case: forward
     h, u after 12 steps, TLM (base and h_tl, u_tl; u and u_tl stay at their
@@ -184,7 +194,8 @@ def main(names):
         for n in names:
             {"tc5_l3_dissipation": dissipation_case, "tc5_l3_recon_adjoint": recon_adjoint_case,
              "tc5_l6_1day_digest": level6_day_digest, "tc6_l7_window_digest": level7_window_digest,
-             "l7_ic_digest": level7_ic_digest, "quirks": quirk_cases}[n]()
+             "l7_ic_digest": level7_ic_digest, "quirks": quirk_cases,
+             "tc6_l7_1day_digest": level7_day_digest}[n]()
         return
 
     # 1. Williamson TC2, level 5, dt = 300 s, 288 steps (one day)
@@ -252,6 +263,9 @@ def main(names):
 
     # 10. TC1 (frozen wind) and thickness advection orders 3/4
     quirk_cases()
+
+    # 11. one day on the benchmark's mesh and case, as digests
+    level7_day_digest()
 
 
 if __name__ == "__main__":

@@ -478,3 +478,33 @@ def test_thickness_advection_orders_against_reference_fortran():
     assert sha(o["h_ad1"]) == dig["order2"][4] and sha(o["u_ad1"]) == dig["order2"][5]
     assert rel_l2(adj["order2"][0], o["h_ad1"]) < 1e-11
     assert rel_l2(adj["order2"][1], o["u_ad1"]) < 1e-11
+
+
+def test_level7_one_day_forward_and_fp32_accuracy():
+    """The benchmark's mesh and case (level 7, TC6, dt 75 s): the FP64 GPU
+    forward after one day (1152 steps) hashes to the reference Fortran's;
+    the FP32 forward of the same initial state stays within 1e-5 relative L2
+    of it after one day and within 1e-4 after five days (5760 steps)."""
+    print("L7 tc6 one-day forward vs reference, FP32 accuracy 1 and 5 days", flush=True)
+    g, meta = golden("tc6_l7_1day_digest")
+    n, dt = meta["nsteps"], meta["dt"]
+    mesh, phys = case(7, 6)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        m = _model(7, 6, dt, graphs=True, precision=prec)
+        h = m.to_internal(phys["h"], "c")
+        u = m.to_internal(phys["u"], "e")
+        m.fwd(h, u, n)
+        torch.cuda.synchronize()
+        day1 = (m.to_reference(h, "c"), m.to_reference(u, "e"))
+        if prec == "fp64":
+            assert _sha(day1[0]) == meta["sha_h"] and _sha(day1[1]) == meta["sha_u"]
+        m.fwd(h, u, 4 * n)
+        torch.cuda.synchronize()
+        out[prec] = (day1, (m.to_reference(h, "c"), m.to_reference(u, "e")))
+        del m
+    for day, tol in ((0, 1e-5), (1, 1e-4)):
+        eh = rel_l2(out["fp32"][day][0], out["fp64"][day][0])
+        eu = rel_l2(out["fp32"][day][1], out["fp64"][day][1])
+        print(f"FP32 vs FP64 after {1 if day == 0 else 5} day(s): h {eh:.2e}, u {eu:.2e}", flush=True)
+        assert eh < tol and eu < tol
