@@ -93,10 +93,10 @@ def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True, f
     ht0 = m.to_internal(dh[:, sel] if batch > 1 else dh[:, 0], "c")
     ut0 = m.to_internal(du[:, sel] if batch > 1 else du[:, 0], "e")
     if tune:
-        tune = m.autotune(h0, u0)     # launch stripe per XCD, chosen by timing (not timed here)
-        log(f"stripe autotune B={batch} {precision}: {tune}")
+        tune = m.autotune(h0, u0)     # launch stripe and block size, chosen by timing (not timed here)
+        log(f"launch autotune B={batch} {precision} (stripe, block): {tune}")
     else:
-        tune = {m.stripe_log2: 0.0}
+        tune = {(m.stripe_log2, m.block): 0.0}
     n = max(steps, warmup)
     bufs = (torch.empty_like(h0), torch.empty_like(u0), torch.empty_like(ht0), torch.empty_like(ut0),
             torch.empty((n, m.nC), dtype=torch.float64, device=m.device),
@@ -110,7 +110,8 @@ def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True, f
     torch.cuda.empty_cache()
     T = t_tlm + t_adj
     return dict(seconds=T, wall=wall, t_tlm=t_tlm, t_adj=t_adj,
-                rate=mesh.nCells * batch * steps / T, finite=ok, stripe_log2=min(tune, key=tune.get))
+                rate=mesh.nCells * batch * steps / T, finite=ok, stripe_log2=min(tune, key=tune.get)[0],
+                block=min(tune, key=tune.get)[1])
 
 
 def calibration_copy(nbytes=1 << 30, reps=3):
@@ -419,7 +420,7 @@ def main():
                    "cells": mesh.nCells, "edges": mesh.nEdges, "vertices": mesh.nVertices,
                    "batch": BATCH, "dt": DT, "window_steps": K,
                    "tl_tendency": "fused TL flux (MSW_OPT_TL_FUSED_FLUX)",
-                   "xcd_stripe_log2": res["stripe_log2"]},
+                   "xcd_stripe_log2": res["stripe_log2"], "block": res["block"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp64_reference_term_order": None if exact is None else {

@@ -116,8 +116,13 @@ int msw_set_graphs(msw_ctx *ctx, int enable);
  * MSW_OPT_XCD_STRIPE (default 2): log2 of the number of consecutive blocks of
  * a launch that go to one XCD before the next XCD's turn (0..6).  Results do
  * not depend on it; the fastest value depends on mesh, batch and precision
- * (the Python layer's SWModel.autotune times the candidates). */
-enum msw_option { MSW_OPT_TL_FUSED_FLUX = 1, MSW_OPT_XCD_STRIPE = 2 };
+ * (the Python layer's SWModel.autotune times the candidates).
+ *
+ * MSW_OPT_BLOCK (default 256): threads per block of the stage kernels, 64,
+ * 128, 256 or 512 (with 64 or more members a wavefront is one entity, so a
+ * block covers block/64 consecutive entities).  Results do not depend on it;
+ * SWModel.autotune times it after the stripe. */
+enum msw_option { MSW_OPT_TL_FUSED_FLUX = 1, MSW_OPT_XCD_STRIPE = 2, MSW_OPT_BLOCK = 3 };
 int msw_set_option(msw_ctx *ctx, int option, int value);
 
 /* Replaces: sw_rk4 (src/mpas_sw_module.f90:528-635), nsteps times, on a

@@ -29,6 +29,7 @@ constexpr double kDragCoef = 1.0e-3;   // bottom drag, src/mpas_sw_dyncore.f90:6
 struct DMesh {
     int nC, nE, nV;
     int stripe_log2;      // blocks are dealt to the 8 XCDs in stripes of 2^stripe_log2
+    int block;            // threads per block of those launches: 64, 128, 256 or 512
     // cells
     const int* nEoC;      // [C]
     const int* eoc;       // [C][6]  edgesOnCell, slot order

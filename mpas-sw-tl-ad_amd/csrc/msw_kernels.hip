@@ -28,7 +28,8 @@ namespace msw {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;      // grid-stride and reduction kernels
+constexpr int kMaxBlock = 512;   // largest block of the striped launches (DMesh::block)
 constexpr int kE2 = kMaxEdges2;    // stride of the edgesOnEdge tables
 constexpr int kEoE = kEdgesOnEdge;  // slots the member gathers visit
 constexpr int kC6 = kMaxEdges;
@@ -73,16 +74,16 @@ inline long round_blocks(long nb, int sl)
 }
 
 // blocks for `items` threads, rounded up to a multiple of 8 * 2^sl
-inline int grid_items(long items, int sl)
+inline int grid_items(long items, int sl, int block)
 {
-    const long nb = (items + kBlock - 1) / kBlock;
+    const long nb = (items + block - 1) / block;
     return static_cast<int>(round_blocks(nb > 0 ? nb : 1, sl));
 }
 
-inline int grid_edge_cell(int nE, int nC, int lb, int sl)
+inline int grid_edge_cell(int nE, int nC, int lb, int sl, int block)
 {
-    const long be = ((static_cast<long>(nE) << lb) + kBlock - 1) / kBlock;
-    const long bc = ((static_cast<long>(nC) << lb) + kBlock - 1) / kBlock;
+    const long be = ((static_cast<long>(nE) << lb) + block - 1) / block;
+    const long bc = ((static_cast<long>(nC) << lb) + block - 1) / block;
     return static_cast<int>(round_blocks(be, sl) + round_blocks(bc, sl));
 }
 
@@ -99,7 +100,7 @@ inline int grid_for(long threads)
 // one item per thread over `count` entities
 #define ENTITY_ITEM(count)                                                     \
     const unsigned total_ = static_cast<unsigned>(count) << lb;                \
-    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * kBlock + threadIdx.x; \
+    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * static_cast<unsigned>(m.block) + threadIdx.x; \
     if (t >= total_) return;
 
 // one item per thread over the edges (is_edge: the first round_blocks(nE
@@ -107,14 +108,15 @@ inline int grid_for(long threads)
 #define EDGE_CELL_ITEM(is_edge, t)                                             \
     const unsigned tE_ = static_cast<unsigned>(m.nE) << lb;                    \
     const unsigned tC_ = static_cast<unsigned>(m.nC) << lb;                    \
-    const unsigned gE_ = round_blocks_dev((tE_ + kBlock - 1) / kBlock, m.stripe_log2); \
+    const unsigned bs_ = static_cast<unsigned>(m.block);                      \
+    const unsigned gE_ = round_blocks_dev((tE_ + bs_ - 1) / bs_, m.stripe_log2); \
     const bool is_edge = blockIdx.x < gE_;                                     \
-    const unsigned t = stripe_block(is_edge ? blockIdx.x : blockIdx.x - gE_, m.stripe_log2) * kBlock + threadIdx.x; \
+    const unsigned t = stripe_block(is_edge ? blockIdx.x : blockIdx.x - gE_, m.stripe_log2) * bs_ + threadIdx.x; \
     if (t >= (is_edge ? tE_ : tC_)) return;
 
 // one flat item per thread (no member axis), blocks dealt XCD-aware
 #define FLAT_ITEM(count)                                                       \
-    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * kBlock + threadIdx.x; \
+    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * static_cast<unsigned>(m.block) + threadIdx.x; \
     if (t >= static_cast<unsigned>(count)) return;
 
 #define GRID_LOOP(total)                                                               \
@@ -222,7 +224,7 @@ __device__ __forceinline__ void ld_rows(const T* p, const int (&r)[N], int B, co
 
 // circulation/vorticity (:208-219), h_vertex and pv_vertex (:282-292)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_fwd_vertex(DMesh m, int lb, const T* __restrict__ h,
+__global__ __launch_bounds__(kMaxBlock) void k_fwd_vertex(DMesh m, int lb, const T* __restrict__ h,
                                                        const T* __restrict__ u, T* __restrict__ pvv,
                                                        T* __restrict__ hv, T* __restrict__ vort)
 {
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_vertex(DMesh m, int lb, const T*
 
 // kinetic energy (:245-256), pv_cell (:324-340)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_fwd_cell(DMesh m, int lb, const T* __restrict__ u,
+__global__ __launch_bounds__(kMaxBlock) void k_fwd_cell(DMesh m, int lb, const T* __restrict__ u,
                                                      const T* __restrict__ pvv, T* __restrict__ ke,
                                                      T* __restrict__ pc)
 {
@@ -286,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_cell(DMesh m, int lb, const T* _
 // h_edge (:95-105), v (:259-267), gradPVt (:296-301), pv_edge (:305-320),
 // gradPVn and the second upwinding (:344-359)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_fwd_edge(DMesh m, DConfig c, int lb,
+__global__ __launch_bounds__(kMaxBlock) void k_fwd_edge(DMesh m, DConfig c, int lb,
                                                      const T* __restrict__ h,
                                                      const T* __restrict__ u,
                                                      const T* __restrict__ pvv,
@@ -389,7 +391,7 @@ __device__ __forceinline__ T tend_forcing(const DMesh& m, const DConfig& c, cons
 // divergence (src/mpas_sw_dyncore.f90:221-242); the TLM applies the same
 // operator to u_tl (src/mpas_sw_dyncore_tlm.f90)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_div(DMesh m, int lb, const T* __restrict__ u,
+__global__ __launch_bounds__(kMaxBlock) void k_div(DMesh m, int lb, const T* __restrict__ u,
                                                 T* __restrict__ div)
 {
     const int B = 1 << lb;
@@ -413,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_div(DMesh m, int lb, const T* __rest
 
 // nabla^2 u on edges (src/mpas_sw_dyncore.f90:538-550)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_del4_edge(DMesh m, int lb, const T* __restrict__ div,
+__global__ __launch_bounds__(kMaxBlock) void k_del4_edge(DMesh m, int lb, const T* __restrict__ div,
                                                       const T* __restrict__ vort,
                                                       T* __restrict__ dsu)
 {
@@ -429,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_del4_edge(DMesh m, int lb, const T* 
 
 // vorticity of nabla^2 u (:552-570)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_del4_vertex(DMesh m, int lb, const T* __restrict__ dsu,
+__global__ __launch_bounds__(kMaxBlock) void k_del4_vertex(DMesh m, int lb, const T* __restrict__ dsu,
                                                         T* __restrict__ dsv)
 {
     const int B = 1 << lb;
@@ -453,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_del4_vertex(DMesh m, int lb, const T
 
 // divergence of nabla^2 u (:572-589)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_del4_cell(DMesh m, int lb, const T* __restrict__ dsu,
+__global__ __launch_bounds__(kMaxBlock) void k_del4_cell(DMesh m, int lb, const T* __restrict__ dsu,
                                                       T* __restrict__ dsd)
 {
     const int B = 1 << lb;
@@ -477,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_del4_cell(DMesh m, int lb, const T* 
 
 // tend_u (:485-505) on edges and tend_h (:468-482) on cells, then RK update
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb, int stage,
+__global__ __launch_bounds__(kMaxBlock) void k_fwd_tend(DMesh m, DConfig c, int lb, int stage,
                                                      T w, T sw, const T* __restrict__ h,
                                                      const T* __restrict__ u,
                                                      const T* __restrict__ he,
@@ -555,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tend(DMesh m, DConfig c, int lb,
 
 // :255-271, :344-360
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_tl_vertex(DMesh m, int lb, const double* __restrict__ hvb,
+__global__ __launch_bounds__(kMaxBlock) void k_tl_vertex(DMesh m, int lb, const double* __restrict__ hvb,
                                                       const double* __restrict__ vortb,
                                                       const T* __restrict__ h_tl,
                                                       const T* __restrict__ u_tl,
@@ -591,7 +593,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_vertex(DMesh m, int lb, const dou
 
 // :301-316, :401-421
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_tl_cell(DMesh m, int lb, const double* __restrict__ c32,
+__global__ __launch_bounds__(kMaxBlock) void k_tl_cell(DMesh m, int lb, const double* __restrict__ c32,
                                                     const T* __restrict__ u_tl,
                                                     const T* __restrict__ pvv_tl,
                                                     T* __restrict__ ke_tl, T* __restrict__ pc_tl)
@@ -627,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_cell(DMesh m, int lb, const doubl
 
 // :117-126 (h_edge_tl), :319-329, :364-393, :425-445
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_tl_edge(DMesh m, DConfig c, int lb, BaseStage bs,
+__global__ __launch_bounds__(kMaxBlock) void k_tl_edge(DMesh m, DConfig c, int lb, BaseStage bs,
                                                     const T* __restrict__ h_tl,
                                                     const T* __restrict__ u_tl,
                                                     const T* __restrict__ pvv_tl,
@@ -674,7 +676,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_edge(DMesh m, DConfig c, int lb, 
 //            for the 12 edgesOnEdge slots (src/mpas_sw_dyncore_tlm.f90:564-595)
 //   c32[c] = h_edge, u*dvEdge, dvEdge over the reference-ordered edges of the
 //            cell (:541-561); 2*u and 0.25*dcEdge*dvEdge over its edge slots (:301-316)
-__global__ __launch_bounds__(kBlock) void k_tl_coef(DMesh m, BaseStage bs, int fused,
+__global__ __launch_bounds__(kMaxBlock) void k_tl_coef(DMesh m, BaseStage bs, int fused,
                                                     double* __restrict__ e48,
                                                     double* __restrict__ c32)
 {
@@ -744,7 +746,7 @@ __device__ __forceinline__ T tend_forcing_tl(const DMesh& m, const DConfig& c, c
 
 // tend_u_tl (:564-595) and tend_h_tl (:541-561), then RK update
 template <typename T, bool WU, bool FUSED>
-__global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, int stage, T w,
+__global__ __launch_bounds__(kMaxBlock) void k_tl_tend(DMesh m, DConfig c, int lb, int stage, T w,
                                                     T sw, BaseStage bs, const double* __restrict__ e48,
                                                     const double* __restrict__ c32,
                                                     const T* __restrict__ h_tl,
@@ -836,13 +838,13 @@ __global__ __launch_bounds__(kBlock) void k_tl_tend(DMesh m, DConfig c, int lb, 
 // edges: tend_u/q adjoint (src/mpas_sw_dyncore_adj.f90:689-717), tend_h
 // adjoint (:720-736), normal upwinding (:193-200), drag and wind (:549-572),
 // v -> u through the transposed edgesOnEdge (:296-304), ke -> u (:306-318)
-__global__ __launch_bounds__(kBlock) void k_adj_coef_edge(DMesh m, DConfig c, BaseStage bs,
+__global__ __launch_bounds__(kMaxBlock) void k_adj_coef_edge(DMesh m, DConfig c, BaseStage bs,
                                                           double* __restrict__ e24,
                                                           double* __restrict__ e16)
 {
     // 16 lanes per edge (all of them active through the reduction below):
     // lanes 0-9 fill the edgesOnEdge slots, lanes 12-15 the per-edge values
-    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * kBlock + threadIdx.x;
+    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * static_cast<unsigned>(m.block) + threadIdx.x;
     const unsigned total = 16u * static_cast<unsigned>(m.nE);
     if (t >= total) return;   // whole 16-lane groups only, so the shuffles stay inside live lanes
     const int e = static_cast<int>(t >> 4), j = static_cast<int>(t & 15);
@@ -893,7 +895,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_coef_edge(DMesh m, DConfig c, Ba
 
 // vertices: pv_cell -> pv_vertex (:215-229), pv_edge and gradPVt ->
 // pv_vertex (:235-262), pv_vertex -> h_vertex, vorticity (:268-284)
-__global__ __launch_bounds__(kBlock) void k_adj_coef_vertex(DMesh m, DConfig c, BaseStage bs,
+__global__ __launch_bounds__(kMaxBlock) void k_adj_coef_vertex(DMesh m, DConfig c, BaseStage bs,
                                                             double* __restrict__ v8)
 {
     FLAT_ITEM(8u * static_cast<unsigned>(m.nV))
@@ -915,7 +917,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_coef_vertex(DMesh m, DConfig c, 
 
 // cells: gradPVn -> pv_cell (:202-213), gravity -> h (:696-697), ke
 // gradient and drag -> ke (:555-562, :693-695), h_edge -> h (:358-369)
-__global__ __launch_bounds__(kBlock) void k_adj_coef_cell(DMesh m, DConfig c, BaseStage bs,
+__global__ __launch_bounds__(kMaxBlock) void k_adj_coef_cell(DMesh m, DConfig c, BaseStage bs,
                                                           double* __restrict__ c32)
 {
     // 8 lanes per cell: lanes 0-5 fill the edge slots, lane 6 the area
@@ -947,7 +949,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_coef_cell(DMesh m, DConfig c, Ba
 
 // edges: pv_edge, h_edge and the partial u adjoints
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q, T gs,
+__global__ __launch_bounds__(kMaxBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q, T gs,
                                                       const T* __restrict__ gh,
                                                       const T* __restrict__ gu,
                                                       T* __restrict__ pe_ad,
@@ -996,7 +998,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_edge1(DMesh m, int lb, AdjCoef q
 
 // cells: pv_cell, h and ke adjoints (+ divergence adjoint of del2/del4)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb, AdjCoef q, T gs,
+__global__ __launch_bounds__(kMaxBlock) void k_adj_cell1(DMesh m, DConfig c, int lb, AdjCoef q, T gs,
                                                       const T* __restrict__ gu,
                                                       const T* __restrict__ pe_ad,
                                                       const T* __restrict__ he_ad,
@@ -1050,7 +1052,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell1(DMesh m, DConfig c, int lb
 // vertices: h_vertex and circulation adjoints (+ vorticity adjoint of
 // del2/del4, :664-665, :679-680)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int lb, AdjCoef q, T gs,
+__global__ __launch_bounds__(kMaxBlock) void k_adj_vertex(DMesh m, DConfig c, int lb, AdjCoef q, T gs,
                                                        const T* __restrict__ pe_ad,
                                                        const T* __restrict__ pc_ad,
                                                        T* __restrict__ hv_ad,
@@ -1095,7 +1097,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_vertex(DMesh m, DConfig c, int l
 // tend_u_ad, the adjoints of the vorticity (times 1/areaTriangle: the
 // circulation adjoint) and of the divergence (times 1/areaCell) of nabla^2 u
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_del4_vertex(DMesh m, DConfig c, int lb, T gs,
+__global__ __launch_bounds__(kMaxBlock) void k_adj_del4_vertex(DMesh m, DConfig c, int lb, T gs,
                                                             const T* __restrict__ gu,
                                                             T* __restrict__ dsc_ad)
 {
@@ -1118,7 +1120,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_del4_vertex(DMesh m, DConfig c, 
 }
 
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_del4_cell(DMesh m, DConfig c, int lb, T gs,
+__global__ __launch_bounds__(kMaxBlock) void k_adj_del4_cell(DMesh m, DConfig c, int lb, T gs,
                                                           const T* __restrict__ gu,
                                                           T* __restrict__ dsd_ad)
 {
@@ -1142,7 +1144,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_del4_cell(DMesh m, DConfig c, in
 
 // del4 adjoint, second half (:618-652): nabla^2 u adjoint on edges
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_del4_edge(DMesh m, int lb,
+__global__ __launch_bounds__(kMaxBlock) void k_adj_del4_edge(DMesh m, int lb,
                                                           const T* __restrict__ dsd_ad,
                                                           const T* __restrict__ dsc_ad,
                                                           T* __restrict__ dsu_ad)
@@ -1184,7 +1186,7 @@ __device__ __forceinline__ void adj_update(int stage, bool first, T w, T sw, T p
 // (:350-355) on edges; h_vertex -> h (:284-290) on cells; then the RK4
 // adjoint bookkeeping (src/mpas_sw_module.f90:912-980)
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb, int stage,
+__global__ __launch_bounds__(kMaxBlock) void k_adj_final(DMesh m, DConfig c, int lb, int stage,
                                                       bool first, T w, T sw, AdjCoef q,
                                                       const T* __restrict__ pe_ad,
                                                       const T* __restrict__ ua_p,
@@ -1237,7 +1239,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_final(DMesh m, DConfig c, int lb
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_scale(long n, T* __restrict__ g,
+__global__ __launch_bounds__(kMaxBlock) void k_scale(long n, T* __restrict__ g,
                                                   const T* __restrict__ lam, T w)
 {
     GRID_LOOP(n) { g[t] = w * lam[t]; }
@@ -1248,7 +1250,7 @@ __global__ __launch_bounds__(kBlock) void k_scale(long n, T* __restrict__ g,
 // lat, cos lon, sin lon) per cell, evaluated on the host with the same libm
 // as the reference.
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_reconstruct(int nC, int lb, const int* __restrict__ nEoC,
+__global__ __launch_bounds__(kMaxBlock) void k_reconstruct(int nC, int lb, const int* __restrict__ nEoC,
                                                         const int* __restrict__ eoc,
                                                         const double* __restrict__ coeffs,
                                                         const double* __restrict__ trig,
@@ -1283,7 +1285,7 @@ __global__ __launch_bounds__(kBlock) void k_reconstruct(int nC, int lb, const in
 // zonal/meridional seeds are rotated back onto X, Y, Z per cell, then each
 // edge gathers coeffs . (X, Y, Z) from its two cells.
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_recon_adj_cell(DMesh m, int lb,
+__global__ __launch_bounds__(kMaxBlock) void k_recon_adj_cell(DMesh m, int lb,
                                                            const double* __restrict__ trig,
                                                            const T* __restrict__ seeds,
                                                            T* __restrict__ xyz)
@@ -1310,7 +1312,7 @@ __global__ __launch_bounds__(kBlock) void k_recon_adj_cell(DMesh m, int lb,
 }
 
 template <typename T, bool WU>
-__global__ __launch_bounds__(kBlock) void k_recon_adj_edge(DMesh m, int lb,
+__global__ __launch_bounds__(kMaxBlock) void k_recon_adj_edge(DMesh m, int lb,
                                                            const double* __restrict__ coeffs,
                                                            const T* __restrict__ xyz,
                                                            T* __restrict__ u_ad)
@@ -1379,20 +1381,20 @@ __global__ void k_dot_final(int nblk, int B, int accumulate, const double* __res
 }
 
 template <typename TD, typename TS>
-__global__ __launch_bounds__(kBlock) void k_convert(long n, TD* __restrict__ d,
+__global__ __launch_bounds__(kMaxBlock) void k_convert(long n, TD* __restrict__ d,
                                                     const TS* __restrict__ s)
 {
     GRID_LOOP(n) { d[t] = static_cast<TD>(s[t]); }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_fill(long n, T* __restrict__ d, T v)
+__global__ __launch_bounds__(kMaxBlock) void k_fill(long n, T* __restrict__ d, T v)
 {
     GRID_LOOP(n) { d[t] = v; }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gather_rows(long n, int lb, const int* __restrict__ perm,
+__global__ __launch_bounds__(kMaxBlock) void k_gather_rows(long n, int lb, const int* __restrict__ perm,
                                                         const T* __restrict__ s, T* __restrict__ d)
 {
     const int B = 1 << lb;
@@ -1400,7 +1402,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rows(long n, int lb, const in
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_scatter_rows(long n, int lb, const int* __restrict__ perm,
+__global__ __launch_bounds__(kMaxBlock) void k_scatter_rows(long n, int lb, const int* __restrict__ perm,
                                                          const T* __restrict__ s, T* __restrict__ d)
 {
     const int B = 1 << lb;
@@ -1408,7 +1410,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_rows(long n, int lb, const i
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_nonfinite(long n, int lb, const T* __restrict__ x,
+__global__ __launch_bounds__(kMaxBlock) void k_nonfinite(long n, int lb, const T* __restrict__ x,
                                                       int* __restrict__ flags)
 {
     const int B = 1 << lb;
@@ -1432,22 +1434,22 @@ RKCoeffs rk_coeffs(double dt)
     return k;
 }
 
-#define LAUNCH_WU(kernel, T, lb, grid, stream, ...)                                          \
+#define LAUNCH_WU(kernel, T, lb, grid, block, stream, ...)                                   \
     do {                                                                                     \
-        const int _g = (grid);                                                               \
+        const int _g = (grid), _b = (block);                                                 \
         if ((lb) >= 6)                                                                       \
-            hipLaunchKernelGGL((kernel<T, true>), dim3(_g), dim3(kBlock), 0, stream, __VA_ARGS__); \
+            hipLaunchKernelGGL((kernel<T, true>), dim3(_g), dim3(_b), 0, stream, __VA_ARGS__); \
         else                                                                                 \
-            hipLaunchKernelGGL((kernel<T, false>), dim3(_g), dim3(kBlock), 0, stream, __VA_ARGS__); \
+            hipLaunchKernelGGL((kernel<T, false>), dim3(_g), dim3(_b), 0, stream, __VA_ARGS__); \
     } while (0)
 
 template <typename T>
 void fwd_diag(const DMesh& m, const DConfig& c, int lb, const T* h, const T* u, T* pvv, T* hv,
               T* vort, T* ke, T* pc, T* he, T* pe, T* v, T* gpt, T* gpn, double* rec, hipStream_t s)
 {
-    LAUNCH_WU(k_fwd_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, lb, h, u, pvv, hv, vort);
-    LAUNCH_WU(k_fwd_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, lb, u, pvv, ke, pc);
-    LAUNCH_WU(k_fwd_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, c, lb, h, u, pvv, pc, he, pe, v,
+    LAUNCH_WU(k_fwd_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2, m.block), m.block, s, m, lb, h, u, pvv, hv, vort);
+    LAUNCH_WU(k_fwd_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2, m.block), m.block, s, m, lb, u, pvv, ke, pc);
+    LAUNCH_WU(k_fwd_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2, m.block), m.block, s, m, c, lb, h, u, pvv, pc, he, pe, v,
               gpt, gpn, rec);
 }
 
@@ -1457,7 +1459,7 @@ void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, con
                         const Dissip<T>& dp, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
                         const double* rec, hipStream_t s)
 {
-    LAUNCH_WU(k_fwd_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2), s, m, c, lb, stage, T(k.w[stage]),
+    LAUNCH_WU(k_fwd_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2, m.block), m.block, s, m, c, lb, stage, T(k.w[stage]),
               T(k.sw[stage]), h, u, he, pe, ke, dp, xh, xu, acch, accu, ph, pu, lb == 0 ? rec : nullptr);
 }
 
@@ -1467,15 +1469,15 @@ Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, co
 {
     Dissip<T> dp{nullptr, nullptr, nullptr, nullptr};
     if (!(c.visc2 > 0.0 || c.visc4 > 0.0)) return dp;
-    LAUNCH_WU(k_div, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, lb, u, ds.div);
+    LAUNCH_WU(k_div, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2, m.block), m.block, s, m, lb, u, ds.div);
     dp.div = ds.div;
     dp.vort = vort;
     if (c.visc4 > 0.0) {
-        LAUNCH_WU(k_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, lb, ds.div,
+        LAUNCH_WU(k_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2, m.block), m.block, s, m, lb, ds.div,
                   vort, ds.dsu);
-        LAUNCH_WU(k_del4_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, lb, ds.dsu,
+        LAUNCH_WU(k_del4_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2, m.block), m.block, s, m, lb, ds.dsu,
                   ds.dsv);
-        LAUNCH_WU(k_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, lb, ds.dsu,
+        LAUNCH_WU(k_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2, m.block), m.block, s, m, lb, ds.dsu,
                   ds.dsd);
         dp.dsd = ds.dsd;
         dp.dsv = ds.dsv;
@@ -1485,17 +1487,17 @@ Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, co
 
 void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_tl_coef, dim3(grid_items(16L * m.nE + 8L * m.nC, m.stripe_log2)), dim3(kBlock), 0, s, m,
+    hipLaunchKernelGGL(k_tl_coef, dim3(grid_items(16L * m.nE + 8L * m.nC, m.stripe_log2, m.block)), dim3(m.block), 0, s, m,
                        b, c.tl_fused, q.e48, q.c32);
 }
 
 void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adj_coef_edge, dim3(grid_items(16L * m.nE, m.stripe_log2)), dim3(kBlock), 0, s, m, c, b,
+    hipLaunchKernelGGL(k_adj_coef_edge, dim3(grid_items(16L * m.nE, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
                        q.e48, q.e8);
-    hipLaunchKernelGGL(k_adj_coef_vertex, dim3(grid_items(8L * m.nV, m.stripe_log2)), dim3(kBlock), 0, s, m, c, b,
+    hipLaunchKernelGGL(k_adj_coef_vertex, dim3(grid_items(8L * m.nV, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
                        q.v8);
-    hipLaunchKernelGGL(k_adj_coef_cell, dim3(grid_items(8L * m.nC, m.stripe_log2)), dim3(kBlock), 0, s, m, c, b,
+    hipLaunchKernelGGL(k_adj_coef_cell, dim3(grid_items(8L * m.nC, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
                        q.c32);
 }
 
@@ -1506,28 +1508,28 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
               T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s)
 {
     const bool visc = c.visc2 > 0.0 || c.visc4 > 0.0;
-    LAUNCH_WU(k_tl_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, lb, b.hv, b.vort,
+    LAUNCH_WU(k_tl_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2, m.block), m.block, s, m, lb, b.hv, b.vort,
               h_tl, u_tl, pvv_tl, visc ? vort_tl : nullptr);
-    LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, lb, q.c32, u_tl,
+    LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2, m.block), m.block, s, m, lb, q.c32, u_tl,
               pvv_tl, ke_tl, pc_tl);
-    LAUNCH_WU(k_tl_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, c, lb, b, h_tl,
+    LAUNCH_WU(k_tl_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2, m.block), m.block, s, m, c, lb, b, h_tl,
               u_tl, pvv_tl, pc_tl, pe_tl, he_tl);
     const Dissip<T> dp = dissip_fields<T>(m, c, lb, u_tl, vort_tl, ds, s);
     // the fused and reference-order forms are separate instances, so each
     // gets its own register allocation
-    const int g = grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2);
+    const int g = grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2, m.block);
 #define TL_TEND_ARGS s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), b, q.e48, q.c32, h_tl, u_tl, \
                      pe_tl, he_tl, ke_tl, dp, xh, xu, acch, accu, ph, pu
     if (lb >= 6) {
         if (c.tl_fused)
-            hipLaunchKernelGGL((k_tl_tend<T, true, true>), dim3(g), dim3(kBlock), 0, TL_TEND_ARGS);
+            hipLaunchKernelGGL((k_tl_tend<T, true, true>), dim3(g), dim3(m.block), 0, TL_TEND_ARGS);
         else
-            hipLaunchKernelGGL((k_tl_tend<T, true, false>), dim3(g), dim3(kBlock), 0, TL_TEND_ARGS);
+            hipLaunchKernelGGL((k_tl_tend<T, true, false>), dim3(g), dim3(m.block), 0, TL_TEND_ARGS);
     } else {
         if (c.tl_fused)
-            hipLaunchKernelGGL((k_tl_tend<T, false, true>), dim3(g), dim3(kBlock), 0, TL_TEND_ARGS);
+            hipLaunchKernelGGL((k_tl_tend<T, false, true>), dim3(g), dim3(m.block), 0, TL_TEND_ARGS);
         else
-            hipLaunchKernelGGL((k_tl_tend<T, false, false>), dim3(g), dim3(kBlock), 0, TL_TEND_ARGS);
+            hipLaunchKernelGGL((k_tl_tend<T, false, false>), dim3(g), dim3(m.block), 0, TL_TEND_ARGS);
     }
 #undef TL_TEND_ARGS
 }
@@ -1550,22 +1552,22 @@ void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, 
     // adjoint [C]; ds.dsu: nabla^2 u adjoint [E]; ds.div: divergence
     // adjoint / area [C]
     if (c.visc4 > 0.0) {
-        LAUNCH_WU(k_adj_del4_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, c, lb,
+        LAUNCH_WU(k_adj_del4_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2, m.block), m.block, s, m, c, lb,
                   gs, rgu, ds.dsv);
-        LAUNCH_WU(k_adj_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, c, lb, gs,
+        LAUNCH_WU(k_adj_del4_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2, m.block), m.block, s, m, c, lb, gs,
                   rgu, ds.dsd);
-        LAUNCH_WU(k_adj_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, lb,
+        LAUNCH_WU(k_adj_del4_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2, m.block), m.block, s, m, lb,
                   ds.dsd, ds.dsv, ds.dsu);
     }
-    LAUNCH_WU(k_adj_edge1, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, lb, q, gs, rgh,
+    LAUNCH_WU(k_adj_edge1, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2, m.block), m.block, s, m, lb, q, gs, rgh,
               rgu, pe_ad, he_ad, ua_p);
-    LAUNCH_WU(k_adj_cell1, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, c, lb, q, gs,
+    LAUNCH_WU(k_adj_cell1, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2, m.block), m.block, s, m, c, lb, q, gs,
               rgu, pe_ad, he_ad, pc_ad, ha_p, ke_ad, ds.dsu, ds.div);
-    LAUNCH_WU(k_adj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2), s, m, c, lb, q, gs,
+    LAUNCH_WU(k_adj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2, m.block), m.block, s, m, c, lb, q, gs,
               pe_ad, pc_ad, hv_ad, circ_ad, rgu, ds.dsu);
     const double w = stage > 0 ? k.w[stage - 1] : 0.0;
     const double sw = stage > 0 ? k.sw[stage - 1] : 0.0;
-    LAUNCH_WU(k_adj_final, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2), s, m, c, lb, stage, first, T(w),
+    LAUNCH_WU(k_adj_final, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2, m.block), m.block, s, m, c, lb, stage, first, T(w),
               T(sw), q, pe_ad, ua_p, ha_p, hv_ad, circ_ad, ke_ad, ds.div, lamh, lamu, acch, accu, gh,
               gu);
 }
@@ -1581,9 +1583,9 @@ template <typename T>
 void reconstruct_adj(const DMesh& m, int lb, const double* coeffs, const double* trig,
                      const T* seeds5, T* xyz, T* u_ad, hipStream_t s)
 {
-    LAUNCH_WU(k_recon_adj_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2), s, m, lb, trig,
+    LAUNCH_WU(k_recon_adj_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2, m.block), m.block, s, m, lb, trig,
               seeds5, xyz);
-    LAUNCH_WU(k_recon_adj_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2), s, m, lb, coeffs,
+    LAUNCH_WU(k_recon_adj_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2, m.block), m.block, s, m, lb, coeffs,
               xyz, u_ad);
 }
 
@@ -1591,7 +1593,7 @@ template <typename T>
 void reconstruct(int nC, int lb, const int* nEoC, const int* eoc, const double* coeffs,
                  const double* trig, const T* u, T* out5, hipStream_t s)
 {
-    LAUNCH_WU(k_reconstruct, T, lb, grid_for(static_cast<long>(nC) << lb), s, nC, lb, nEoC, eoc, coeffs, trig,
+    LAUNCH_WU(k_reconstruct, T, lb, grid_for(static_cast<long>(nC) << lb), kBlock, s, nC, lb, nEoC, eoc, coeffs, trig,
               u, out5);
 }
 
@@ -1687,7 +1689,7 @@ template void convert<float, float>(long, float*, const float*, hipStream_t);
 
 namespace msw {
 namespace {
-__global__ __launch_bounds__(kBlock) void k_diag_extra(DMesh m, const double* __restrict__ u,
+__global__ __launch_bounds__(kMaxBlock) void k_diag_extra(DMesh m, const double* __restrict__ u,
                                                        const double* __restrict__ vort,
                                                        double* __restrict__ circ,
                                                        double* __restrict__ div,

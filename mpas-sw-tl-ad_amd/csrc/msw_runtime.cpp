@@ -373,6 +373,7 @@ int build_mesh(msw_ctx* c, const msw_mesh_desc* d)
     DMesh& m = c->dm;
     m.nC = C; m.nE = E; m.nV = V;
     m.stripe_log2 = 2;
+    m.block = 256;
     int rc = 0;
 #define UP(dst, vec) if ((rc = c->upload(&(dst), vec))) return rc
     UP(m.nEoC, nEoC); UP(m.eoc, eoc); UP(m.sgnC, sgnC); UP(m.eocR, eocR); UP(m.sgnCR, sgnCR); UP(m.dvCR, dvCR);
@@ -846,11 +847,14 @@ int msw_set_option(msw_ctx* ctx, int option, int value)
     if (!ctx) return fail(MSW_EINVAL, "NULL ctx");
     if (option == MSW_OPT_XCD_STRIPE && (value < 0 || value > 6))
         return fail(MSW_EINVAL, "MSW_OPT_XCD_STRIPE takes log2 of the stripe length, 0..6");
-    if (option != MSW_OPT_TL_FUSED_FLUX && option != MSW_OPT_XCD_STRIPE)
+    if (option == MSW_OPT_BLOCK && value != 64 && value != 128 && value != 256 && value != 512)
+        return fail(MSW_EINVAL, "MSW_OPT_BLOCK takes 64, 128, 256 or 512 threads");
+    if (option != MSW_OPT_TL_FUSED_FLUX && option != MSW_OPT_XCD_STRIPE && option != MSW_OPT_BLOCK)
         return fail(MSW_EINVAL, "unknown option");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (option == MSW_OPT_TL_FUSED_FLUX) ctx->tl_fused = value != 0;
-    else ctx->dm.stripe_log2 = value;
+    else if (option == MSW_OPT_XCD_STRIPE) ctx->dm.stripe_log2 = value;
+    else ctx->dm.block = value;
     set_dcfg(ctx);
     for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     ctx->graphs.clear();

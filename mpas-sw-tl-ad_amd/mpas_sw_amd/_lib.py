@@ -88,6 +88,7 @@ EXPORTS = (
 
 OPT_TL_FUSED_FLUX = 1   # msw_option
 OPT_XCD_STRIPE = 2
+OPT_BLOCK = 3
 
 _lib = None
 

@@ -136,6 +136,7 @@ class SWModel:
         if tl_fused_flux:
             self.set_option(_lib.OPT_TL_FUSED_FLUX, 1)
         self.stripe_log2 = 2
+        self.block = 256
 
     # ------------------------------------------------------------------ utils
     def close(self):
@@ -170,13 +171,22 @@ class SWModel:
 
     def set_option(self, option: int, value: int):
         _lib.check(self.lib.msw_set_option(self._h, int(option), int(value)))
+        if option == _lib.OPT_XCD_STRIPE:
+            self.stripe_log2 = int(value)
+        elif option == _lib.OPT_BLOCK:
+            self.block = int(value)
 
-    def autotune(self, h, u, nsteps: int = 2, candidates=(0, 1, 2, 3, 4), seed: int = 0):
-        """Choose the launch stripe (MSW_OPT_XCD_STRIPE: 2^k consecutive blocks
-        per XCD turn) by timing a TLM sweep plus an adjoint sweep of `nsteps`
-        from the FP64 base state (h, u) with random member perturbations, each
-        candidate once after a warm-up.  Sets the fastest and returns
-        {k: seconds}.  (h, u) are not modified."""
+    def autotune(self, h, u, nsteps: int = 2, candidates=(0, 1, 2, 3, 4), blocks=(128, 256, 512),
+                 seed: int = 0):
+        """Choose the launch shape by timing a TLM sweep plus an adjoint sweep
+        of `nsteps` from the FP64 base state (h, u) with random member
+        perturbations, each candidate once after a warm-up: first the stripe
+        (MSW_OPT_XCD_STRIPE: 2^k consecutive blocks per XCD turn) at the
+        current block size, then the block size (MSW_OPT_BLOCK) at that
+        stripe.  Sets the fastest pair and returns {(k, block): seconds}.
+        (h, u) are not modified.  With 64 or more members a wavefront holds one
+        entity's members, so the members per wavefront are fixed by the batch
+        and the block size sets the entities per block."""
         g = torch.Generator(device="cpu").manual_seed(seed)
         shp_c, shp_e = self._shape("c"), self._shape("e")
         ht0 = torch.randn(shp_c, generator=g, dtype=torch.float64).to(self.device, self.dtype)
@@ -185,8 +195,10 @@ class SWModel:
         tu = torch.empty((nsteps, self.nE), dtype=torch.float64, device=self.device)
         bufs = [torch.empty_like(h), torch.empty_like(u), torch.empty_like(ht0), torch.empty_like(ut0)]
         times = {}
-        for k in candidates:
+
+        def run(k, blk):
             self.set_option(_lib.OPT_XCD_STRIPE, int(k))
+            self.set_option(_lib.OPT_BLOCK, int(blk))
             for rep in range(2):   # the first run captures the step graphs
                 for b, src in zip(bufs, (h, u, ht0, ut0)):
                     b.copy_(src)
@@ -198,10 +210,18 @@ class SWModel:
                 self.adj(th, tu, bufs[2], bufs[3], nsteps)
                 e1.record(self.stream)
                 e1.synchronize()
-            times[int(k)] = e0.elapsed_time(e1) / 1e3
+            times[(int(k), int(blk))] = e0.elapsed_time(e1) / 1e3
+
+        blk0 = self.block
+        for k in candidates:
+            run(k, blk0)
+        k_best = min((kb for kb in times if kb[1] == blk0), key=times.get)[0]
+        for blk in blocks:
+            if (k_best, blk) not in times:
+                run(k_best, blk)
         best = min(times, key=times.get)
-        self.set_option(_lib.OPT_XCD_STRIPE, best)
-        self.stripe_log2 = best
+        self.set_option(_lib.OPT_XCD_STRIPE, best[0])
+        self.set_option(_lib.OPT_BLOCK, best[1])
         return times
 
     def set_forcing(self, u_src):

@@ -464,18 +464,21 @@ def test_invariants_match_oracle_and_mass_is_conserved():
     assert abs(inv[0] - m0[0]) <= 1e-13 * abs(m0[0])
 
 
-def test_launch_stripe_does_not_change_results_and_autotune():
-    """The block-to-XCD stripe only reorders work: TLM and adjoint results are
-    bitwise the same for every stripe; the autotuner picks one of them."""
-    print("stripes L3 tc6 B=4", flush=True)
+@pytest.mark.parametrize("B", [4, 64])
+def test_launch_stripe_does_not_change_results_and_autotune(B):
+    """The block-to-XCD stripe and the block size only reorder work: TLM and
+    adjoint results are bitwise the same for every stripe and block size;
+    the autotuner picks one of the pairs it timed."""
+    print(f"stripes and block sizes L3 tc6 B={B}", flush=True)
     from mpas_sw_amd import _lib
-    tc, dt, B, n = 6, 600.0, 4, 3
+    tc, dt, n = 6, 600.0, 3
     mesh, phys = case(LEVEL, tc)
     dh, du = perturbations(mesh, B)
     m = model(LEVEL, tc, dt, batch=B, graphs=True, tl_fused_flux=True)
     outs = []
-    for k in (0, 2, 5):
+    for k, blk in ((0, 256), (2, 256), (5, 256), (2, 64), (1, 128), (3, 512)):
         m.set_option(_lib.OPT_XCD_STRIPE, k)
+        m.set_option(_lib.OPT_BLOCK, blk)
         h = m.to_internal(phys["h"], "c", member=False)
         u = m.to_internal(phys["u"], "e", member=False)
         ht, ut = m.to_internal(dh, "c"), m.to_internal(du, "e")
@@ -491,8 +494,14 @@ def test_launch_stripe_does_not_change_results_and_autotune():
             assert torch.equal(a, b)
     h = m.to_internal(phys["h"], "c", member=False)
     u = m.to_internal(phys["u"], "e", member=False)
-    times = m.autotune(h, u, nsteps=1, candidates=(0, 2))
-    assert set(times) == {0, 2} and m.stripe_log2 in times
+    m.set_option(_lib.OPT_BLOCK, 256)
+    times = m.autotune(h, u, nsteps=1, candidates=(0, 2), blocks=(128, 512))
+    k = min(((0, 256), (2, 256)), key=times.get)[0]
+    assert set(times) == {(0, 256), (2, 256), (k, 128), (k, 512)}
+    assert (m.stripe_log2, m.block) == min(times, key=times.get)
+    from mpas_sw_amd._lib import MSWError
+    with pytest.raises(MSWError):
+        m.set_option(_lib.OPT_BLOCK, 96)
 
 
 @pytest.mark.parametrize("B", [64, 256])
