@@ -82,8 +82,10 @@ def run_window(m, bufs, h0, u0, ht0, ut0, nsteps):
     return wall, ev[0].elapsed_time(ev[1]) / 1e3, ev[1].elapsed_time(ev[2]) / 1e3
 
 
-def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True, fused=True, tune=True):
+def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True, fused=True, tune=True,
+             launch=None):
     import torch
+    from mpas_sw_amd import _lib
     from mpas_sw_amd.core import SWModel
     m = SWModel(mesh, phys, DT, test_case=TC, batch=batch, precision=precision, graphs=graphs,
                 tl_fused_flux=fused)
@@ -92,7 +94,11 @@ def gpu_rate(mesh, phys, dh, du, batch, precision, steps, warmup, graphs=True, f
     sel = slice(0, batch)
     ht0 = m.to_internal(dh[:, sel] if batch > 1 else dh[:, 0], "c")
     ut0 = m.to_internal(du[:, sel] if batch > 1 else du[:, 0], "e")
-    if tune:
+    if launch is not None:        # a fixed (stripe, block) pair instead of the autotuner
+        m.set_option(_lib.OPT_XCD_STRIPE, launch[0])
+        m.set_option(_lib.OPT_BLOCK, launch[1])
+        tune = {(m.stripe_log2, m.block): 0.0}
+    elif tune:
         tune = m.autotune(h0, u0)     # launch stripe and block size, chosen by timing (not timed here)
         log(f"launch autotune B={batch} {precision} (stripe, block): {tune}")
     else:
@@ -313,6 +319,8 @@ def main():
                     help="skip the TC6 run with f = 2 Omega sin(lat)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch stripe (no autotuning runs before the timed window)")
+    ap.add_argument("--launch", nargs=2, type=int, metavar=("STRIPE_LOG2", "BLOCK"),
+                    help="fixed XCD stripe and block size for the batched runs (no autotuning)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 1 GiB device copy (reference for HBM counters)")
     ap.add_argument("--packed", action="store_true",
@@ -359,7 +367,8 @@ def main():
     if args.calibrate:
         calibration_copy()
     g = not args.no_graphs
-    res = gpu_rate(mesh, phys, dh, du, BATCH, "fp64", args.steps, args.warmup, g, tune=not args.no_tune)
+    res = gpu_rate(mesh, phys, dh, du, BATCH, "fp64", args.steps, args.warmup, g, tune=not args.no_tune,
+                   launch=args.launch)
     log(f"batched fp64: {res}")
     exact = None
     if not args.no_plain:
