@@ -159,6 +159,15 @@ int msw_adj_steps(msw_ctx *ctx, const double *traj_h, const double *traj_u, void
  * one FP64 state; writes the 13 diagnostic fields into `out` (double,
  * packed in the order of msw_diag_names, each of length C, E or V). */
 int msw_diagnostics(msw_ctx *ctx, const double *h, const double *u, double *out);
+/* Replaces: sw_compute_solve_diagnostics_adj (src/mpas_sw_dyncore_adj.f90:1-453)
+ * as sw_rk4_adj applies it to the seeds left in the module's diagnostic
+ * adjoint arrays at the end state of its step (src/mpas_sw_module.f90:894-905):
+ * h_ad += and u_ad += the adjoint of the diagnostics of the FP64 state (h,u),
+ * applied to `seeds`: 13 batched arrays of `prec` in the order and sizes of
+ * msw_diag_names, each [n][B], packed back to back.  Seeds are not modified
+ * (the reference zeroes them after use). */
+int msw_diagnostics_adj(msw_ctx *ctx, const double *h, const double *u, const void *seeds, void *h_ad,
+                        void *u_ad);
 /* comma-separated names and entity kinds ('c','e','v') of msw_diagnostics */
 const char *msw_diag_names(void);
 const char *msw_diag_kinds(void);

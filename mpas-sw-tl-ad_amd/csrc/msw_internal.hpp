@@ -190,6 +190,22 @@ void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, 
 template <typename T>
 void adj_seed(int n, T* g, const T* lam, double w3, hipStream_t s);
 
+// FP64 fields of the state a diagnostic adjoint is linearised about (B = 1)
+struct DiagBase {
+    const double *u, *v, *gpn, *gpt;   // [E]
+    const double *hv, *vort;           // [V]
+};
+// member seeds of the 13 diagnostic adjoints, [n][B] each
+template <typename T>
+struct DiagSeeds {
+    const T *v, *he, *hv, *circ, *vort, *div, *ke, *pe, *pvv, *pc, *vc, *gpn, *gpt;
+};
+// h_ad += and u_ad += the adjoint of the diagnostics applied to the seeds
+// (src/mpas_sw_dyncore_adj.f90:1-453); e1..e3 [E], c1 [C], v1, v2 [V]: member scratch
+template <typename T>
+void diag_adj(const DMesh& m, const DConfig& c, int lb, const DiagBase& bd, const DiagSeeds<T>& sd,
+              T* e1, T* e2, T* e3, T* c1, T* v1, T* v2, T* h_ad, T* u_ad, hipStream_t s);
+
 // Reconstruction of the cell-centre velocity (src/mpas_sw_operators.f90:283-370)
 template <typename T>
 void reconstruct(int nC, int lb, const int* nEoC, const int* eoc, const double* coeffs,

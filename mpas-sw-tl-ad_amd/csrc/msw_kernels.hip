@@ -1246,6 +1246,128 @@ __global__ __launch_bounds__(kMaxBlock) void k_scale(long n, T* __restrict__ g,
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint of the diagnostics for seeds placed in the diagnostic adjoint
+// arrays (src/mpas_sw_dyncore_adj.f90:1-453, which sw_rk4_adj applies at the
+// end state of its step, src/mpas_sw_module.f90:894-905), linearised about
+// the FP64 state whose diagnostics are in `bd` (B = 1).  The reverse sweep is
+// written as four gather passes; h_ad and u_ad are accumulated into.
+
+// pv_edge upwinding (:193-200, :235-242): gradPVn, gradPVt and v adjoints
+template <typename T, bool WU>
+__global__ __launch_bounds__(kMaxBlock) void k_dadj_edge(DMesh m, DConfig c, int lb, DiagBase bd,
+                                                         DiagSeeds<T> s, T* __restrict__ gpn2,
+                                                         T* __restrict__ gpt2, T* __restrict__ v2)
+{
+    const int B = 1 << lb;
+    ENTITY_ITEM(m.nE)
+    {
+        const int e = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        const T pe = ld(s.pe, e, B, b);
+        const T apvm = T(c.apvm), dt = T(c.dt);
+        rowp(gpn2, e, B)[b] = ld(s.gpn, e, B, b) - apvm * T(bd.u[e]) * dt * pe;
+        rowp(gpt2, e, B)[b] = ld(s.gpt, e, B, b) - apvm * T(bd.v[e]) * dt * pe;
+        rowp(v2, e, B)[b] = ld(s.v, e, B, b) - apvm * dt * T(bd.gpt[e]) * pe;
+    }
+}
+
+// gradPVn -> pv_cell (:202-213)
+template <typename T, bool WU>
+__global__ __launch_bounds__(kMaxBlock) void k_dadj_cell(DMesh m, int lb, DiagSeeds<T> s,
+                                                         const T* __restrict__ gpn2, T* __restrict__ pc2)
+{
+    const int B = 1 << lb;
+    ENTITY_ITEM(m.nC)
+    {
+        const int ic = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        T a = ld(s.pc, ic, B, b);
+#pragma unroll
+        for (int k = 0; k < kC6; ++k) {
+            const int e = m.eoc[kC6 * ic + k];
+            a = a - T(m.sgnC[kC6 * ic + k]) * ld(gpn2, e, B, b) / T(m.dc[e]);
+        }
+        rowp(pc2, ic, B)[b] = a;
+    }
+}
+
+// pv_cell and vorticity_cell -> vertices (:215-229), pv_edge and gradPVt ->
+// pv_vertex (:244-262), pv_vertex -> h_vertex and vorticity (:268-292),
+// vorticity -> circulation (:343-349); h_vertex leaves scaled by 1/areaTriangle
+template <typename T, bool WU>
+__global__ __launch_bounds__(kMaxBlock) void k_dadj_vertex(DMesh m, int lb, DiagBase bd, DiagSeeds<T> s,
+                                                           const T* __restrict__ pc2,
+                                                           const T* __restrict__ gpt2,
+                                                           T* __restrict__ hv2, T* __restrict__ circ2)
+{
+    const int B = 1 << lb;
+    ENTITY_ITEM(m.nV)
+    {
+        const int iv = entity_of<WU>(t, lb);
+        const int b = static_cast<int>(t & (B - 1));
+        T va = ld(s.vort, iv, B, b), pa = ld(s.pvv, iv, B, b);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int ic = m.cov[3 * iv + i];
+            const T w = T(m.kite[3 * iv + i]) / T(m.areaC[ic]);
+            va = va + w * ld(s.vc, ic, B, b);
+            pa = pa + w * ld(pc2, ic, B, b);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int e = m.eovR[3 * iv + k];
+            pa = pa + T(0.5) * ld(s.pe, e, B, b) + T(m.sgnV[3 * iv + k]) * ld(gpt2, e, B, b) / T(m.dv[e]);
+        }
+        const T hvb = T(bd.hv[iv]);
+        const T ha = ld(s.hv, iv, B, b) - (T(m.fV[iv]) + T(bd.vort[iv])) / (hvb * hvb) * pa;
+        va = va + pa / hvb;
+        rowp(hv2, iv, B)[b] = ha / T(m.areaT[iv]);
+        rowp(circ2, iv, B)[b] = ld(s.circ, iv, B, b) + va / T(m.areaT[iv]);
+    }
+}
+
+// u_ad: upwinding, v through the transposed edgesOnEdge (:296-304), ke
+// (:306-318), divergence (:321-341), circulation (:350-355); h_ad: h_vertex
+// (:284-290) and h_edge (:358-369)
+template <typename T, bool WU>
+__global__ __launch_bounds__(kMaxBlock) void k_dadj_final(DMesh m, DConfig c, int lb, DiagBase bd,
+                                                          DiagSeeds<T> s, const T* __restrict__ v2,
+                                                          const T* __restrict__ hv2,
+                                                          const T* __restrict__ circ2, T* h_ad, T* u_ad)
+{
+    const int B = 1 << lb;
+    EDGE_CELL_ITEM(is_edge, t)
+    {
+        const int b = static_cast<int>(t & (B - 1));
+        if (is_edge) {
+            const int e = entity_of<WU>(t, lb);
+            T a = -T(c.apvm) * T(c.dt) * T(bd.gpn[e]) * ld(s.pe, e, B, b);
+#pragma unroll
+            for (int j = 0; j < kEoE; ++j) a = a + T(m.woeT[kE2 * e + j]) * ld(v2, m.eoe[kE2 * e + j], B, b);
+            const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+            const int w1 = m.voe[2 * e], w2 = m.voe[2 * e + 1];
+            const T dc = T(m.dc[e]), dv = T(m.dv[e]);
+            const T r1 = T(1) / T(m.areaC[c1]), r2 = T(1) / T(m.areaC[c2]);
+            a = a + T(0.5) * dc * dv * T(bd.u[e]) * (ld(s.ke, c1, B, b) * r1 + ld(s.ke, c2, B, b) * r2);
+            a = a + dv * (ld(s.div, c1, B, b) * r1 - ld(s.div, c2, B, b) * r2);
+            a = a + dc * (ld(circ2, w2, B, b) - ld(circ2, w1, B, b));
+            T* p = &rowp(u_ad, e, B)[b];
+            *p = *p + a;
+        } else {
+            const int ic = entity_of<WU>(t, lb);
+            T a = T(0);
+#pragma unroll
+            for (int k = 0; k < kC6; ++k) a = a + T(m.kiteCR[kC6 * ic + k]) * ld(hv2, m.vocR[kC6 * ic + k], B, b);
+#pragma unroll
+            for (int k = 0; k < kC6; ++k)
+                a = a + T(0.5) * T(fabs(m.sgnC[kC6 * ic + k])) * ld(s.he, m.eoc[kC6 * ic + k], B, b);
+            T* p = &rowp(h_ad, ic, B)[b];
+            *p = *p + a;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Reconstruction (src/mpas_sw_operators.f90:283-370); trig = (cos lat, sin
 // lat, cos lon, sin lon) per cell, evaluated on the host with the same libm
 // as the reference.
@@ -1573,6 +1695,20 @@ void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, 
 }
 
 template <typename T>
+void diag_adj(const DMesh& m, const DConfig& c, int lb, const DiagBase& bd, const DiagSeeds<T>& sd,
+              T* e1, T* e2, T* e3, T* c1, T* v1, T* v2, T* h_ad, T* u_ad, hipStream_t s)
+{
+    LAUNCH_WU(k_dadj_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2, m.block), m.block, s,
+              m, c, lb, bd, sd, e1, e2, e3);
+    LAUNCH_WU(k_dadj_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2, m.block), m.block, s,
+              m, lb, sd, e1, c1);
+    LAUNCH_WU(k_dadj_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2, m.block), m.block,
+              s, m, lb, bd, sd, c1, e2, v1, v2);
+    LAUNCH_WU(k_dadj_final, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2, m.block), m.block, s, m, c, lb,
+              bd, sd, e3, v1, v2, h_ad, u_ad);
+}
+
+template <typename T>
 void adj_seed(int n, T* g, const T* lam, double w3, hipStream_t s)
 {
     hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, static_cast<long>(n), g,
@@ -1667,6 +1803,8 @@ void nonfinite_flags(long n, int lb, const T* x, int* flags, hipStream_t s)
                                const DissipScratch<T>&, const AdjCoef&, T*, T*, T*, T*,             \
                                hipStream_t);                                                        \
     template void adj_seed<T>(int, T*, const T*, double, hipStream_t);                              \
+    template void diag_adj<T>(const DMesh&, const DConfig&, int, const DiagBase&, const DiagSeeds<T>&, \
+                              T*, T*, T*, T*, T*, T*, T*, T*, hipStream_t);                         \
     template void reconstruct_adj<T>(const DMesh&, int, const double*, const double*, const T*, T*,  \
                                      T*, hipStream_t);                                              \
     template void reconstruct<T>(int, int, const int*, const int*, const double*, const double*,    \

@@ -78,6 +78,7 @@ struct msw_ctx {
     // member scratch ([entity][B] of prec)
     void* macc = nullptr;             // [C+E]
     void* mpA = nullptr;              // [C+E]
+    double* dgb = nullptr;            // 13 FP64 diagnostics for msw_diagnostics_adj
     void* mpB = nullptr;              // [C+E]
     void* mv1 = nullptr;              // [V]
     void* mv2 = nullptr;              // [V]
@@ -766,6 +767,22 @@ int run_steps(msw_ctx* c, int kind, const void* p0, const void* p1, const void* 
 // ---------------------------------------------------------------------------
 // C ABI
 
+template <typename T>
+DiagSeeds<T> diag_seeds(const msw_ctx* c, const void* p)
+{
+    const size_t C = c->nC, E = c->nE, V = c->nV, B = c->B;
+    const T* q = static_cast<const T*>(p);
+    DiagSeeds<T> s;
+    const T** f[13] = {&s.v, &s.he, &s.hv, &s.circ, &s.vort, &s.div, &s.ke,
+                       &s.pe, &s.pvv, &s.pc, &s.vc, &s.gpn, &s.gpt};
+    const size_t n[13] = {E, E, V, V, V, C, C, E, V, C, C, E, E};
+    for (int k = 0; k < 13; ++k) {
+        *f[k] = q;
+        q += n[k] * B;
+    }
+    return s;
+}
+
 extern "C" {
 
 int msw_abi_version(void) { return MSW_ABI_VERSION; }
@@ -1020,6 +1037,41 @@ int msw_diagnostics(msw_ctx* ctx, const double* h, const double* u, double* out)
     double* gpt = o; o += E;
     fwd_diag<double>(ctx->dm, ctx->dcfg, 0, h, u, pvv, hv, vort, ke, pc, he, pe, v, gpt, gpn, nullptr, ctx->stream);
     diag_extra(ctx->dm, u, vort, circ, div, vc, ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return MSW_OK;
+}
+
+int msw_diagnostics_adj(msw_ctx* ctx, const double* h, const double* u, const void* seeds, void* h_ad,
+                        void* u_ad)
+{
+    if (!ctx || !h || !u || !seeds || !h_ad || !u_ad) return fail(MSW_EINVAL, "bad argument");
+    int rc = check_cfg(ctx);
+    if (rc) return rc;
+    const size_t C = ctx->nC, E = ctx->nE, V = ctx->nV;
+    if (!ctx->dgb && (rc = ctx->alloc(&ctx->dgb, 5 * E + 4 * V + 4 * C))) return rc;
+    // the 13 diagnostics of the state, packed as msw_diagnostics writes them
+    if ((rc = msw_diagnostics(ctx, h, u, ctx->dgb))) return rc;
+    const double* d = ctx->dgb;
+    DiagBase bd;
+    bd.u = u;
+    bd.v = d;
+    bd.hv = d + 2 * E;
+    bd.vort = d + 2 * E + 2 * V;
+    bd.gpn = d + 3 * E + 4 * V + 4 * C;
+    bd.gpt = bd.gpn + E;
+    const DMesh& m = ctx->dm;
+    if (ctx->prec == MSW_FP64)
+        diag_adj<double>(m, ctx->dcfg, ctx->lb, bd, diag_seeds<double>(ctx, seeds), static_cast<double*>(ctx->me1),
+                         static_cast<double*>(ctx->me2), static_cast<double*>(ctx->me3),
+                         static_cast<double*>(ctx->mc1), static_cast<double*>(ctx->mv1),
+                         static_cast<double*>(ctx->mv2), static_cast<double*>(h_ad), static_cast<double*>(u_ad),
+                         ctx->stream);
+    else
+        diag_adj<float>(m, ctx->dcfg, ctx->lb, bd, diag_seeds<float>(ctx, seeds), static_cast<float*>(ctx->me1),
+                        static_cast<float*>(ctx->me2), static_cast<float*>(ctx->me3),
+                        static_cast<float*>(ctx->mc1), static_cast<float*>(ctx->mv1),
+                        static_cast<float*>(ctx->mv2), static_cast<float*>(h_ad), static_cast<float*>(u_ad),
+                        ctx->stream);
     HIP_TRY(hipGetLastError());
     return MSW_OK;
 }

@@ -423,9 +423,6 @@ class _Module:
         A = self._arrays
         if "u_ad" not in A:
             raise RuntimeError("var_allocation_adj() has not been called")
-        for n in _DIAG:
-            if np.any(A[n + "_ad"] != 0):
-                raise NotImplementedError(f"a non-zero seed in {n}_ad is not supported on the GPU path")
         h = self._up(m, A["h"][0, 0], "c")
         u = self._up(m, A["u"][0, 0], "e")
         ha = self._up(m, A["h_ad"][0, 0], "c")
@@ -440,6 +437,23 @@ class _Module:
             ua = ua.to(torch.float64)
             for n in _RECON:
                 A[n + "_ad"][...] = 0.0
+        seeded = [n for n in _DIAG if n != "vh" and np.any(A[n + "_ad"] != 0)]
+        if seeded:
+            # sw_compute_solve_diagnostics_adj at the step's end state consumes
+            # the seeds of the diagnostic adjoints (src/mpas_sw_module.f90:894-905;
+            # vh_ad takes no part, src/mpas_sw_dyncore_adj.f90:294)
+            he, ue = h.clone(), u.clone()
+            m.base(he, ue, 1)
+            kinds = dict(zip(m.lib.msw_diag_names().decode().split(","),
+                             m.lib.msw_diag_kinds().decode().split(",")))
+            seeds = {}
+            for n in seeded:
+                g = _GPU_DIAG_NAME.get(n, n)
+                seeds[g] = m.to_internal(np.asarray(A[n + "_ad"][0], np.float64), kinds[g])
+            m.diagnostics_adj(he, ue, seeds, ha, ua)
+            for n in _DIAG:
+                if n != "vh":
+                    A[n + "_ad"][...] = 0.0
         he, ue = m.adj_step(h, u, ha, ua, want_end=True)
         A["h_ad"][0, 0, :] = m.to_reference(ha, "c")
         A["u_ad"][0, 0, :] = m.to_reference(ua, "e")

@@ -82,6 +82,7 @@ EXPORTS = (
     "msw_create", "msw_destroy", "msw_set_config", "msw_set_forcing", "msw_set_option",
     "msw_set_graphs", "msw_fwd_steps",
     "msw_base_steps", "msw_tlm_steps", "msw_adj_step", "msw_adj_steps", "msw_diagnostics",
+    "msw_diagnostics_adj",
     "msw_diag_names", "msw_diag_kinds", "msw_reconstruct", "msw_reconstruct_adj", "msw_invariants", "msw_dot", "msw_nonfinite",
     "msw_permute", "msw_device_bytes", "msw_abi_version", "msw_last_error",
 )
@@ -125,6 +126,7 @@ def load():
         "msw_adj_step": ([vp, vp, vp, vp, vp, vp, vp], i32),
         "msw_adj_steps": ([vp, vp, vp, vp, vp, i32], i32),
         "msw_diagnostics": ([vp, vp, vp, vp], i32),
+        "msw_diagnostics_adj": ([vp, vp, vp, vp, vp, vp], i32),
         "msw_diag_names": ([], C.c_char_p),
         "msw_diag_kinds": ([], C.c_char_p),
         "msw_reconstruct": ([vp, vp, vp], i32),

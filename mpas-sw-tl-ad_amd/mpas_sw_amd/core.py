@@ -339,6 +339,29 @@ class SWModel:
             off += s
         return res
 
+    def diagnostics_adj(self, h, u, seeds, h_ad, u_ad):
+        """h_ad += and u_ad += the adjoint of the diagnostics of the FP64 state
+        (h, u) applied to `seeds` (msw_diagnostics_adj): a dict from the names
+        of msw_diag_names to member tensors in internal order; missing names
+        are zero.  h_ad, u_ad: member tensors, updated in place."""
+        self._check(h, "c", False, "h"); self._check(u, "e", False, "u")
+        self._check(h_ad, "c", name="h_ad"); self._check(u_ad, "e", name="u_ad")
+        names = self.lib.msw_diag_names().decode().split(",")
+        kinds = self.lib.msw_diag_kinds().decode().split(",")
+        unknown = set(seeds) - set(names)
+        if unknown:
+            raise ValueError(f"unknown diagnostic seeds: {sorted(unknown)}")
+        parts = []
+        for nm, k in zip(names, kinds):
+            t = seeds.get(nm)
+            if t is None:
+                parts.append(torch.zeros(self._shape(k), dtype=self.dtype, device=self.device).reshape(-1))
+            else:
+                self._check(t, k, name=nm)
+                parts.append(t.reshape(-1))
+        packed = torch.cat(parts)
+        _lib.check(self.lib.msw_diagnostics_adj(self._h, _ptr(h), _ptr(u), _ptr(packed), _ptr(h_ad), _ptr(u_ad)))
+
     def reconstruct(self, u):
         """[5, C(, B)] = X, Y, Z, zonal, meridional of the batch `u`."""
         self._check(u, "e", name="u")

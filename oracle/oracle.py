@@ -182,6 +182,15 @@ class Oracle:
         a = [np.ascontiguousarray(x, np.float64) for x in (h, h_tl, u, u_tl)]
         self._lib.mswo_tend_tlm(self._h, *[_ptr(x, _d) for x in a])
 
+    def diag_adj(self, h, u):
+        """sw_compute_solve_diagnostics_adj at (h,u) applied to the seeds in
+        the diagnostic adjoint arrays (self["<name>_ad"]), which it zeroes:
+        returns (h_ad, u_ad)."""
+        h = np.ascontiguousarray(h, np.float64); u = np.ascontiguousarray(u, np.float64)
+        h_ad = np.zeros_like(h); u_ad = np.zeros_like(u)
+        self._lib.mswo_diag_adj(self._h, _ptr(h, _d), _ptr(h_ad, _d), _ptr(u, _d), _ptr(u_ad, _d))
+        return h_ad, u_ad
+
     def tend_full_adj(self, h, u, g_h, g_u):
         """Adjoint of the full tendency operator at (h,u): returns
         (h_ad, u_ad) = F'(h,u)^T (g_h, g_u), via diag, tend_adj, diag_adj."""

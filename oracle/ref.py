@@ -27,9 +27,15 @@ def available() -> bool:
     return os.path.isfile(REF_EXE) and os.access(REF_EXE, os.X_OK)
 
 
+# order in which ref_main reads the diagnostic adjoint seeds
+DIAG_SEED_ORDER = ("v", "h_edge", "h_vertex", "circulation", "vorticity", "divergence", "ke",
+                   "pv_edge", "pv_vertex", "pv_cell", "vorticity_cell", "gradPVn", "gradPVt")
+DIAG_SEED_KINDS = "eevvvccevccee"   # entity of each, in that order
+
+
 def write_input(path, mesh, test_case, dt, apvm=0.5, visc2=0.0, visc4=0.0, adv_order=2,
                 monotonic=False, wind_stress=False, bottom_drag=False,
-                state=None, tl=None, ad=None, u_src=None, recon_seeds=None):
+                state=None, tl=None, ad=None, u_src=None, recon_seeds=None, diag_seeds=None):
     hdr = np.zeros(16, np.int32)
     hdr[0] = MAGIC
     hdr[1] = 1
@@ -46,6 +52,7 @@ def write_input(path, mesh, test_case, dt, apvm=0.5, visc2=0.0, visc4=0.0, adv_o
     cfg[:4] = [dt, apvm, visc2, visc4]
     cfg[4] = 1.0 if u_src is not None else 0.0
     cfg[5] = 1.0 if recon_seeds is not None else 0.0
+    cfg[6] = 1.0 if diag_seeds is not None else 0.0
     f8 = lambda x: np.asarray(x, np.float64).ravel(order="C")
     i4 = lambda x: np.asarray(x, np.int32).ravel(order="C")
     with open(path, "wb") as f:
@@ -76,6 +83,13 @@ def write_input(path, mesh, test_case, dt, apvm=0.5, visc2=0.0, visc4=0.0, adv_o
             f.write(f8(u_src).tobytes())
         if recon_seeds is not None:   # (5, C): X, Y, Z, zonal, meridional adjoint seeds
             f.write(f8(np.asarray(recon_seeds, np.float64).reshape(5, -1)).tobytes())
+        if diag_seeds is not None:    # dict name -> array, the 13 diagnostics of DIAG_SEED_ORDER
+            size = {"c": mesh.nCells, "e": mesh.nEdges, "v": mesh.nVertices}
+            for name, kind in zip(DIAG_SEED_ORDER, DIAG_SEED_KINDS):
+                a = f8(diag_seeds[name])
+                if a.size != size[kind]:   # the reader takes raw streams: a wrong size would misalign
+                    raise ValueError(f"diagnostic seed {name}: {a.size} values, expected {size[kind]}")
+                f.write(a.tobytes())
 
 
 def read_output(path):

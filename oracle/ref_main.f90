@@ -20,6 +20,7 @@ program msw_ref
   integer :: has_state, has_tl, has_ad
   real(kind=RKIND), allocatable :: st_h(:), st_u(:), tl_h(:), tl_u(:), ad_h(:), ad_u(:)
   real(kind=RKIND), allocatable :: traj_h(:,:), traj_u(:,:), usrc_in(:), rseed(:,:)
+  real(kind=RKIND), allocatable :: ds_e(:,:), ds_v(:,:), ds_c(:,:)
   integer(8) :: t0, t1, rate
   integer, parameter :: uin = 21, uout = 22
 
@@ -88,6 +89,15 @@ program msw_ref
      allocate(rseed(nCells, 5))
      read(uin) rseed
   end if
+  ! cfg(7) /= 0: seeds of the diagnostic adjoints follow, in the order v,
+  ! h_edge, h_vertex, circulation, vorticity, divergence, ke, pv_edge,
+  ! pv_vertex, pv_cell, vorticity_cell, gradPVn, gradPVt (module arrays,
+  ! src/mpas_sw_module.f90:450-463), consumed by the first sw_rk4_adj
+  if (cfg(7) /= 0.0) then
+     allocate(ds_e(nEdges, 5), ds_v(nVertices, 4), ds_c(nCells, 4))
+     read(uin) ds_e(:,1), ds_e(:,2), ds_v(:,1), ds_v(:,2), ds_v(:,3), ds_c(:,1), ds_c(:,2), &
+          ds_e(:,3), ds_v(:,4), ds_c(:,3), ds_c(:,4), ds_e(:,4), ds_e(:,5)
+  end if
   close(uin)
 
   ! ---- initial conditions, as initial_conditions does
@@ -117,6 +127,21 @@ program msw_ref
      uReconstructZ_ad(1,:) = rseed(:,3)
      uReconstructZonal_ad(1,:) = rseed(:,4)
      uReconstructMeridional_ad(1,:) = rseed(:,5)
+  end if
+  if (allocated(ds_e)) then
+     v_ad(1,:) = ds_e(:,1)
+     h_edge_ad(1,:) = ds_e(:,2)
+     h_vertex_ad(1,:) = ds_v(:,1)
+     circulation_ad(1,:) = ds_v(:,2)
+     vorticity_ad(1,:) = ds_v(:,3)
+     divergence_ad(1,:) = ds_c(:,1)
+     ke_ad(1,:) = ds_c(:,2)
+     pv_edge_ad(1,:) = ds_e(:,3)
+     pv_vertex_ad(1,:) = ds_v(:,4)
+     pv_cell_ad(1,:) = ds_c(:,3)
+     vorticity_cell_ad(1,:) = ds_c(:,4)
+     gradPVn_ad(1,:) = ds_e(:,4)
+     gradPVt_ad(1,:) = ds_e(:,5)
   end if
 
   open(uout, file=trim(fout), access='stream', form='unformatted', status='replace')

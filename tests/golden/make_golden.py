@@ -145,6 +145,23 @@ def level7_ic_digest():
 IC_FIELDS = ("h", "u", "h_s", "fEdge", "fVertex", "lonCell")
 
 
+def diag_seed_case():
+    """Level 3, TC5, dt 900 s, 3 adjoint steps seeded with end-state
+    adjoints (h, u) and with seeds in all 13 diagnostic adjoint arrays, which
+    the first sw_rk4_adj consumes at the end state of its step
+    (src/mpas_sw_module.f90:894-905)."""
+    m3 = get_mesh(3)
+    tc, dt, n = 5, 900.0, 3
+    dh, du = perturb(m3, SEED + 9)
+    rng = np.random.default_rng(SEED + 10)
+    size = {"c": m3.nCells, "e": m3.nEdges, "v": m3.nVertices}
+    kinds = dict(zip(ref.DIAG_SEED_ORDER, ref.DIAG_SEED_KINDS))
+    seeds = {k: rng.standard_normal(size[kinds[k]]) for k in ref.DIAG_SEED_ORDER}
+    a = ref.run(m3, "adj", nsteps=n, test_case=tc, dt=dt, ad=(dh, du), diag_seeds=seeds)
+    save("tc5_l3_diag_seeds", m3, dict(test_case=tc, dt=dt, nsteps=n, perturbation_seed=SEED + 9),
+         dh=dh, du=du, h_ad=a["h_ad"], u_ad=a["u_ad"], **{f"seed_{k}": v for k, v in seeds.items()})
+
+
 def level7_day_digest():
     """The benchmark's mesh and case for one day: level 7, TC6, dt 75 s,
     1152 steps.  SHA-256 digests and norms of the reference's forward h, u."""
@@ -195,7 +212,7 @@ def main(names):
             {"tc5_l3_dissipation": dissipation_case, "tc5_l3_recon_adjoint": recon_adjoint_case,
              "tc5_l6_1day_digest": level6_day_digest, "tc6_l7_window_digest": level7_window_digest,
              "l7_ic_digest": level7_ic_digest, "quirks": quirk_cases,
-             "tc6_l7_1day_digest": level7_day_digest}[n]()
+             "tc6_l7_1day_digest": level7_day_digest, "tc5_l3_diag_seeds": diag_seed_case}[n]()
         return
 
     # 1. Williamson TC2, level 5, dt = 300 s, 288 steps (one day)
@@ -266,6 +283,9 @@ def main(names):
 
     # 11. one day on the benchmark's mesh and case, as digests
     level7_day_digest()
+
+    # 12. diagnostic adjoint seeds consumed by sw_rk4_adj
+    diag_seed_case()
 
 
 if __name__ == "__main__":

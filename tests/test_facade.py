@@ -111,3 +111,45 @@ def test_facade_steps_on_gpu():
     np.testing.assert_array_equal(mpsw.h[1, 0], o["h2"])
     lhs, rhs = mpsw.link_tlm_adj()
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+@pytest.mark.gpu
+def test_facade_diagnostic_adjoint_seeds_against_reference_fortran():
+    """The reference's adjoint flow (ref run-adj: forward storing each step's
+    start state, then sw_rk4_adj backwards) with seeds in the end-state
+    adjoints and in all 13 diagnostic adjoint arrays: sw_rk4_adj consumes
+    and zeroes the diagnostic seeds in its first call, and after three steps
+    h_ad, u_ad are the reference Fortran's to 1e-11 (golden vectors written
+    by the reference itself)."""
+    print("facade diagnostic adjoint seeds L3 tc5 vs reference", flush=True)
+    from helpers import golden
+    g, meta = golden("tc5_l3_diag_seeds")
+    n, dt = meta["nsteps"], meta["dt"]
+    mesh, _ = case(3, 5)
+    mpsw = fresh_module()
+    load_mesh(mpsw, mesh)
+    mpsw.config_test_case = 5
+    mpsw.config_dt = dt
+    mpsw.sw_test_case_5()
+    mpsw.var_allocation_tlm()
+    mpsw.var_allocation_adj()
+    mpsw.sw_mpas_init_block()
+    traj = []
+    for _ in range(n):
+        traj.append((mpsw.h[0, 0].copy(), mpsw.u[0, 0].copy()))
+        mpsw.sw_rk4()
+    mpsw.h_ad[0, 0] = g["dh"]
+    mpsw.u_ad[0, 0] = g["du"]
+    names = [k[5:] for k in g if k.startswith("seed_")]
+    assert len(names) == 13
+    for k in names:
+        getattr(mpsw, k.lower() + "_ad")[0] = g[f"seed_{k}"]
+    for s in range(n - 1, -1, -1):
+        mpsw.h[0, 0] = traj[s][0]
+        mpsw.u[0, 0] = traj[s][1]
+        mpsw.sw_rk4_adj()
+        if s == n - 1:
+            for k in names:
+                assert not getattr(mpsw, k.lower() + "_ad").any(), k
+    assert rel_l2(mpsw.h_ad[0, 0], g["h_ad"]) < 1e-11
+    assert rel_l2(mpsw.u_ad[0, 0], g["u_ad"]) < 1e-11

@@ -779,3 +779,42 @@ def test_pipelined_adjoint_sweep_equals_single_steps(level, prec):
         outs.append((hs.cpu(), us.cpu()))
     for a, b in outs[1:]:
         assert torch.equal(a, outs[0][0]) and torch.equal(b, outs[0][1])
+
+
+@pytest.mark.parametrize("B,prec,tol", [(1, "fp64", 1e-11), (64, "fp64", 1e-11), (4, "fp32", 1e-4)])
+def test_diagnostics_adjoint_matches_oracle(B, prec, tol):
+    """msw_diagnostics_adj: seeds in all 13 diagnostic adjoints, different
+    per member, at a TC5 state: each sampled member's (h_ad, u_ad) matches the
+    oracle's sw_compute_solve_diagnostics_adj (the sums run in another order:
+    1e-11 in FP64), added to what h_ad and u_ad held."""
+    print(f"diagnostics adjoint L3 tc5 B={B} {prec}", flush=True)
+    tc, dt = 5, 900.0
+    mesh, phys = case(LEVEL, tc)
+    m = model(LEVEL, tc, dt, batch=B, precision=prec)
+    o = make_oracle(LEVEL, tc, dt)
+    o.rk4(2)   # a state away from the initial one
+    h0, u0 = o["h1"].copy(), o["u1"].copy()
+    names = m.lib.msw_diag_names().decode().split(",")
+    kinds = m.lib.msw_diag_kinds().decode().split(",")
+    rng = np.random.default_rng(17)
+    size = {"c": mesh.nCells, "e": mesh.nEdges, "v": mesh.nVertices}
+    seeds = {n: rng.standard_normal((size[k], B)) for n, k in zip(names, kinds)}
+    # what h_ad, u_ad already hold (zero in FP32, where subtracting it would
+    # cost digits of a small contribution)
+    amp = 1.0 if prec == "fp64" else 0.0
+    base_h = amp * rng.standard_normal((mesh.nCells, B))
+    base_u = amp * rng.standard_normal((mesh.nEdges, B))
+    sq = (lambda a: a[:, 0]) if B == 1 else (lambda a: a)
+    h = m.to_internal(h0, "c", member=False)
+    u = m.to_internal(u0, "e", member=False)
+    ha, ua = m.to_internal(sq(base_h), "c"), m.to_internal(sq(base_u), "e")
+    m.diagnostics_adj(h, u, {n: m.to_internal(sq(seeds[n]), k) for n, k in zip(names, kinds)}, ha, ua)
+    torch.cuda.synchronize()
+    HA = m.to_reference(ha, "c").reshape(-1, B)
+    UA = m.to_reference(ua, "e").reshape(-1, B)
+    for b in sorted({0, B // 2, B - 1}):
+        for n in names:
+            o[f"{n}_ad"][:] = seeds[n][:, b]
+        oh, ou = o.diag_adj(h0, u0)
+        assert rel_l2(HA[:, b] - base_h[:, b], oh) < tol, b
+        assert rel_l2(UA[:, b] - base_u[:, b], ou) < tol, b

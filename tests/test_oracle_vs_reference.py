@@ -241,3 +241,32 @@ def test_thickness_advection_orders_are_order_two():
         o.rk4_adj()
     adj = (sha(o["h_ad1"]), sha(o["u_ad1"]))
     assert list(fwd + tlm + adj) == dig["order2"]
+
+
+def test_diagnostic_adjoint_seeds_bitwise():
+    """Seeds put in the 13 diagnostic adjoint arrays are consumed by the
+    first sw_rk4_adj at its end state (src/mpas_sw_module.f90:894-905) and
+    zeroed; three adjoint steps with those seeds and end-state adjoints give
+    the reference's h_ad, u_ad bit for bit."""
+    g, meta = golden("tc5_l3_diag_seeds")
+    n, dt = meta["nsteps"], meta["dt"]
+    o = make_oracle(3, 5, dt)
+    traj = []
+    for _ in range(n):
+        traj.append((o["h1"].copy(), o["u1"].copy()))
+        o.rk4()
+    o["h_ad1"][:] = g["dh"]
+    o["u_ad1"][:] = g["du"]
+    names = [k[5:] for k in g if k.startswith("seed_")]
+    assert len(names) == 13
+    for k in names:
+        o[f"{k}_ad"][:] = g[f"seed_{k}"]
+    for s in range(n - 1, -1, -1):
+        o["h1"][:] = traj[s][0]
+        o["u1"][:] = traj[s][1]
+        o.rk4_adj()
+        if s == n - 1:
+            for k in names:
+                assert not o[f"{k}_ad"].any(), k
+    np.testing.assert_array_equal(o["h_ad1"], g["h_ad"])
+    np.testing.assert_array_equal(o["u_ad1"], g["u_ad"])
