@@ -102,6 +102,13 @@ int msw_set_config(msw_ctx *ctx, const msw_config *cfg);
  * creation stream; NULL resets it to zero (its value after var_allocation).
  * Read only when config wind_stress is set. */
 int msw_set_forcing(msw_ctx *ctx, const double *u_src);
+/* Replaces: the module arrays meshScalingDel2/meshScalingDel4 that
+ * compute_mesh_scaling fills (src/mpas_sw_dyncore.f90:362-392) and the del2
+ * and del4 terms scale per edge (:520, :606, and their TLM and adjoint
+ * twins).  del2, del4: device FP64 [nEdges] in the internal edge order,
+ * copied on the creation stream; both NULL (the default) means all factors
+ * are 1, their value unless config_h_ScaleWithMesh is set. */
+int msw_set_mesh_scaling(msw_ctx *ctx, const double *del2, const double *del4);
 /* Use a captured hipGraph per step for the step functions (default off). */
 int msw_set_graphs(msw_ctx *ctx, int enable);
 

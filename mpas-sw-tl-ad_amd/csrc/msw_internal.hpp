@@ -76,6 +76,8 @@ struct DConfig {
     int wind;       // config_wind_stress
     int drag;       // config_bottom_drag
     const double* usrc;   // [E] wind-stress source u_src (internal order), FP64
+    const double* msd2;   // [E] meshScalingDel2 of the del2 term, nullptr: all 1
+    const double* msd4;   // [E] meshScalingDel4 of the del4 term, nullptr: all 1
 };
 
 // Fields the del2/del4 terms of tend_u read (src/mpas_sw_dyncore.f90:508-617):

@@ -369,8 +369,16 @@ __device__ __forceinline__ T grad_div_curl(const T* f, const T* g, int c1, int c
 
 // Dissipation and forcing added to tend_u in the reference's order
 // (src/mpas_sw_dyncore.f90:508-637): + del2, - del4, + wind stress, - drag.
-// The mesh scaling factors are 1 (config_h_ScaleWithMesh cannot be set
-// through the reference's Python interface), so msd * visc * x == visc * x.
+// Viscosity of the del2/del4 term at edge e: meshScalingDel2/4(e) * visc, in
+// that order (src/mpas_sw_dyncore.f90:520,606); without scaling factors
+// (the default, compute_mesh_scaling without config_h_ScaleWithMesh) it is
+// visc itself, the same number.
+template <typename T>
+__device__ __forceinline__ T visc_at(const double* msd, int e, double visc)
+{
+    return msd ? T(msd[e] * visc) : T(visc);
+}
+
 template <typename T>
 __device__ __forceinline__ T tend_forcing(const DMesh& m, const DConfig& c, const Dissip<T>& dp,
                                           int e, int c1, int c2, int B, int b, T tend, T ue, T hee,
@@ -378,8 +386,10 @@ __device__ __forceinline__ T tend_forcing(const DMesh& m, const DConfig& c, cons
 {
     const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
     const T dc = T(m.dc[e]), dv = T(m.dv[e]);
-    if (c.visc2 > 0.0) tend = tend + T(c.visc2) * grad_div_curl(dp.div, dp.vort, c1, c2, v1, v2, B, b, dc, dv);
-    if (c.visc4 > 0.0) tend = tend - T(c.visc4) * grad_div_curl(dp.dsd, dp.dsv, c1, c2, v1, v2, B, b, dc, dv);
+    if (c.visc2 > 0.0)
+        tend = tend + visc_at<T>(c.msd2, e, c.visc2) * grad_div_curl(dp.div, dp.vort, c1, c2, v1, v2, B, b, dc, dv);
+    if (c.visc4 > 0.0)
+        tend = tend - visc_at<T>(c.msd4, e, c.visc4) * grad_div_curl(dp.dsd, dp.dsv, c1, c2, v1, v2, B, b, dc, dv);
     if (c.wind) tend = tend + T(c.usrc[e]) / T(kRhoRef) / hee;
     if (c.drag) {
         const T kee = T(0.5) * (ke1 + ke2);
@@ -729,8 +739,10 @@ __device__ __forceinline__ T tend_forcing_tl(const DMesh& m, const DConfig& c, c
 {
     const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
     const T dc = T(m.dc[e]), dv = T(m.dv[e]);
-    if (c.visc2 > 0.0) tend = tend + T(c.visc2) * grad_div_curl(dp.div, dp.vort, c1, c2, v1, v2, B, b, dc, dv);
-    if (c.visc4 > 0.0) tend = tend - T(c.visc4) * grad_div_curl(dp.dsd, dp.dsv, c1, c2, v1, v2, B, b, dc, dv);
+    if (c.visc2 > 0.0)
+        tend = tend + visc_at<T>(c.msd2, e, c.visc2) * grad_div_curl(dp.div, dp.vort, c1, c2, v1, v2, B, b, dc, dv);
+    if (c.visc4 > 0.0)
+        tend = tend - visc_at<T>(c.msd4, e, c.visc4) * grad_div_curl(dp.dsd, dp.dsv, c1, c2, v1, v2, B, b, dc, dv);
     const T he = T(bs.he[e]);
     if (c.wind) tend = tend - het * T(c.usrc[e]) / T(kRhoRef) / (he * he);
     if (c.drag) {
@@ -1037,7 +1049,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_cell1(DMesh m, DConfig c, int
 #pragma unroll
             for (int i = 0; i < kC6; ++i) {
                 T x = T(0);
-                if (c.visc2 > 0.0) x = T(c.visc2) * ge[i];
+                if (c.visc2 > 0.0) x = visc_at<T>(c.msd2, ie[i], c.visc2) * ge[i];
                 if (c.visc4 > 0.0) x = x + ld(dsu_ad, ie[i], B, b);
                 da = da - T(m.sgnC[kC6 * ic + i]) * x / T(m.dc[ie[i]]);
             }
@@ -1082,7 +1094,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_vertex(DMesh m, DConfig c, in
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 T x = T(0);
-                if (c.visc2 > 0.0) x = T(c.visc2) * (gs * ld(gu, ie[i], B, b));
+                if (c.visc2 > 0.0) x = visc_at<T>(c.msd2, ie[i], c.visc2) * (gs * ld(gu, ie[i], B, b));
                 if (c.visc4 > 0.0) x = x + ld(dsu_ad, ie[i], B, b);
                 vv = vv - T(m.sgnV[3 * iv + i]) * x / T(m.dv[ie[i]]);
             }
@@ -1114,7 +1126,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_del4_vertex(DMesh m, DConfig 
         for (int i = 0; i < 3; ++i) g[i] = gs * ld(gu, ie[i], B, b);
         T a = T(0);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) a = a + T(m.sgnV[3 * iv + i]) * (T(c.visc4) * g[i]) / T(m.dv[ie[i]]);
+        for (int i = 0; i < 3; ++i)
+            a = a + T(m.sgnV[3 * iv + i]) * (visc_at<T>(c.msd4, ie[i], c.visc4) * g[i]) / T(m.dv[ie[i]]);
         stn(&rowp(dsc_ad, iv, B)[b], a * (T(1.0) / T(m.areaT[iv])));
     }
 }
@@ -1137,7 +1150,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_del4_cell(DMesh m, DConfig c,
         for (int i = 0; i < kC6; ++i) g[i] = gs * ld(gu, ie[i], B, b);
         T a = T(0);
 #pragma unroll
-        for (int i = 0; i < kC6; ++i) a = a + T(m.sgnC[kC6 * ic + i]) * (T(c.visc4) * g[i]) / T(m.dc[ie[i]]);
+        for (int i = 0; i < kC6; ++i)
+            a = a + T(m.sgnC[kC6 * ic + i]) * (visc_at<T>(c.msd4, ie[i], c.visc4) * g[i]) / T(m.dc[ie[i]]);
         stn(&rowp(dsd_ad, ic, B)[b], a * (T(1.0) / T(m.areaC[ic])));
     }
 }

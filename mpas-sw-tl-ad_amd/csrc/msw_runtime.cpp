@@ -96,6 +96,8 @@ struct msw_ctx {
     double* bdsd = nullptr;           // [C]
     double* bdsv = nullptr;           // [V]
     double* usrc = nullptr;           // [E]  wind-stress source, internal order
+    double* msd = nullptr;            // [2E] meshScalingDel2, meshScalingDel4 (internal order)
+    bool msd_on = false;              // false: the factors are all 1
     AdjCoef adjq{};                   // stage coefficients (B = 1) of the eager single-stream path
     AdjCoef qs[4]{};                  // per-stage coefficients of the two-stream steps
     // second base-stage / coefficient set and step-start state: a multi-step
@@ -471,6 +473,8 @@ void set_dcfg(msw_ctx* c)
     c->dcfg.wind = g.wind_stress ? 1 : 0;
     c->dcfg.drag = g.bottom_drag ? 1 : 0;
     c->dcfg.usrc = c->usrc;
+    c->dcfg.msd2 = c->msd_on ? c->msd : nullptr;
+    c->dcfg.msd4 = c->msd_on ? c->msd + c->nE : nullptr;
     c->dcfg.tl_fused = c->tl_fused ? 1 : 0;
 }
 
@@ -856,6 +860,27 @@ int msw_set_forcing(msw_ctx* ctx, const double* u_src)
                                hipMemcpyDeviceToDevice, ctx->stream));
     else
         HIP_TRY(hipMemsetAsync(ctx->usrc, 0, static_cast<size_t>(ctx->nE) * sizeof(double), ctx->stream));
+    return MSW_OK;
+}
+
+int msw_set_mesh_scaling(msw_ctx* ctx, const double* del2, const double* del4)
+{
+    if (!ctx) return fail(MSW_EINVAL, "NULL ctx");
+    if ((!del2) != (!del4)) return fail(MSW_EINVAL, "del2 and del4 go together");
+    const size_t E = ctx->nE;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (del2) {
+        if (!ctx->msd) {
+            int rc = ctx->alloc(&ctx->msd, 2 * E);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipMemcpyAsync(ctx->msd, del2, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->msd + E, del4, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    ctx->msd_on = del2 != nullptr;
+    set_dcfg(ctx);
+    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
     return MSW_OK;
 }
 

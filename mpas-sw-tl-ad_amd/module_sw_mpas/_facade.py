@@ -278,11 +278,6 @@ class _Module:
         """The GPU model for the current mesh and configuration."""
         v = self._vars
         A = self._arrays
-        visc = v["config_h_mom_eddy_visc2"] > 0 or v["config_h_mom_eddy_visc4"] > 0
-        if visc and (np.any(A["meshscalingdel2"] != 1.0) or np.any(A["meshscalingdel4"] != 1.0)):
-            # the GPU kernels take meshScalingDel2/4 == 1, the value
-            # compute_mesh_scaling gives unless config_h_ScaleWithMesh is set
-            raise NotImplementedError("viscosity with mesh scaling (config_h_scalewithmesh) on the GPU path")
         if v["config_thickness_adv_order"] in (3, 4) and np.any(A["deriv_two"] != 0):
             raise NotImplementedError("thickness advection orders 3/4 with a non-zero deriv_two")
         key = (v["config_dt"], v["config_apvm_upwinding"], v["config_test_case"],
@@ -315,6 +310,18 @@ class _Module:
         if v["config_wind_stress"]:
             # the reference reads the module array u_src at every tendency call
             gpu["model"].set_forcing(A["u_src"][0])
+        # meshScalingDel2/4 scale the del2/del4 terms per edge; they are 1
+        # unless config_h_ScaleWithMesh was set for compute_mesh_scaling
+        # (src/mpas_sw_dyncore.f90:362-392); uploaded when they change
+        d2, d4 = A["meshscalingdel2"], A["meshscalingdel4"]
+        scaled = bool(np.any(d2 != 1.0) or np.any(d4 != 1.0))
+        sig = (d2.tobytes(), d4.tobytes()) if scaled else None
+        if gpu.get("scaling") != sig:
+            if scaled:
+                gpu["model"].set_mesh_scaling(d2, d4)
+            else:
+                gpu["model"].set_mesh_scaling()
+            gpu["scaling"] = sig
         return gpu["model"]
 
     def _up(self, m, arr, kind):
@@ -366,9 +373,12 @@ class _Module:
         if self._vars["config_h_scalewithmesh"]:
             c1 = A["cellsonedge"][0] - 1
             c2 = A["cellsonedge"][1] - 1
-            md = A["meshdensity"]
-            A["meshscalingdel2"][...] = 1.0 / ((md[c1] + md[c2]) / 2.0) ** (5.0 / 12.0)
-            A["meshscalingdel4"][...] = 1.0 / ((md[c1] + md[c2]) / 2.0) ** (5.0 / 6.0)
+            # compute_mesh_scaling (src/mpas_sw_dyncore.f90:362-392), pow of the
+            # C library element by element, as the reference's compiled code calls it
+            import math
+            avg = ((A["meshdensity"][c1] + A["meshdensity"][c2]) / 2.0).tolist()
+            A["meshscalingdel2"][...] = [1.0 / math.pow(x, 5.0 / 12.0) for x in avg]
+            A["meshscalingdel4"][...] = [1.0 / math.pow(x, 5.0 / 6.0) for x in avg]
         xc = np.stack([A["xcell"], A["ycell"], A["zcell"]], 1)
         xe = np.stack([A["xedge"], A["yedge"], A["zedge"]], 1)
         lvu, env, ctp = initialize_vectors(xc, xe, A["cellsonedge"], A["edgesoncell"])

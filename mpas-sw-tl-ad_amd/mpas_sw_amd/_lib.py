@@ -79,7 +79,8 @@ class Config(C.Structure):
 
 # every symbol include/mpas_sw_amd.h declares
 EXPORTS = (
-    "msw_create", "msw_destroy", "msw_set_config", "msw_set_forcing", "msw_set_option",
+    "msw_create", "msw_destroy", "msw_set_config", "msw_set_forcing", "msw_set_mesh_scaling",
+    "msw_set_option",
     "msw_set_graphs", "msw_fwd_steps",
     "msw_base_steps", "msw_tlm_steps", "msw_adj_step", "msw_adj_steps", "msw_diagnostics",
     "msw_diagnostics_adj",
@@ -118,6 +119,7 @@ def load():
         "msw_destroy": ([vp], i32),
         "msw_set_config": ([vp, C.POINTER(Config)], i32),
         "msw_set_forcing": ([vp, vp], i32),
+        "msw_set_mesh_scaling": ([vp, vp, vp], i32),
         "msw_set_option": ([vp, i32, i32], i32),
         "msw_set_graphs": ([vp, i32], i32),
         "msw_fwd_steps": ([vp, vp, vp, i32], i32),

@@ -224,6 +224,19 @@ class SWModel:
         self.set_option(_lib.OPT_BLOCK, best[1])
         return times
 
+    def set_mesh_scaling(self, del2=None, del4=None):
+        """Per-edge factors of the del2 and del4 terms (meshScalingDel2/4 of
+        compute_mesh_scaling, [E] reference order); None for both: all 1."""
+        if (del2 is None) != (del4 is None):
+            raise ValueError("del2 and del4 go together")
+        if del2 is None:
+            _lib.check(self.lib.msw_set_mesh_scaling(self._h, None, None))
+            return
+        d2 = self.to_internal(np.asarray(del2, np.float64).reshape(-1), "e", member=False)
+        d4 = self.to_internal(np.asarray(del4, np.float64).reshape(-1), "e", member=False)
+        _lib.check(self.lib.msw_set_mesh_scaling(self._h, _ptr(d2), _ptr(d4)))
+        self.stream.synchronize()   # the copies read d2, d4 before they go
+
     def set_forcing(self, u_src):
         """Wind-stress source u_src [E] (reference order); None resets it to 0."""
         if u_src is None:

@@ -35,7 +35,8 @@ DIAG_SEED_KINDS = "eevvvccevccee"   # entity of each, in that order
 
 def write_input(path, mesh, test_case, dt, apvm=0.5, visc2=0.0, visc4=0.0, adv_order=2,
                 monotonic=False, wind_stress=False, bottom_drag=False,
-                state=None, tl=None, ad=None, u_src=None, recon_seeds=None, diag_seeds=None):
+                state=None, tl=None, ad=None, u_src=None, recon_seeds=None, diag_seeds=None,
+                scale_with_mesh=False):
     hdr = np.zeros(16, np.int32)
     hdr[0] = MAGIC
     hdr[1] = 1
@@ -53,6 +54,7 @@ def write_input(path, mesh, test_case, dt, apvm=0.5, visc2=0.0, visc4=0.0, adv_o
     cfg[4] = 1.0 if u_src is not None else 0.0
     cfg[5] = 1.0 if recon_seeds is not None else 0.0
     cfg[6] = 1.0 if diag_seeds is not None else 0.0
+    cfg[7] = 1.0 if scale_with_mesh else 0.0
     f8 = lambda x: np.asarray(x, np.float64).ravel(order="C")
     i4 = lambda x: np.asarray(x, np.int32).ravel(order="C")
     with open(path, "wb") as f:

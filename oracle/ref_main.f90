@@ -54,6 +54,9 @@ program msw_ref
   config_wind_stress = hdr(12) /= 0
   config_bottom_drag = hdr(13) /= 0
   has_state = hdr(14); has_tl = hdr(15); has_ad = hdr(16)
+  ! cfg(8) /= 0: config_h_ScaleWithMesh, which src/mpas_sw_driver.py cannot
+  ! reach (mixed case through f2py) but compute_mesh_scaling reads
+  config_h_ScaleWithMesh = cfg(8) /= 0.0
   config_dt = cfg(1)
   config_apvm_upwinding = cfg(2)
   config_h_mom_eddy_visc2 = cfg(3)

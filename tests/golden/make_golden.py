@@ -162,6 +162,33 @@ def diag_seed_case():
          dh=dh, du=du, h_ad=a["h_ad"], u_ad=a["u_ad"], **{f"seed_{k}": v for k, v in seeds.items()})
 
 
+def mesh_density(mesh):
+    """A smooth non-uniform cell density in [1, 1.75] (the generated meshes are
+    uniform, density 1, which makes every mesh-scaling factor 1)."""
+    return 1.0 + 0.375 * (1.0 + np.sin(mesh.latCell) * np.cos(mesh.lonCell))
+
+
+def mesh_scaling_case():
+    """Level 3, TC5, dt 900 s, del2 and del4 viscosity with
+    config_h_ScaleWithMesh on a non-uniform meshDensity: forward, TLM and
+    adjoint after 6 steps, so the per-edge meshScalingDel2/4 factors of
+    compute_mesh_scaling (src/mpas_sw_dyncore.f90:362-392) enter every term."""
+    import copy
+    m3 = copy.copy(get_mesh(3))
+    m3.meshDensity = mesh_density(m3)
+    tc, dt, n = 5, 900.0, 6
+    kw = dict(test_case=tc, dt=dt, visc2=DISSIPATION["visc2"], visc4=DISSIPATION["visc4"],
+              scale_with_mesh=True)
+    dh, du = perturb(m3, SEED + 11)
+    f = ref.run(m3, "fwd", nsteps=n, **kw)
+    t = ref.run(m3, "tlm", nsteps=n, tl=(dh, du), **kw)
+    a = ref.run(m3, "adj", nsteps=n, ad=(dh, du), **kw)
+    save("tc5_l3_mesh_scaling", get_mesh(3), dict(test_case=tc, dt=dt, nsteps=n, perturbation_seed=SEED + 11,
+                                          visc2=kw["visc2"], visc4=kw["visc4"]),
+         density=m3.meshDensity, dh=dh, du=du, h=f["h"], u=f["u"], h_tl=t["h_tl"], u_tl=t["u_tl"],
+         h_base=t["h"], u_base=t["u"], h_ad=a["h_ad"], u_ad=a["u_ad"])
+
+
 def level7_day_digest():
     """The benchmark's mesh and case for one day: level 7, TC6, dt 75 s,
     1152 steps.  SHA-256 digests and norms of the reference's forward h, u."""
@@ -212,7 +239,8 @@ def main(names):
             {"tc5_l3_dissipation": dissipation_case, "tc5_l3_recon_adjoint": recon_adjoint_case,
              "tc5_l6_1day_digest": level6_day_digest, "tc6_l7_window_digest": level7_window_digest,
              "l7_ic_digest": level7_ic_digest, "quirks": quirk_cases,
-             "tc6_l7_1day_digest": level7_day_digest, "tc5_l3_diag_seeds": diag_seed_case}[n]()
+             "tc6_l7_1day_digest": level7_day_digest, "tc5_l3_diag_seeds": diag_seed_case,
+             "tc5_l3_mesh_scaling": mesh_scaling_case}[n]()
         return
 
     # 1. Williamson TC2, level 5, dt = 300 s, 288 steps (one day)
@@ -286,6 +314,9 @@ def main(names):
 
     # 12. diagnostic adjoint seeds consumed by sw_rk4_adj
     diag_seed_case()
+
+    # 13. del2/del4 scaled with a non-uniform mesh density
+    mesh_scaling_case()
 
 
 if __name__ == "__main__":

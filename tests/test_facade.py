@@ -153,3 +153,60 @@ def test_facade_diagnostic_adjoint_seeds_against_reference_fortran():
                 assert not getattr(mpsw, k.lower() + "_ad").any(), k
     assert rel_l2(mpsw.h_ad[0, 0], g["h_ad"]) < 1e-11
     assert rel_l2(mpsw.u_ad[0, 0], g["u_ad"]) < 1e-11
+
+
+@pytest.mark.gpu
+def test_facade_mesh_scaling_against_reference_fortran():
+    """del2 and del4 viscosity with config_h_scalewithmesh on a non-uniform
+    meshDensity, so every edge scales both terms by its own
+    meshScalingDel2/4 factor (src/mpas_sw_dyncore.f90:362-392): after 6 steps
+    the forward and the TLM equal the reference Fortran's bit for bit and the
+    adjoint is within 1e-11 of it."""
+    print("facade mesh-scaled viscosity L3 tc5 vs reference", flush=True)
+    from helpers import golden
+    g, meta = golden("tc5_l3_mesh_scaling")
+    n, dt = meta["nsteps"], meta["dt"]
+    mesh, _ = case(3, 5)
+
+    def setup():
+        mpsw = fresh_module()
+        load_mesh(mpsw, mesh)
+        mpsw.meshdensity = g["density"]
+        mpsw.config_test_case = 5
+        mpsw.config_dt = dt
+        mpsw.config_h_mom_eddy_visc2 = meta["visc2"]
+        mpsw.config_h_mom_eddy_visc4 = meta["visc4"]
+        mpsw.config_h_scalewithmesh = True
+        mpsw.sw_test_case_5()
+        mpsw.var_allocation_tlm()
+        mpsw.var_allocation_adj()
+        mpsw.sw_mpas_init_block()
+        assert mpsw.meshscalingdel2.min() < 0.8 and mpsw.meshscalingdel4.min() < 0.7
+        return mpsw
+
+    mpsw = setup()
+    for _ in range(n):
+        mpsw.sw_rk4()
+    np.testing.assert_array_equal(mpsw.h[0, 0], g["h"])
+    np.testing.assert_array_equal(mpsw.u[0, 0], g["u"])
+    mpsw = setup()
+    mpsw.h_tl[0, 0] = g["dh"]
+    mpsw.u_tl[0, 0] = g["du"]
+    for _ in range(n):
+        mpsw.sw_rk4_tlm()
+    np.testing.assert_array_equal(mpsw.h_tl[0, 0], g["h_tl"])
+    np.testing.assert_array_equal(mpsw.u_tl[0, 0], g["u_tl"])
+    np.testing.assert_array_equal(mpsw.h[0, 0], g["h_base"])
+    mpsw = setup()
+    traj = []
+    for _ in range(n):
+        traj.append((mpsw.h[0, 0].copy(), mpsw.u[0, 0].copy()))
+        mpsw.sw_rk4()
+    mpsw.h_ad[0, 0] = g["dh"]
+    mpsw.u_ad[0, 0] = g["du"]
+    for s in range(n - 1, -1, -1):
+        mpsw.h[0, 0] = traj[s][0]
+        mpsw.u[0, 0] = traj[s][1]
+        mpsw.sw_rk4_adj()
+    assert rel_l2(mpsw.h_ad[0, 0], g["h_ad"]) < 1e-11
+    assert rel_l2(mpsw.u_ad[0, 0], g["u_ad"]) < 1e-11
