@@ -9,6 +9,7 @@
 #include "msw_internal.hpp"
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cmath>
@@ -787,6 +788,17 @@ DiagSeeds<T> diag_seeds(const msw_ctx* c, const void* p)
     return s;
 }
 
+// A named range around each step and operator entry point, recorded by
+// rocprofv3 --marker-trace (a no-op when no profiler is attached); the range
+// covers the host call, i.e. the enqueueing of the asynchronous work.
+struct TraceRange {
+    explicit TraceRange(const char* name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
+#define MSW_TRACE() TraceRange msw_trace_range_(__func__)
+
 extern "C" {
 
 int msw_abi_version(void) { return MSW_ABI_VERSION; }
@@ -796,6 +808,7 @@ const char* msw_last_error(void) { return g_err.c_str(); }
 int msw_create(const msw_mesh_desc* mesh, const msw_config* cfg, int prec, int batch,
                void* hip_stream, msw_ctx** out)
 {
+    MSW_TRACE();
     if (!out) return fail(MSW_EINVAL, "out is NULL");
     *out = nullptr;
     if (!mesh || !cfg) return fail(MSW_EINVAL, "mesh or cfg is NULL");
@@ -919,6 +932,7 @@ static bool members_aligned(const msw_ctx* ctx, const void* a, const void* b)
 
 int msw_fwd_steps(msw_ctx* ctx, void* h, void* u, int nsteps)
 {
+    MSW_TRACE();
     if (!ctx || !h || !u || nsteps < 0) return fail(MSW_EINVAL, "bad argument");
     if (!members_aligned(ctx, h, u)) return fail(MSW_EINVAL, "h/u must be 16-byte aligned");
     int rc = check_cfg(ctx);
@@ -932,6 +946,7 @@ int msw_fwd_steps(msw_ctx* ctx, void* h, void* u, int nsteps)
 
 int msw_base_steps(msw_ctx* ctx, double* h, double* u, int nsteps, double* traj_h, double* traj_u)
 {
+    MSW_TRACE();
     if (!ctx || !h || !u || nsteps < 0 || (!traj_h) != (!traj_u)) return fail(MSW_EINVAL, "bad argument");
     int rc = check_cfg(ctx);
     if (rc) return rc;
@@ -947,6 +962,7 @@ int msw_base_steps(msw_ctx* ctx, double* h, double* u, int nsteps, double* traj_
 int msw_tlm_steps(msw_ctx* ctx, double* h, double* u, void* h_tl, void* u_tl, int nsteps,
                   double* traj_h, double* traj_u)
 {
+    MSW_TRACE();
     if (!ctx || !h || !u || !h_tl || !u_tl || nsteps < 0 || (!traj_h) != (!traj_u))
         return fail(MSW_EINVAL, "bad argument");
     if (!members_aligned(ctx, h_tl, u_tl)) return fail(MSW_EINVAL, "h_tl/u_tl must be 16-byte aligned");
@@ -983,6 +999,7 @@ static int adj_common(msw_ctx* ctx, void* h_ad, void* u_ad, int nsteps, bool wan
 int msw_adj_step(msw_ctx* ctx, const double* h, const double* u, void* h_ad, void* u_ad,
                  double* h_end, double* u_end)
 {
+    MSW_TRACE();
     if (!ctx || !h || !u || !h_ad || !u_ad) return fail(MSW_EINVAL, "bad argument");
     int rc = check_cfg(ctx);
     if (rc) return rc;
@@ -1002,6 +1019,7 @@ int msw_adj_step(msw_ctx* ctx, const double* h, const double* u, void* h_ad, voi
 int msw_adj_steps(msw_ctx* ctx, const double* traj_h, const double* traj_u, void* h_ad, void* u_ad,
                   int nsteps)
 {
+    MSW_TRACE();
     if (!ctx || !traj_h || !traj_u || !h_ad || !u_ad || nsteps < 0) return fail(MSW_EINVAL, "bad argument");
     int rc = check_cfg(ctx);
     if (rc) return rc;
@@ -1044,6 +1062,7 @@ const char* msw_diag_kinds(void) { return kDiagKinds; }
 
 int msw_diagnostics(msw_ctx* ctx, const double* h, const double* u, double* out)
 {
+    MSW_TRACE();
     if (!ctx || !h || !u || !out) return fail(MSW_EINVAL, "bad argument");
     const size_t C = ctx->nC, E = ctx->nE, V = ctx->nV;
     double* o = out;
@@ -1069,6 +1088,7 @@ int msw_diagnostics(msw_ctx* ctx, const double* h, const double* u, double* out)
 int msw_diagnostics_adj(msw_ctx* ctx, const double* h, const double* u, const void* seeds, void* h_ad,
                         void* u_ad)
 {
+    MSW_TRACE();
     if (!ctx || !h || !u || !seeds || !h_ad || !u_ad) return fail(MSW_EINVAL, "bad argument");
     int rc = check_cfg(ctx);
     if (rc) return rc;
@@ -1103,6 +1123,7 @@ int msw_diagnostics_adj(msw_ctx* ctx, const double* h, const double* u, const vo
 
 int msw_invariants(msw_ctx* ctx, const double* h, const double* u, double* out)
 {
+    MSW_TRACE();
     if (!ctx || !h || !u || !out) return fail(MSW_EINVAL, "bad argument");
     const size_t C = ctx->nC, E = ctx->nE, V = ctx->nV;
     const DMesh& m = ctx->dm;
@@ -1136,6 +1157,7 @@ int msw_invariants(msw_ctx* ctx, const double* h, const double* u, double* out)
 
 int msw_reconstruct(msw_ctx* ctx, const void* u, void* out5)
 {
+    MSW_TRACE();
     if (!ctx || !u || !out5) return fail(MSW_EINVAL, "bad argument");
     if (ctx->prec == MSW_FP64)
         reconstruct<double>(ctx->nC, ctx->lb, ctx->dm.nEoC, ctx->dm.eoc, ctx->coeffs, ctx->trig,
@@ -1149,6 +1171,7 @@ int msw_reconstruct(msw_ctx* ctx, const void* u, void* out5)
 
 int msw_reconstruct_adj(msw_ctx* ctx, const void* seeds5, void* u_ad)
 {
+    MSW_TRACE();
     if (!ctx || !seeds5 || !u_ad) return fail(MSW_EINVAL, "bad argument");
     // scratch: the member C-fields mc1..mc3 are contiguous in the member block
     if (ctx->prec == MSW_FP64)
@@ -1166,6 +1189,7 @@ int msw_reconstruct_adj(msw_ctx* ctx, const void* seeds5, void* u_ad)
 int msw_dot(msw_ctx* ctx, const void* xh, const void* xu, const void* yh, const void* yu,
             const double* wh, const double* wu, double* out)
 {
+    MSW_TRACE();
     if (!ctx || !out || (!xh && !xu)) return fail(MSW_EINVAL, "bad argument");
     bool acc = false;
     auto run = [&](long n, const void* x, const void* y, const double* w) {
@@ -1185,6 +1209,7 @@ int msw_dot(msw_ctx* ctx, const void* xh, const void* xu, const void* yh, const 
 
 int msw_nonfinite(msw_ctx* ctx, const void* h, const void* u, int* flags)
 {
+    MSW_TRACE();
     if (!ctx || !flags) return fail(MSW_EINVAL, "bad argument");
     if (ctx->prec == MSW_FP64) {
         if (h) nonfinite_flags<double>(ctx->nC, ctx->lb, static_cast<const double*>(h), flags, ctx->stream);
@@ -1199,6 +1224,7 @@ int msw_nonfinite(msw_ctx* ctx, const void* h, const void* u, int* flags)
 
 int msw_permute(msw_ctx* ctx, int kind, int to_internal, int prec, int batch, const void* src, void* dst)
 {
+    MSW_TRACE();
     if (!ctx || !src || !dst || src == dst) return fail(MSW_EINVAL, "bad argument");
     if (!is_pow2(batch) || batch > 1024) return fail(MSW_EINVAL, "batch must be a power of two <= 1024");
     int lb = 0;

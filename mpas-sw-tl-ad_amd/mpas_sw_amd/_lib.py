@@ -27,6 +27,9 @@ HIPCC_FLAGS = [
     "-Wall",
 ]
 
+# roctx ranges around the entry points (rocprofv3 --marker-trace)
+LINK_FLAGS = ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx"]
+
 MSW_FP64 = 0
 MSW_FP32 = 1
 
@@ -43,7 +46,7 @@ def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "hipcc")
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, *srcs]
+    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, *srcs, *LINK_FLAGS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
