@@ -91,3 +91,36 @@ def test_run_sw_fwd_and_check_tlm(tmp_path):
     res = mpas_sw_driver.check_tlm(nml, idx2=100, npts=9)
     r6 = res[6]
     assert abs(r6[1] - 1) < 1e-3 and abs(r6[2] - 1) < 1e-3
+
+
+@pytest.mark.gpu
+def test_run_sw_adj(tmp_path):
+    """The flow of demo/run-adj.py (forward sweep keeping every step's start
+    state, then sw_rk4_adj backwards from a unit seed at one cell and one
+    edge) against the oracle's adjoint sweep over the same 13 steps."""
+    print("driver run_sw_adj L3 tc5", flush=True)
+    import importlib
+    import module_sw_mpas._facade as fac
+    importlib.reload(fac)
+    import module_sw_mpas
+    module_sw_mpas.mpas_sw_module = fac.mpas_sw_module
+    import mpas_sw_driver
+    importlib.reload(mpas_sw_driver)
+    from helpers import rel_l2
+    nml = write_case(str(tmp_path), 3, 5, 900.0, "00_03:00:00")
+    u_ad, h_ad = mpas_sw_driver.run_sw_adj(nml, seed_index=7)
+    o = make_oracle(3, 5, 900.0)
+    traj = []
+    for _ in range(13):
+        traj.append((o["h1"].copy(), o["u1"].copy()))
+        o.rk4()
+    o["h_ad1"][:] = 0.0
+    o["u_ad1"][:] = 0.0
+    o["h_ad1"][7] = 1.0
+    o["u_ad1"][7] = 1.0
+    for s in range(12, -1, -1):
+        o["h1"][:] = traj[s][0]
+        o["u1"][:] = traj[s][1]
+        o.rk4_adj()
+    assert rel_l2(h_ad, o["h_ad1"]) < 1e-11
+    assert rel_l2(u_ad, o["u_ad1"]) < 1e-11
