@@ -121,7 +121,8 @@ int msw_set_graphs(msw_ctx *ctx, int enable);
  * term order (bit-identical FP64 TLM).
  *
  * MSW_OPT_XCD_STRIPE (default 2): log2 of the number of consecutive blocks of
- * a launch that go to one XCD before the next XCD's turn (0..6).  Results do
+ * a launch that go to one XCD before the next XCD's turn (0..6), or 7: each
+ * XCD takes one contiguous range of the launch's blocks.  Results do
  * not depend on it; the fastest value depends on mesh, batch and precision
  * (the Python layer's SWModel.autotune times the candidates).
  *

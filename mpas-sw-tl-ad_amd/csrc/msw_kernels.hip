@@ -55,21 +55,26 @@ __device__ __forceinline__ int entity_of(unsigned t, int lb)
 // Stripe length 2^sl blocks (DMesh::stripe_log2, chosen per context; the
 // autotuner of the Python layer times the candidates).  Hardware block i
 // (i below a multiple of 8 * 2^sl) runs logical block stripe_block(i, sl).
-__device__ __forceinline__ unsigned stripe_block(unsigned i, int sl)
+// sl == kRangePerXcd instead gives each XCD one contiguous range of the
+// segment's nseg blocks (a multiple of 8), walked in order.
+constexpr int kRangePerXcd = 7;
+
+__device__ __forceinline__ unsigned stripe_block(unsigned i, int sl, unsigned nseg)
 {
     const unsigned x = i & 7u, j = i >> 3;
+    if (sl == kRangePerXcd) return x * (nseg >> 3) + j;
     return ((j >> sl) << (3 + sl)) + (x << sl) + (j & ((1u << sl) - 1u));
 }
 
 __device__ __forceinline__ unsigned round_blocks_dev(unsigned nb, int sl)
 {
-    const unsigned q = 8u << sl;
+    const unsigned q = sl == kRangePerXcd ? 8u : 8u << sl;
     return ((nb + q - 1) / q) * q;
 }
 
 inline long round_blocks(long nb, int sl)
 {
-    const long q = 8L << sl;
+    const long q = sl == kRangePerXcd ? 8L : 8L << sl;
     return ((nb + q - 1) / q) * q;
 }
 
@@ -100,7 +105,7 @@ inline int grid_for(long threads)
 // one item per thread over `count` entities
 #define ENTITY_ITEM(count)                                                     \
     const unsigned total_ = static_cast<unsigned>(count) << lb;                \
-    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * static_cast<unsigned>(m.block) + threadIdx.x; \
+    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2, gridDim.x) * static_cast<unsigned>(m.block) + threadIdx.x; \
     if (t >= total_) return;
 
 // one item per thread over the edges (is_edge: the first round_blocks(nE
@@ -111,12 +116,13 @@ inline int grid_for(long threads)
     const unsigned bs_ = static_cast<unsigned>(m.block);                      \
     const unsigned gE_ = round_blocks_dev((tE_ + bs_ - 1) / bs_, m.stripe_log2); \
     const bool is_edge = blockIdx.x < gE_;                                     \
-    const unsigned t = stripe_block(is_edge ? blockIdx.x : blockIdx.x - gE_, m.stripe_log2) * bs_ + threadIdx.x; \
+    const unsigned t = stripe_block(is_edge ? blockIdx.x : blockIdx.x - gE_, m.stripe_log2,                  \
+                                    is_edge ? gE_ : gridDim.x - gE_) * bs_ + threadIdx.x;             \
     if (t >= (is_edge ? tE_ : tC_)) return;
 
 // one flat item per thread (no member axis), blocks dealt XCD-aware
 #define FLAT_ITEM(count)                                                       \
-    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * static_cast<unsigned>(m.block) + threadIdx.x; \
+    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2, gridDim.x) * static_cast<unsigned>(m.block) + threadIdx.x; \
     if (t >= static_cast<unsigned>(count)) return;
 
 #define GRID_LOOP(total)                                                               \
@@ -856,7 +862,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_coef_edge(DMesh m, DConfig c,
 {
     // 16 lanes per edge (all of them active through the reduction below):
     // lanes 0-9 fill the edgesOnEdge slots, lanes 12-15 the per-edge values
-    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2) * static_cast<unsigned>(m.block) + threadIdx.x;
+    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2, gridDim.x) * static_cast<unsigned>(m.block) + threadIdx.x;
     const unsigned total = 16u * static_cast<unsigned>(m.nE);
     if (t >= total) return;   // whole 16-lane groups only, so the shuffles stay inside live lanes
     const int e = static_cast<int>(t >> 4), j = static_cast<int>(t & 15);

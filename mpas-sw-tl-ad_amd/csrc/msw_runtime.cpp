@@ -900,8 +900,9 @@ int msw_set_mesh_scaling(msw_ctx* ctx, const double* del2, const double* del4)
 int msw_set_option(msw_ctx* ctx, int option, int value)
 {
     if (!ctx) return fail(MSW_EINVAL, "NULL ctx");
-    if (option == MSW_OPT_XCD_STRIPE && (value < 0 || value > 6))
-        return fail(MSW_EINVAL, "MSW_OPT_XCD_STRIPE takes log2 of the stripe length, 0..6");
+    if (option == MSW_OPT_XCD_STRIPE && (value < 0 || value > 7))
+        return fail(MSW_EINVAL, "MSW_OPT_XCD_STRIPE takes log2 of the stripe length, 0..6, or 7 "
+                                "(one contiguous range per XCD)");
     if (option == MSW_OPT_BLOCK && value != 64 && value != 128 && value != 256 && value != 512)
         return fail(MSW_EINVAL, "MSW_OPT_BLOCK takes 64, 128, 256 or 512 threads");
     if (option != MSW_OPT_TL_FUSED_FLUX && option != MSW_OPT_XCD_STRIPE && option != MSW_OPT_BLOCK)

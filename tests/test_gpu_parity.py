@@ -476,7 +476,7 @@ def test_launch_stripe_does_not_change_results_and_autotune(B):
     dh, du = perturbations(mesh, B)
     m = model(LEVEL, tc, dt, batch=B, graphs=True, tl_fused_flux=True)
     outs = []
-    for k, blk in ((0, 256), (2, 256), (5, 256), (2, 64), (1, 128), (3, 512)):
+    for k, blk in ((0, 256), (2, 256), (5, 256), (2, 64), (1, 128), (3, 512), (7, 256), (7, 64)):
         m.set_option(_lib.OPT_XCD_STRIPE, k)
         m.set_option(_lib.OPT_BLOCK, blk)
         h = m.to_internal(phys["h"], "c", member=False)
