@@ -3,7 +3,7 @@
  * its tangent-linear model and its adjoint (xtian15/MPAS-SW-TL-AD, the src/ Fortran files).
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load it, and only as the checker.  The
+ * cpu_baseline leg of bench.py may load it, and only to check results.  The
  * product path (the HIP library) never calls it.
  *
  * Every routine follows the reference loop order and scatter form so that

@@ -1,6 +1,6 @@
 """ctypes binding of the C oracle (oracle/libmsw_oracle.so).
 
-Test infrastructure only (the checker): imported by tests/, by
+Test infrastructure only (it checks results): imported by tests/, by
 __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the
 product package.  ``Oracle`` mirrors the reference module: named state,
 diagnostic and tendency arrays plus the L1 kernels and the RK4 drivers

@@ -1,6 +1,6 @@
 """Runner for the reference Fortran model built into ``oracle/_ref/msw_ref``.
 
-Test infrastructure (the checker), not product code: only ``tests/``,
+Test infrastructure (it checks results), not product code: only ``tests/``,
 ``tests/golden/make_golden.py`` and the ``cpu_baseline`` leg of ``bench.py``
 use it.  It writes the flat input file that ``oracle/ref_main.f90`` reads
 (mesh arrays in the order ``src/mpas_sw_driver.py:87-119`` sets them, plus
