@@ -464,17 +464,17 @@ def test_invariants_match_oracle_and_mass_is_conserved():
     assert abs(inv[0] - m0[0]) <= 1e-13 * abs(m0[0])
 
 
-@pytest.mark.parametrize("B", [4, 64])
-def test_launch_stripe_does_not_change_results_and_autotune(B):
+@pytest.mark.parametrize("B,prec", [(4, "fp64"), (64, "fp64"), (64, "fp32")])
+def test_launch_stripe_does_not_change_results_and_autotune(B, prec):
     """The block-to-XCD stripe and the block size only reorder work: TLM and
     adjoint results are bitwise the same for every stripe and block size;
     the autotuner picks one of the pairs it timed."""
-    print(f"stripes and block sizes L3 tc6 B={B}", flush=True)
+    print(f"stripes and block sizes L3 tc6 B={B} {prec}", flush=True)
     from mpas_sw_amd import _lib
     tc, dt, n = 6, 600.0, 3
     mesh, phys = case(LEVEL, tc)
     dh, du = perturbations(mesh, B)
-    m = model(LEVEL, tc, dt, batch=B, graphs=True, tl_fused_flux=True)
+    m = model(LEVEL, tc, dt, batch=B, precision=prec, graphs=True, tl_fused_flux=True)
     outs = []
     for k, blk in ((0, 256), (2, 256), (5, 256), (2, 64), (1, 128), (3, 512), (7, 256), (7, 64)):
         m.set_option(_lib.OPT_XCD_STRIPE, k)
