@@ -14,7 +14,7 @@
 // arithmetic, so a wavefront keeps many loads in flight instead of walking a
 // chain of dependent loads.  Padded terms add exact zeros.
 //
-// Blocks are mapped XCD-aware (stripe_block(): stripes of consecutive curve
+// Blocks are mapped XCD-aware (msw_launch.hpp: stripes of consecutive curve
 // positions dealt to the XCDs in turn).
 //
 // All gathers visit neighbours in the order the reference's scatter loops
@@ -23,6 +23,7 @@
 // tangent-linear results are the reference's to the bit.  Citations name the
 // reference loop each kernel restates.
 #include "msw_internal.hpp"
+#include "msw_launch.hpp"
 
 namespace msw {
 
@@ -42,55 +43,14 @@ __device__ __forceinline__ int entity_of(unsigned t, int lb)
     return e;
 }
 
-// Blocks are dealt round-robin to the 8 XCDs (block i runs on XCD i % 8).
-// Logical blocks are grouped in stripes of consecutive blocks (4 blocks = 16
-// entities at B = 64 by default) and the stripes are dealt to the XCDs in turn: each
-// XCD's L2 sees short runs of consecutive entities along the curve, and the
-// rows that neighbouring stripes share on different XCDs are caught by the
-// memory-side cache all XCDs share.  Measured against one contiguous range
-// per XCD and against plain block order, stripes of 4 were fastest (FP64,
-// B = 64, level 7: 12.68 vs 13.45 and 13.14 ms per TLM+ADJ step).
-// Every thread handles one item (one member of one entity); all index
-// arithmetic is 32-bit (items < 2^31 is checked when a context is made).
-// Stripe length 2^sl blocks (DMesh::stripe_log2, chosen per context; the
-// autotuner of the Python layer times the candidates).  Hardware block i
-// (i below a multiple of 8 * 2^sl) runs logical block stripe_block(i, sl).
-// sl == kRangePerXcd instead gives each XCD one contiguous range of the
-// segment's nseg blocks (a multiple of 8), walked in order.
-constexpr int kRangePerXcd = 7;
-
-__device__ __forceinline__ unsigned stripe_block(unsigned i, int sl, unsigned nseg)
-{
-    const unsigned x = i & 7u, j = i >> 3;
-    if (sl == kRangePerXcd) return x * (nseg >> 3) + j;
-    return ((j >> sl) << (3 + sl)) + (x << sl) + (j & ((1u << sl) - 1u));
-}
-
-__device__ __forceinline__ unsigned round_blocks_dev(unsigned nb, int sl)
-{
-    const unsigned q = sl == kRangePerXcd ? 8u : 8u << sl;
-    return ((nb + q - 1) / q) * q;
-}
-
-inline long round_blocks(long nb, int sl)
-{
-    const long q = sl == kRangePerXcd ? 8L : 8L << sl;
-    return ((nb + q - 1) / q) * q;
-}
-
-// blocks for `items` threads, rounded up to a multiple of 8 * 2^sl
-inline int grid_items(long items, int sl, int block)
-{
-    const long nb = (items + block - 1) / block;
-    return static_cast<int>(round_blocks(nb > 0 ? nb : 1, sl));
-}
-
-inline int grid_edge_cell(int nE, int nC, int lb, int sl, int block)
-{
-    const long be = ((static_cast<long>(nE) << lb) + block - 1) / block;
-    const long bc = ((static_cast<long>(nC) << lb) + block - 1) / block;
-    return static_cast<int>(round_blocks(be, sl) + round_blocks(bc, sl));
-}
+// Blocks are dealt to the XCDs in stripes of consecutive curve positions
+// (msw_launch.hpp); measured against one contiguous range per XCD and against
+// plain block order, stripes of 4 blocks were fastest (FP64, B = 64, level 7:
+// 12.68 vs 13.45 and 13.14 ms per TLM+ADJ step).  The stripe length is a
+// per-context option (DMesh::stripe_log2) that the Python layer's autotuner
+// times.
+using launch::grid_edge_cell;
+using launch::grid_items;
 
 // grid-stride form for the elementwise kernels (any grid)
 inline int grid_for(long threads)
@@ -103,27 +63,27 @@ inline int grid_for(long threads)
 }
 
 // one item per thread over `count` entities
-#define ENTITY_ITEM(count)                                                     \
-    const unsigned total_ = static_cast<unsigned>(count) << lb;                \
-    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2, gridDim.x) * static_cast<unsigned>(m.block) + threadIdx.x; \
-    if (t >= total_) return;
+#define ENTITY_ITEM(count)                                                                     \
+    const launch::Item it_ = launch::flat_item(blockIdx.x, threadIdx.x, gridDim.x, m.block,    \
+                                               m.stripe_log2, static_cast<unsigned>(count) << lb); \
+    if (!it_.live) return;                                                                     \
+    const unsigned t = it_.t;
 
-// one item per thread over the edges (is_edge: the first round_blocks(nE
-// items) blocks) or the cells (the rest)
-#define EDGE_CELL_ITEM(is_edge, t)                                             \
-    const unsigned tE_ = static_cast<unsigned>(m.nE) << lb;                    \
-    const unsigned tC_ = static_cast<unsigned>(m.nC) << lb;                    \
-    const unsigned bs_ = static_cast<unsigned>(m.block);                      \
-    const unsigned gE_ = round_blocks_dev((tE_ + bs_ - 1) / bs_, m.stripe_log2); \
-    const bool is_edge = blockIdx.x < gE_;                                     \
-    const unsigned t = stripe_block(is_edge ? blockIdx.x : blockIdx.x - gE_, m.stripe_log2,                  \
-                                    is_edge ? gE_ : gridDim.x - gE_) * bs_ + threadIdx.x;             \
-    if (t >= (is_edge ? tE_ : tC_)) return;
+// one item per thread over the edges (is_edge: the first segment of blocks)
+// or the cells (the rest)
+#define EDGE_CELL_ITEM(is_edge, t)                                                             \
+    const launch::Item it_ = launch::edge_cell_item(blockIdx.x, threadIdx.x, gridDim.x, m.block, \
+                                                    m.stripe_log2, m.nE, m.nC, lb);            \
+    if (!it_.live) return;                                                                     \
+    const bool is_edge = it_.is_edge;                                                          \
+    const unsigned t = it_.t;
 
 // one flat item per thread (no member axis), blocks dealt XCD-aware
-#define FLAT_ITEM(count)                                                       \
-    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2, gridDim.x) * static_cast<unsigned>(m.block) + threadIdx.x; \
-    if (t >= static_cast<unsigned>(count)) return;
+#define FLAT_ITEM(count)                                                                       \
+    const launch::Item it_ = launch::flat_item(blockIdx.x, threadIdx.x, gridDim.x, m.block,    \
+                                               m.stripe_log2, static_cast<unsigned>(count));   \
+    if (!it_.live) return;                                                                     \
+    const unsigned t = it_.t;
 
 #define GRID_LOOP(total)                                                               \
     for (long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x; t < (total); \
@@ -862,9 +822,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_coef_edge(DMesh m, DConfig c,
 {
     // 16 lanes per edge (all of them active through the reduction below):
     // lanes 0-9 fill the edgesOnEdge slots, lanes 12-15 the per-edge values
-    const unsigned t = stripe_block(blockIdx.x, m.stripe_log2, gridDim.x) * static_cast<unsigned>(m.block) + threadIdx.x;
-    const unsigned total = 16u * static_cast<unsigned>(m.nE);
-    if (t >= total) return;   // whole 16-lane groups only, so the shuffles stay inside live lanes
+    const launch::Item it_ = launch::flat_item(blockIdx.x, threadIdx.x, gridDim.x, m.block, m.stripe_log2,
+                                               16u * static_cast<unsigned>(m.nE));
+    if (!it_.live) return;   // whole 16-lane groups only, so the shuffles stay inside live lanes
+    const unsigned t = it_.t;
     const int e = static_cast<int>(t >> 4), j = static_cast<int>(t & 15);
     double term = 0.0;   // 0.5*w*u(eo)*he(eo), summed over the slots
     if (j < kEoE) {      // the member kernels read the 10 real slots only

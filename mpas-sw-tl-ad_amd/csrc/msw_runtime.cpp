@@ -117,14 +117,27 @@ struct msw_ctx {
     double* dpartial = nullptr;       // dot partials
     int dot_nblk = 0;
 
-    // graphs
+    // graphs, keyed by (kind * 16 + variant, the pointers they bake in);
+    // at most kMaxGraphs are kept, the least recently launched is dropped
+    struct GraphEntry {
+        hipGraphExec_t exec;
+        unsigned long last;
+    };
+    static constexpr size_t kMaxGraphs = 32;
     bool use_graphs = false;
     hipStream_t cap_stream = nullptr;
-    std::map<std::tuple<int, const void*, const void*, const void*, const void*>, hipGraphExec_t> graphs;
+    std::map<std::tuple<int, const void*, const void*, const void*, const void*>, GraphEntry> graphs;
+    unsigned long graph_clock = 0;
+
+    void clear_graphs()
+    {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second.exec);
+        graphs.clear();
+    }
 
     ~msw_ctx()
     {
-        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        clear_graphs();
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
         if (side) (void)hipStreamDestroy(side);
         for (auto& e : ev)
@@ -513,6 +526,19 @@ int join_side(msw_ctx* c)
     return MSW_OK;
 }
 
+// Run `work` between fork_side and join_side.  The side stream is joined back
+// also when `work` fails, so a graph capture in progress ends well formed and
+// the error is what the caller sees.
+template <typename F>
+int forked(msw_ctx* c, F&& work)
+{
+    int rc = fork_side(c);
+    if (rc) return rc;
+    rc = work();
+    const int jrc = join_side(c);
+    return rc ? rc : jrc;
+}
+
 // ---------------------------------------------------------------------------
 // drivers
 
@@ -583,8 +609,7 @@ int enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
     T* pe_tl = static_cast<T*>(c->me1);
     T* he_tl = static_cast<T*>(c->me2);
     hipStream_t ss = c->side;
-    int rc = fork_side(c);
-    if (rc) return rc;
+    return forked(c, [&]() -> int {
     // stage 0 reads a copy, so the base RK update may overwrite (h, u) while
     // the member kernels of earlier stages still run
     HIP_TRY(hipMemcpyAsync(c->st0, h, C * sizeof(double), hipMemcpyDeviceToDevice, ss));
@@ -611,7 +636,8 @@ int enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
         th = nb;
         tu = nb + C * B;
     }
-    return join_side(c);
+    return MSW_OK;
+    });
 }
 
 // Side stream: the four base stages of an adjoint step whose start state is
@@ -677,11 +703,10 @@ int main_adj_stages(msw_ctx* c, const BaseStage* st, const AdjCoef* qs, T* lamh,
 template <typename T>
 int enqueue_adj_step(msw_ctx* c, T* lamh, T* lamu, bool want_end)
 {
-    int rc = fork_side(c);
-    if (rc) return rc;
-    if ((rc = side_adj_base(c, c->st, c->qs, c->bx, want_end, true))) return rc;
-    if ((rc = main_adj_stages<T>(c, c->st, c->qs, lamh, lamu, true))) return rc;
-    return join_side(c);
+    return forked(c, [&]() -> int {
+        int rc = side_adj_base(c, c->st, c->qs, c->bx, want_end, true);
+        return rc ? rc : main_adj_stages<T>(c, c->st, c->qs, lamh, lamu, true);
+    });
 }
 
 // One step of a multi-step adjoint sweep: the member stages use base set
@@ -693,16 +718,16 @@ int enqueue_adj_step_piped(msw_ctx* c, int par, bool next, T* lamh, T* lamu)
 {
     BaseStage* cur = par ? c->stB : c->st;
     AdjCoef* qcur = par ? c->qsB : c->qs;
-    int rc = fork_side(c);
-    if (rc) return rc;
-    if (next) {
-        BaseStage* nst = par ? c->st : c->stB;
-        AdjCoef* nq = par ? c->qs : c->qsB;
-        double* nx = par ? c->bx : c->bxB;
-        if ((rc = side_adj_base(c, nst, nq, nx, false, false))) return rc;
-    }
-    if ((rc = main_adj_stages<T>(c, cur, qcur, lamh, lamu, false))) return rc;
-    return join_side(c);
+    return forked(c, [&]() -> int {
+        if (next) {
+            BaseStage* nst = par ? c->st : c->stB;
+            AdjCoef* nq = par ? c->qs : c->qsB;
+            double* nx = par ? c->bx : c->bxB;
+            const int rc = side_adj_base(c, nst, nq, nx, false, false);
+            if (rc) return rc;
+        }
+        return main_adj_stages<T>(c, cur, qcur, lamh, lamu, false);
+    });
 }
 
 int check_cfg(msw_ctx* c)
@@ -713,16 +738,28 @@ int check_cfg(msw_ctx* c)
     return MSW_OK;
 }
 
+// Copy an FP64 state (h [C], u [E]) on the context's stream.
+int store_state(msw_ctx* c, double* dh, double* du, const double* h, const double* u)
+{
+    HIP_TRY(hipMemcpyAsync(dh, h, static_cast<size_t>(c->nC) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(du, u, static_cast<size_t>(c->nE) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return MSW_OK;
+}
+
 // Run `body(v)` once per step, either directly or through a captured graph
 // keyed by (kind, variant, pointers); step i uses variant `variant(i)`
 // (0 when no variant function is given).
+// `pre(i)` enqueues step i's copies outside the graphs and returns a status.
 template <typename F>
 int run_steps_var(msw_ctx* c, int kind, const void* p0, const void* p1, const void* p2, const void* p3,
-                  int nsteps, F&& body, const std::function<void(int)>* pre,
+                  int nsteps, F&& body, const std::function<int(int)>* pre,
                   const std::function<int(int)>* variant)
 {
     for (int i = 0; i < nsteps; ++i) {
-        if (pre) (*pre)(i);
+        if (pre) {
+            const int prc = (*pre)(i);
+            if (prc) return prc;
+        }
         const int v = variant ? (*variant)(i) : 0;
         if (!c->use_graphs) {
             const int brc = body(v);
@@ -732,6 +769,15 @@ int run_steps_var(msw_ctx* c, int kind, const void* p0, const void* p1, const vo
         auto key = std::make_tuple(kind * 16 + v, p0, p1, p2, p3);
         auto it = c->graphs.find(key);
         if (it == c->graphs.end()) {
+            if (c->graphs.size() >= msw_ctx::kMaxGraphs) {
+                // drop the least recently launched graph once it has run
+                auto lru = c->graphs.begin();
+                for (auto g = c->graphs.begin(); g != c->graphs.end(); ++g)
+                    if (g->second.last < lru->second.last) lru = g;
+                HIP_TRY(hipStreamSynchronize(c->stream));
+                (void)hipGraphExecDestroy(lru->second.exec);
+                c->graphs.erase(lru);
+            }
             if (!c->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
             hipStream_t user = c->stream;
             c->stream = c->cap_stream;
@@ -752,9 +798,10 @@ int run_steps_var(msw_ctx* c, int kind, const void* p0, const void* p1, const vo
             e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if (e != hipSuccess) return fail(MSW_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-            it = c->graphs.emplace(key, exec).first;
+            it = c->graphs.emplace(key, msw_ctx::GraphEntry{exec, 0}).first;
         }
-        HIP_TRY(hipGraphLaunch(it->second, c->stream));
+        it->second.last = ++c->graph_clock;
+        HIP_TRY(hipGraphLaunch(it->second.exec, c->stream));
     }
     HIP_TRY(hipGetLastError());
     return MSW_OK;
@@ -762,7 +809,7 @@ int run_steps_var(msw_ctx* c, int kind, const void* p0, const void* p1, const vo
 
 template <typename F>
 int run_steps(msw_ctx* c, int kind, const void* p0, const void* p1, const void* p2, const void* p3,
-              int nsteps, F&& body, const std::function<void(int)>* pre = nullptr)
+              int nsteps, F&& body, const std::function<int(int)>* pre = nullptr)
 {
     return run_steps_var(c, kind, p0, p1, p2, p3, nsteps, [&](int) { return body(); }, pre, nullptr);
 }
@@ -860,8 +907,7 @@ int msw_set_config(msw_ctx* ctx, const msw_config* cfg)
     int rc = ensure_dissip(ctx);
     if (rc) return rc;
     set_dcfg(ctx);
-    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
-    ctx->graphs.clear();
+    ctx->clear_graphs();
     return MSW_OK;
 }
 
@@ -892,8 +938,7 @@ int msw_set_mesh_scaling(msw_ctx* ctx, const double* del2, const double* del4)
     }
     ctx->msd_on = del2 != nullptr;
     set_dcfg(ctx);
-    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
-    ctx->graphs.clear();
+    ctx->clear_graphs();
     return MSW_OK;
 }
 
@@ -912,8 +957,7 @@ int msw_set_option(msw_ctx* ctx, int option, int value)
     else if (option == MSW_OPT_XCD_STRIPE) ctx->dm.stripe_log2 = value;
     else ctx->dm.block = value;
     set_dcfg(ctx);
-    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
-    ctx->graphs.clear();
+    ctx->clear_graphs();
     return MSW_OK;
 }
 
@@ -952,10 +996,7 @@ int msw_base_steps(msw_ctx* ctx, double* h, double* u, int nsteps, double* traj_
     int rc = check_cfg(ctx);
     if (rc) return rc;
     const size_t C = ctx->nC, E = ctx->nE;
-    std::function<void(int)> pre = [&](int i) {
-        (void)hipMemcpyAsync(traj_h + i * C, h, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
-        (void)hipMemcpyAsync(traj_u + i * E, u, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
-    };
+    std::function<int(int)> pre = [&](int i) { return store_state(ctx, traj_h + i * C, traj_u + i * E, h, u); };
     return run_steps(ctx, 3, h, u, nullptr, nullptr, nsteps, [&] { enqueue_base_step(ctx, h, u); return MSW_OK; },
                      traj_h ? &pre : nullptr);
 }
@@ -970,11 +1011,8 @@ int msw_tlm_steps(msw_ctx* ctx, double* h, double* u, void* h_tl, void* u_tl, in
     int rc = check_cfg(ctx);
     if (rc) return rc;
     const size_t C = ctx->nC, E = ctx->nE;
-    std::function<void(int)> pre = [&](int i) {
-        (void)hipMemcpyAsync(traj_h + i * C, h, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
-        (void)hipMemcpyAsync(traj_u + i * E, u, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
-    };
-    const std::function<void(int)>* pp = traj_h ? &pre : nullptr;
+    std::function<int(int)> pre = [&](int i) { return store_state(ctx, traj_h + i * C, traj_u + i * E, h, u); };
+    const std::function<int(int)>* pp = traj_h ? &pre : nullptr;
     if (ctx->prec == MSW_FP64)
         return run_steps(ctx, 4, h, u, h_tl, u_tl, nsteps, [&] {
             return enqueue_tlm_step<double>(ctx, h, u, static_cast<double*>(h_tl), static_cast<double*>(u_tl));
@@ -985,7 +1023,7 @@ int msw_tlm_steps(msw_ctx* ctx, double* h, double* u, void* h_tl, void* u_tl, in
 }
 
 static int adj_common(msw_ctx* ctx, void* h_ad, void* u_ad, int nsteps, bool want_end,
-                      const std::function<void(int)>& pre)
+                      const std::function<int(int)>& pre)
 {
     if (!members_aligned(ctx, h_ad, u_ad)) return fail(MSW_EINVAL, "h_ad/u_ad must be 16-byte aligned");
     if (ctx->prec == MSW_FP64)
@@ -1005,10 +1043,7 @@ int msw_adj_step(msw_ctx* ctx, const double* h, const double* u, void* h_ad, voi
     int rc = check_cfg(ctx);
     if (rc) return rc;
     const size_t C = ctx->nC, E = ctx->nE;
-    std::function<void(int)> pre = [&](int) {
-        (void)hipMemcpyAsync(ctx->bx, h, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
-        (void)hipMemcpyAsync(ctx->bx + C, u, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
-    };
+    std::function<int(int)> pre = [&](int) { return store_state(ctx, ctx->bx, ctx->bx + C, h, u); };
     const bool want_end = h_end || u_end;
     rc = adj_common(ctx, h_ad, u_ad, 1, want_end, pre);
     if (rc) return rc;
@@ -1031,15 +1066,13 @@ int msw_adj_steps(msw_ctx* ctx, const double* traj_h, const double* traj_u, void
     // set i % 2, formed while step i-1 ran; before it, the start state of step
     // k-1 goes into the other set's buffer for the side stream to work on
     auto copy_start = [&](size_t k, double* dst) {
-        (void)hipMemcpyAsync(dst, traj_h + k * C, C * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
-        (void)hipMemcpyAsync(dst + C, traj_u + k * E, E * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+        return store_state(ctx, dst, dst + C, traj_h + k * C, traj_u + k * E);
     };
-    copy_start(static_cast<size_t>(nsteps - 1), ctx->bx);
-    if ((rc = fork_side(ctx))) return rc;
-    if ((rc = side_adj_base(ctx, ctx->st, ctx->qs, ctx->bx, false, false))) return rc;
-    if ((rc = join_side(ctx))) return rc;
-    std::function<void(int)> pre = [&](int i) {
-        if (i + 1 < nsteps) copy_start(static_cast<size_t>(nsteps - 2 - i), (i % 2) ? ctx->bx : ctx->bxB);
+    if ((rc = copy_start(static_cast<size_t>(nsteps - 1), ctx->bx))) return rc;
+    if ((rc = forked(ctx, [&] { return side_adj_base(ctx, ctx->st, ctx->qs, ctx->bx, false, false); }))) return rc;
+    std::function<int(int)> pre = [&](int i) {
+        return i + 1 < nsteps ? copy_start(static_cast<size_t>(nsteps - 2 - i), (i % 2) ? ctx->bx : ctx->bxB)
+                              : static_cast<int>(MSW_OK);
     };
     std::function<int(int)> variant = [&](int i) { return (i % 2) * 2 + (i + 1 < nsteps ? 1 : 0); };
     if (ctx->prec == MSW_FP64)

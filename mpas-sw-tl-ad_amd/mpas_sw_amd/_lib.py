@@ -11,6 +11,8 @@ import ctypes as C
 import os
 import subprocess
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(PKG_ROOT, "csrc")
@@ -80,6 +82,52 @@ class Config(C.Structure):
     ]
 
 
+def mesh_desc(mesh, phys, perm):
+    """msw_mesh_desc of a mesh and its scaled metrics (``phys``, as from
+    :func:`mpas_sw_amd.testcases.initial_conditions`), with the internal
+    order ``perm`` = {"c", "e", "v"} (new -> old, int32).  Returns the
+    descriptor and the host arrays it points to, which must outlive the
+    msw_create call that reads them."""
+    keep = []
+
+    def f8(x):
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data
+
+    def i4(x):
+        a = np.ascontiguousarray(x, dtype=np.int32)
+        keep.append(a)
+        return a.ctypes.data
+
+    d = MeshDesc()
+    d.nCells, d.nEdges, d.nVertices = mesh.nCells, mesh.nEdges, mesh.nVertices
+    d.maxEdges, d.maxEdges2, d.vertexDegree = mesh.maxEdges, mesh.maxEdges2, mesh.vertexDegree
+    d.cellsOnEdge = i4(mesh.cellsOnEdge.T)
+    d.verticesOnEdge = i4(mesh.verticesOnEdge.T)
+    d.nEdgesOnCell = i4(mesh.nEdgesOnCell)
+    d.edgesOnCell = i4(mesh.edgesOnCell.T)
+    d.nEdgesOnEdge = i4(mesh.nEdgesOnEdge)
+    d.edgesOnEdge = i4(mesh.edgesOnEdge.T)
+    d.weightsOnEdge = f8(mesh.weightsOnEdge.T)
+    d.cellsOnVertex = i4(mesh.cellsOnVertex.T)
+    kite = np.asarray(phys["kiteAreasOnVertex"])
+    d.kiteAreasOnVertex = f8(kite.T if kite.ndim == 2 else kite)
+    for name in ("dcEdge", "dvEdge", "areaCell", "areaTriangle", "fVertex", "h_s"):
+        setattr(d, name, f8(phys[name]))
+    d.latCell = f8(mesh.latCell)
+    d.lonCell = f8(phys.get("lonCell", mesh.lonCell))
+    cr = phys.get("coeffs_reconstruct")
+    if cr is not None:
+        cr = np.asarray(cr, np.float64)
+        # (3, maxEdges, C) reference layout -> entity-major flat
+        d.coeffs_reconstruct = f8(np.transpose(cr, (2, 1, 0)) if cr.ndim == 3 else cr)
+    d.cellPerm = i4(perm["c"])
+    d.edgePerm = i4(perm["e"])
+    d.vertexPerm = i4(perm["v"])
+    return d, keep
+
+
 # every symbol include/mpas_sw_amd.h declares
 EXPORTS = (
     "msw_create", "msw_destroy", "msw_set_config", "msw_set_forcing", "msw_set_mesh_scaling",
@@ -106,16 +154,17 @@ class MSWError(RuntimeError):
         self.code = code
 
 
-def load():
-    """Load the library (no GPU call happens here)."""
+def load(path=None):
+    """Load the library (no GPU call happens here).  ``path`` binds the same
+    signatures to another build of the C ABI (not cached)."""
     global _lib
-    if _lib is not None:
+    if path is None and _lib is not None:
         return _lib
-    if not os.path.isfile(LIB_PATH):
+    if path is None and not os.path.isfile(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
             "(the GPU path has no CPU fallback)")
-    lib = C.CDLL(LIB_PATH)
+    lib = C.CDLL(path or LIB_PATH)
     vp, i32, d, i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
     sig = {
         "msw_create": ([C.POINTER(MeshDesc), C.POINTER(Config), i32, i32, vp, C.POINTER(vp)], i32),
@@ -148,7 +197,8 @@ def load():
         f = getattr(lib, name)
         f.argtypes = argt
         f.restype = rest
-    _lib = lib
+    if path is None:
+        _lib = lib
     return lib
 
 

@@ -21,6 +21,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from ._lib import mesh_desc  # noqa: F401  (re-exported)
 from .mesh import Mesh, sfc_permutation
 
 _DT = {"fp64": (torch.float64, _lib.MSW_FP64), "fp32": (torch.float32, _lib.MSW_FP32)}
@@ -28,52 +29,6 @@ _DT = {"fp64": (torch.float64, _lib.MSW_FP64), "fp32": (torch.float32, _lib.MSW_
 
 def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
-
-
-def mesh_desc(mesh, phys, perm):
-    """msw_mesh_desc of a mesh and its scaled metrics (``phys``, as from
-    :func:`mpas_sw_amd.testcases.initial_conditions`), with the internal
-    order ``perm`` = {"c", "e", "v"} (new -> old, int32).  Returns the
-    descriptor and the host arrays it points to, which must outlive the
-    msw_create call that reads them."""
-    keep = []
-
-    def f8(x):
-        a = np.ascontiguousarray(x, dtype=np.float64)
-        keep.append(a)
-        return a.ctypes.data
-
-    def i4(x):
-        a = np.ascontiguousarray(x, dtype=np.int32)
-        keep.append(a)
-        return a.ctypes.data
-
-    d = _lib.MeshDesc()
-    d.nCells, d.nEdges, d.nVertices = mesh.nCells, mesh.nEdges, mesh.nVertices
-    d.maxEdges, d.maxEdges2, d.vertexDegree = mesh.maxEdges, mesh.maxEdges2, mesh.vertexDegree
-    d.cellsOnEdge = i4(mesh.cellsOnEdge.T)
-    d.verticesOnEdge = i4(mesh.verticesOnEdge.T)
-    d.nEdgesOnCell = i4(mesh.nEdgesOnCell)
-    d.edgesOnCell = i4(mesh.edgesOnCell.T)
-    d.nEdgesOnEdge = i4(mesh.nEdgesOnEdge)
-    d.edgesOnEdge = i4(mesh.edgesOnEdge.T)
-    d.weightsOnEdge = f8(mesh.weightsOnEdge.T)
-    d.cellsOnVertex = i4(mesh.cellsOnVertex.T)
-    kite = np.asarray(phys["kiteAreasOnVertex"])
-    d.kiteAreasOnVertex = f8(kite.T if kite.ndim == 2 else kite)
-    for name in ("dcEdge", "dvEdge", "areaCell", "areaTriangle", "fVertex", "h_s"):
-        setattr(d, name, f8(phys[name]))
-    d.latCell = f8(mesh.latCell)
-    d.lonCell = f8(phys.get("lonCell", mesh.lonCell))
-    cr = phys.get("coeffs_reconstruct")
-    if cr is not None:
-        cr = np.asarray(cr, np.float64)
-        # (3, maxEdges, C) reference layout -> entity-major flat
-        d.coeffs_reconstruct = f8(np.transpose(cr, (2, 1, 0)) if cr.ndim == 3 else cr)
-    d.cellPerm = i4(perm["c"])
-    d.edgePerm = i4(perm["e"])
-    d.vertexPerm = i4(perm["v"])
-    return d, keep
 
 
 class SWModel:

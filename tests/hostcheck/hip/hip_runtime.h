@@ -1,0 +1,76 @@
+// Host-check build of libmpas_sw_amd's runtime (msw_runtime.cpp): the part
+// of the HIP runtime API the runtime calls, implemented on the CPU by
+// mock_device.cpp.  "Device" memory is host memory from a registry that
+// checks every access; streams, events and graph capture keep HIP's
+// ordering rules, and kernel launches check the buffers they would touch.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+enum hipError_t {
+    hipSuccess = 0,
+    hipErrorInvalidValue = 1,
+    hipErrorOutOfMemory = 2,
+    hipErrorInvalidConfiguration = 9,
+    hipErrorInvalidResourceHandle = 400,
+    hipErrorLaunchFailure = 719,
+    hipErrorStreamCaptureUnsupported = 900,
+    hipErrorStreamCaptureInvalidated = 901,
+    hipErrorStreamCaptureUnjoined = 904,
+    hipErrorStreamCaptureIsolation = 905,
+    hipErrorCapturedEvent = 907,
+};
+
+enum hipMemcpyKind {
+    hipMemcpyHostToHost = 0,
+    hipMemcpyHostToDevice = 1,
+    hipMemcpyDeviceToHost = 2,
+    hipMemcpyDeviceToDevice = 3,
+    hipMemcpyDefault = 4,
+};
+
+enum hipStreamCaptureMode {
+    hipStreamCaptureModeGlobal = 0,
+    hipStreamCaptureModeThreadLocal = 1,
+    hipStreamCaptureModeRelaxed = 2,
+};
+
+#define hipStreamDefault 0x0
+#define hipStreamNonBlocking 0x1
+#define hipEventDefault 0x0
+#define hipEventDisableTiming 0x2
+
+struct ihipStream_t;
+struct ihipEvent_t;
+struct ihipGraph_t;
+struct ihipGraphExec_t;
+struct ihipGraphNode_t;
+typedef ihipStream_t* hipStream_t;
+typedef ihipEvent_t* hipEvent_t;
+typedef ihipGraph_t* hipGraph_t;
+typedef ihipGraphExec_t* hipGraphExec_t;
+typedef ihipGraphNode_t* hipGraphNode_t;
+
+hipError_t hipMalloc(void** p, size_t bytes);
+hipError_t hipFree(void* p);
+hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
+hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
+hipError_t hipMemset(void* p, int v, size_t bytes);
+hipError_t hipMemsetAsync(void* p, int v, size_t bytes, hipStream_t s);
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned flags);
+hipError_t hipStreamDestroy(hipStream_t s);
+hipError_t hipStreamSynchronize(hipStream_t s);
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
+hipError_t hipEventDestroy(hipEvent_t e);
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags);
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode mode);
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g);
+hipError_t hipGraphInstantiate(hipGraphExec_t* x, hipGraph_t g, hipGraphNode_t* err_node, char* log, size_t log_size);
+hipError_t hipGraphDestroy(hipGraph_t g);
+hipError_t hipGraphExecDestroy(hipGraphExec_t x);
+hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t s);
+hipError_t hipGetLastError(void);
+const char* hipGetErrorString(hipError_t e);
+hipError_t hipDeviceSynchronize(void);
