@@ -464,12 +464,7 @@ def test_invariants_match_oracle_and_mass_is_conserved():
     assert abs(inv[0] - m0[0]) <= 1e-13 * abs(m0[0])
 
 
-@pytest.mark.parametrize("B,prec", [
-    (4, "fp64"), (64, "fp64"),
-    pytest.param(64, "fp32", marks=pytest.mark.skip(
-        reason="ended in a host segmentation fault inside msw_adj_steps in a full GPU run "
-               "(it passed alone); the cause is not found yet, so it stays off until it is")),
-])
+@pytest.mark.parametrize("B,prec", [(4, "fp64"), (64, "fp64"), (64, "fp32")])
 def test_launch_stripe_does_not_change_results_and_autotune(B, prec):
     """The block-to-XCD stripe and the block size only reorder work: TLM and
     adjoint results are bitwise the same for every stripe and block size;
