@@ -43,8 +43,28 @@ MSW_EHIP = -3
 MSW_EUNSUPPORTED = -4
 
 
+HOSTMATH_PATH = os.path.join(HERE, "libmsw_hostmath.so")
+
+
+def build_hostmath() -> str:
+    """Compile the host libm loops of the initial conditions (gcc -O2, no
+    fast-math) in-tree; returns the path."""
+    cc = os.environ.get("CC", "gcc")
+    tmp = HOSTMATH_PATH + ".tmp"
+    # -fno-builtin: the libm calls as written (GCC would otherwise split
+    # sincos into sin and cos, whose results can differ in the last bit)
+    cmd = [cc, "-O2", "-fno-builtin", "-fPIC", "-shared", "-o", tmp, os.path.join(CSRC, "msw_hostmath.c"), "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("gcc failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, HOSTMATH_PATH)
+    return HOSTMATH_PATH
+
+
 def build(verbose: bool = False) -> str:
-    """Compile the HIP library in-tree; returns its path."""
+    """Compile the HIP library (and the host libm loops) in-tree; returns the
+    HIP library's path."""
+    build_hostmath()
     hipcc = os.environ.get("HIPCC", "hipcc")
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB_PATH + ".tmp"

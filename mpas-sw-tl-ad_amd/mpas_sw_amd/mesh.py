@@ -70,6 +70,15 @@ def _tri_area(a, b, c):
     return 2.0 * np.arctan2(num, den)
 
 
+def _unique_pairs(es, n):
+    """np.unique(es, axis=0, return_inverse=True) for rows (a, b) of point
+    indices with a < b < n: the same unique rows in the same (lexicographic)
+    order and the same inverse, from a 1-D key a * n + b."""
+    key = es[:, 0].astype(np.int64) * n + es[:, 1]
+    uk, inv = np.unique(key, return_inverse=True)
+    return np.stack([uk // n, uk % n], 1), inv
+
+
 def _icosahedron():
     p = (1.0 + 5.0 ** 0.5) / 2.0
     v = np.array(
@@ -94,7 +103,7 @@ def _bisect(v, f):
     nf = f.shape[0]
     e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]], axis=0)
     es = np.sort(e, axis=1)
-    uniq, inv = np.unique(es, axis=0, return_inverse=True)
+    uniq, inv = _unique_pairs(es, v.shape[0])
     inv = inv.reshape(-1)
     mid = _normalize(v[uniq[:, 0]] + v[uniq[:, 1]])
     nv = v.shape[0]
@@ -236,7 +245,7 @@ def icosahedral_mesh(level: int) -> Mesh:
     te = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]], axis=0)  # directed, ccw in tri
     tri_of = np.concatenate([np.arange(V)] * 3)
     es = np.sort(te, axis=1)
-    uniq, inv = np.unique(es, axis=0, return_inverse=True)
+    uniq, inv = _unique_pairs(es, C)
     inv = inv.reshape(-1)
     E = uniq.shape[0]
     assert E == 30 * 4 ** level and C == 10 * 4 ** level + 2 and V == 20 * 4 ** level
@@ -390,37 +399,42 @@ def icosahedral_mesh(level: int) -> Mesh:
 
 
 def _hilbert3d_key(x, bits=16):
-    """Skilling's transpose algorithm: 3-D Hilbert index of points in [-1,1]^3."""
-    q = np.clip(((x + 1.0) * 0.5 * (2 ** bits - 1)).round(), 0, 2 ** bits - 1).astype(np.uint64)
+    """Skilling's transpose algorithm: 3-D Hilbert index of points in [-1,1]^3.
+
+    The coordinates (bits <= 21 each) are transformed in uint32 with
+    branch-free masks (m * P for "where m, xor P"); the 3*bits-bit index is
+    interleaved in uint64."""
+    assert bits <= 21
+    q = np.clip(((x + 1.0) * 0.5 * (2 ** bits - 1)).round(), 0, 2 ** bits - 1).astype(np.uint32)
     X = [q[:, 0].copy(), q[:, 1].copy(), q[:, 2].copy()]
     n = 3
-    M = np.uint64(1) << np.uint64(bits - 1)
+    M = np.uint32(1 << (bits - 1))
     Q = M
     while Q > 1:
-        P = Q - np.uint64(1)
+        P = np.uint32(Q - 1)
         for i in range(n):
-            m = (X[i] & Q) != 0
-            # invert
-            X[0] = np.where(m, X[0] ^ P, X[0])
-            # exchange
-            t = np.where(m, np.uint64(0), (X[0] ^ X[i]) & P)
-            X[0] = X[0] ^ t
-            X[i] = X[i] ^ t
-        Q = Q >> np.uint64(1)
+            m = ((X[i] & Q) != 0).view(np.uint8).astype(np.uint32)
+            # invert where the bit is set
+            X[0] ^= m * P
+            # exchange where it is not
+            t = ((X[0] ^ X[i]) & P) * (np.uint32(1) - m)
+            X[0] ^= t
+            X[i] ^= t
+        Q = np.uint32(Q >> 1)
     for i in range(1, n):
-        X[i] = X[i] ^ X[i - 1]
+        X[i] ^= X[i - 1]
     t = np.zeros_like(X[0])
     Q = M
     while Q > 1:
-        m = (X[n - 1] & Q) != 0
-        t = np.where(m, t ^ (Q - np.uint64(1)), t)
-        Q = Q >> np.uint64(1)
+        m = ((X[n - 1] & Q) != 0).view(np.uint8).astype(np.uint32)
+        t ^= m * np.uint32(Q - 1)
+        Q = np.uint32(Q >> 1)
     for i in range(n):
-        X[i] = X[i] ^ t
-    key = np.zeros_like(X[0])
+        X[i] ^= t
+    key = np.zeros(X[0].shape, dtype=np.uint64)
     for b in range(bits - 1, -1, -1):
         for i in range(n):
-            key = (key << np.uint64(1)) | ((X[i] >> np.uint64(b)) & np.uint64(1))
+            key = (key << np.uint64(1)) | ((X[i] >> np.uint32(b)) & np.uint32(1)).astype(np.uint64)
     return key
 
 
@@ -498,8 +512,11 @@ def union_mesh(meshes):
 def union_sfc_permutation(meshes, offsets):
     """Per-instance space-filling-curve order, instances kept contiguous."""
     pcs, pes, pvs = [], [], []
+    done = {}   # instances of one mesh share its curve order
     for m, (oc, oe, ov) in zip(meshes, offsets):
-        pc, pe, pv = sfc_permutation(m)
+        if id(m) not in done:
+            done[id(m)] = sfc_permutation(m)
+        pc, pe, pv = done[id(m)]
         pcs.append(pc + oc)
         pes.append(pe + oe)
         pvs.append(pv + ov)

@@ -18,6 +18,7 @@ from __future__ import annotations
 import ctypes
 import ctypes.util
 import math
+import os
 
 import numpy as np
 
@@ -32,14 +33,36 @@ _LIBM.sincos.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                          ctypes.POINTER(ctypes.c_double)]
 _LIBM.sincos.restype = None
 
+# The same libm calls in C loops (csrc/msw_hostmath.c, built next to this
+# file): one call per element, so the same bits as the per-element Python
+# path below, which is used when the helper is not built.
+_VEC = None
+_HOSTMATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmsw_hostmath.so")
+if os.path.isfile(_HOSTMATH):
+    _VEC = ctypes.CDLL(_HOSTMATH)
+    _dp = ctypes.POINTER(ctypes.c_double)
+    _VEC.msw_vsincos.argtypes = [ctypes.c_long, _dp, _dp, _dp]
+    _VEC.msw_vsin.argtypes = [ctypes.c_long, _dp, _dp]
+    _VEC.msw_vcos.argtypes = [ctypes.c_long, _dp, _dp]
+    _VEC.msw_vpow.argtypes = [ctypes.c_long, _dp, ctypes.c_double, _dp]
+    for _fn in ("msw_vsincos", "msw_vsin", "msw_vcos", "msw_vpow"):
+        getattr(_VEC, _fn).restype = None
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
 
 def _sincos(x):
     """(sin x, cos x) through libm's sincos: where the reference evaluates both
     of one argument its compiler emits a single sincos call, whose results can
     differ from separate sin and cos calls in the last bit."""
-    x = np.asarray(x, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1)
     s = np.empty(x.size)
     c = np.empty(x.size)
+    if _VEC is not None:
+        _VEC.msw_vsincos(x.size, _dptr(x), _dptr(s), _dptr(c))
+        return s, c
     ps, pc = ctypes.c_double(), ctypes.c_double()
     f = _LIBM.sincos
     for i, t in enumerate(x.tolist()):
@@ -53,17 +76,31 @@ def _f(fn, x):
     return np.fromiter(map(fn, np.asarray(x, np.float64).tolist()), np.float64, count=np.size(x))
 
 
+def _vec1(name, fn, x):
+    if _VEC is None:
+        return _f(fn, x)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1)
+    out = np.empty(x.size)
+    getattr(_VEC, name)(x.size, _dptr(x), _dptr(out))
+    return out
+
+
 def _sin(x):
-    return _f(math.sin, x)
+    return _vec1("msw_vsin", math.sin, x)
 
 
 def _cos(x):
-    return _f(math.cos, x)
+    return _vec1("msw_vcos", math.cos, x)
 
 
 def _pow(x, y):
-    return np.fromiter((math.pow(t, y) for t in np.asarray(x, np.float64).tolist()), np.float64,
-                       count=np.size(x))
+    if _VEC is None:
+        return np.fromiter((math.pow(t, y) for t in np.asarray(x, np.float64).tolist()), np.float64,
+                           count=np.size(x))
+    x = np.ascontiguousarray(x, np.float64).reshape(-1)
+    out = np.empty(x.size)
+    _VEC.msw_vpow(x.size, _dptr(x), float(y), _dptr(out))
+    return out
 
 
 def scale_mesh(mesh):

@@ -182,6 +182,17 @@ class Oracle:
         a = [np.ascontiguousarray(x, np.float64) for x in (h, h_tl, u, u_tl)]
         self._lib.mswo_tend_tlm(self._h, *[_ptr(x, _d) for x in a])
 
+    def reconstruct(self, u):
+        """mpas_reconstruct_2d of u into self["uReconstruct*"]."""
+        u = np.ascontiguousarray(u, np.float64)
+        self._lib.mswo_reconstruct(self._h, _ptr(u, _d))
+
+    def reconstruct_tlm(self, u, u_tl):
+        """mpas_reconstruct_2d_tlm: self["uReconstruct*"] and self["uReconstruct*_tl"]."""
+        u = np.ascontiguousarray(u, np.float64)
+        u_tl = np.ascontiguousarray(u_tl, np.float64)
+        self._lib.mswo_reconstruct_tlm(self._h, _ptr(u, _d), _ptr(u_tl, _d))
+
     def diag_adj(self, h, u):
         """sw_compute_solve_diagnostics_adj at (h,u) applied to the seeds in
         the diagnostic adjoint arrays (self["<name>_ad"]), which it zeroes:

@@ -232,6 +232,21 @@ def quirk_cases():
                                                digests=dig), dh=dh, du=du)
 
 
+def recon_tl_case():
+    """Level 3, TC6, dt 600 s, 5 TLM steps: the reconstructed TL velocity
+    (uReconstructZonal_tl, uReconstructMeridional_tl) that sw_rk4_tlm leaves
+    after its last mpas_reconstruct_2d_tlm (src/mpas_sw_module.f90:767), with
+    the TL state it was made from."""
+    m3 = get_mesh(3)
+    tc, dt, n = 6, 600.0, 5
+    dh, du = perturb(m3, SEED + 7)
+    ic = ref.run(m3, "ic", test_case=tc, dt=dt)
+    t = ref.run(m3, "tlm", nsteps=n, test_case=tc, dt=dt, tl=(dh, du))
+    save("tc6_l3_recon_tl", m3, dict(test_case=tc, dt=dt, nsteps=n), h0=ic["h"], u0=ic["u"], dh=dh, du=du,
+         h_tl=t["h_tl"], u_tl=t["u_tl"], zonal_tl=t["uReconstructZonal_tl"],
+         meridional_tl=t["uReconstructMeridional_tl"])
+
+
 def main(names):
     assert ref.available(), "build the reference first: make -C oracle ref"
     if names:
@@ -240,7 +255,7 @@ def main(names):
              "tc5_l6_1day_digest": level6_day_digest, "tc6_l7_window_digest": level7_window_digest,
              "l7_ic_digest": level7_ic_digest, "quirks": quirk_cases,
              "tc6_l7_1day_digest": level7_day_digest, "tc5_l3_diag_seeds": diag_seed_case,
-             "tc5_l3_mesh_scaling": mesh_scaling_case}[n]()
+             "tc5_l3_mesh_scaling": mesh_scaling_case, "tc6_l3_recon_tl": recon_tl_case}[n]()
         return
 
     # 1. Williamson TC2, level 5, dt = 300 s, 288 steps (one day)
@@ -317,6 +332,9 @@ def main(names):
 
     # 13. del2/del4 scaled with a non-uniform mesh density
     mesh_scaling_case()
+
+    # 14. reconstructed TL velocity after sw_rk4_tlm
+    recon_tl_case()
 
 
 if __name__ == "__main__":

@@ -114,6 +114,32 @@ def test_facade_steps_on_gpu():
 
 
 @pytest.mark.gpu
+def test_facade_tl_reconstruction_against_reference_fortran():
+    """mpsw.ureconstruct{zonal,meridional}_tl after sw_rk4_tlm, which the
+    reference's TLM demo writes at every output time (demo/run-tlm.py:38-39),
+    equal to the reference Fortran's (5 TC6 steps on level 3)."""
+    from helpers import golden
+    print("facade TL reconstruction L3 tc6", flush=True)
+    g, meta = golden("tc6_l3_recon_tl")
+    mesh, _ = case(3, 6)
+    mpsw = fresh_module()
+    load_mesh(mpsw, mesh)
+    mpsw.config_test_case = 6
+    mpsw.config_dt = meta["dt"]
+    mpsw.sw_test_case_6()
+    mpsw.var_allocation_tlm()
+    mpsw.sw_mpas_init_block()
+    np.testing.assert_array_equal(mpsw.h[0, 0], g["h0"])
+    mpsw.h_tl[0, 0] = g["dh"]
+    mpsw.u_tl[0, 0] = g["du"]
+    for _ in range(meta["nsteps"]):
+        mpsw.sw_rk4_tlm()
+    np.testing.assert_array_equal(mpsw.u_tl[0, 0], g["u_tl"])
+    np.testing.assert_array_equal(mpsw.ureconstructzonal_tl[0], g["zonal_tl"])
+    np.testing.assert_array_equal(mpsw.ureconstructmeridional_tl[0], g["meridional_tl"])
+
+
+@pytest.mark.gpu
 def test_facade_diagnostic_adjoint_seeds_against_reference_fortran():
     """The reference's adjoint flow (ref run-adj: forward storing each step's
     start state, then sw_rk4_adj backwards) with seeds in the end-state

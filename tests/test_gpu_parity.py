@@ -504,6 +504,30 @@ def test_launch_stripe_does_not_change_results_and_autotune(B, prec):
         m.set_option(_lib.OPT_BLOCK, 96)
 
 
+@pytest.mark.parametrize("B,prec", [(64, "fp64"), (256, "fp64"), (64, "fp32")])
+def test_batched_reconstruction_matches_oracle(B, prec):
+    """msw_reconstruct of B members at once (the operator sw_rk4_tlm applies
+    to u_tl, src/mpas_sw_dyncore_tlm.f90:772-870): every sampled member's X,
+    Y, Z, zonal and meridional components equal to the oracle's
+    reconstruction of that member, bit for bit in FP64, 1e-6 relative in FP32."""
+    print(f"batched reconstruction L3 tc6 B={B} {prec}", flush=True)
+    mesh, phys = case(LEVEL, 6)
+    dh, du = perturbations(mesh, B)
+    m = model(LEVEL, 6, 600.0, batch=B, precision=prec)
+    out = m.reconstruct(m.to_internal(du, "e"))
+    torch.cuda.synchronize()
+    o = make_oracle(LEVEL, 6, 600.0)
+    names = ("uReconstructX", "uReconstructY", "uReconstructZ", "uReconstructZonal", "uReconstructMeridional")
+    for b in sorted({0, 1, 31, 32, 33, 63, B // 2 + 5, B - 1}):
+        o.reconstruct(du[:, b])
+        for i, n in enumerate(names):
+            got = m.to_reference(out[i][:, b], "c")
+            if prec == "fp64":
+                np.testing.assert_array_equal(got, o[n], err_msg=f"{n} member {b}")
+            else:
+                assert rel_l2(got, o[n]) < 1e-6, (n, b)
+
+
 @pytest.mark.parametrize("B", [64, 256])
 def test_wave_per_entity_batch_matches_oracle(B):
     """B = 64 (a wavefront per entity: connectivity and coefficients are

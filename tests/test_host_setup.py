@@ -42,3 +42,26 @@ def test_level7_initial_conditions_digests():
         phys = initial_conditions(mesh, tc)
         for k in ("h", "u", "h_s", "fEdge", "fVertex", "lonCell"):
             assert _sha(phys[k]) == meta[f"sha_tc{tc}_{k}"], (tc, k)
+
+
+def test_libm_loops_equal_per_element_calls():
+    """The C loops of the initial conditions (csrc/msw_hostmath.c) give the
+    bits of one libm call per element, the path used without them."""
+    import ctypes
+    import math
+
+    from mpas_sw_amd import testcases as T
+    assert T._VEC is not None, "libmsw_hostmath.so is not built"
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(-7.0, 7.0, 20000), rng.uniform(-1e-3, 1e-3, 2000),
+                        [0.0, -0.0, math.pi / 2, math.pi, 1e-300, 710.0]])
+    s, c = T._sincos(x)
+    ps, pc = ctypes.c_double(), ctypes.c_double()
+    for i, t in enumerate(x.tolist()):
+        T._LIBM.sincos(t, ctypes.byref(ps), ctypes.byref(pc))
+        assert s[i] == ps.value and c[i] == pc.value, t
+    np.testing.assert_array_equal(T._sin(x), np.array([math.sin(t) for t in x.tolist()]))
+    np.testing.assert_array_equal(T._cos(x), np.array([math.cos(t) for t in x.tolist()]))
+    xp = np.abs(x) + 0.5
+    for y in (4.0, 8.0, -2.0, 2.0, 5.0 / 12.0):
+        np.testing.assert_array_equal(T._pow(xp, y), np.array([math.pow(t, y) for t in xp.tolist()]))

@@ -82,6 +82,25 @@ def test_reconstruction_bitwise():
     np.testing.assert_array_equal(o["uReconstructMeridional"], g["tc2_uReconstructMeridional"])
 
 
+def test_tl_reconstruction_bitwise():
+    """The reconstructed TL velocity sw_rk4_tlm leaves after its last
+    mpas_reconstruct_2d_tlm (src/mpas_sw_dyncore_tlm.f90:772-870), 5 steps of
+    TC6 on level 3; and the same operator applied directly to that TL state."""
+    g, meta = golden("tc6_l3_recon_tl")
+    o = _oracle_from(3, 6, meta["dt"], g["h0"], g["u0"])
+    o["h_tl1"][:] = g["dh"]
+    o["u_tl1"][:] = g["du"]
+    o.rk4_tlm(meta["nsteps"])
+    np.testing.assert_array_equal(o["h_tl1"], g["h_tl"])
+    np.testing.assert_array_equal(o["u_tl1"], g["u_tl"])
+    np.testing.assert_array_equal(o["uReconstructZonal_tl"], g["zonal_tl"])
+    np.testing.assert_array_equal(o["uReconstructMeridional_tl"], g["meridional_tl"])
+    # linear in u_tl and independent of u: the forward operator on u_tl
+    o.reconstruct(g["u_tl"])
+    np.testing.assert_array_equal(o["uReconstructZonal"], g["zonal_tl"])
+    np.testing.assert_array_equal(o["uReconstructMeridional"], g["meridional_tl"])
+
+
 def test_dissipation_and_forcing_bitwise():
     """del2 + del4 viscosity, bottom drag and wind stress (all on): forward,
     TLM and adjoint windows and the adjoint of one tendency evaluation."""
