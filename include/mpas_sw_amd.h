@@ -203,6 +203,18 @@ int msw_reconstruct_adj(msw_ctx *ctx, const void *seeds5, void *u_ad);
 int msw_dot(msw_ctx *ctx, const void *xh, const void *xu, const void *yh, const void *yu,
             const double *wh, const double *wu, double *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): msw_dot for
+ * members of a mesh with n_cells cells and n_edges edges, without a context,
+ * ordered on hip_stream (NULL: the null stream) and bitwise equal to msw_dot
+ * on a context of that mesh, batch and precision.  It allocates nothing: the
+ * partial sums live in a 4 MiB device array of the library (one per GPU,
+ * held until the process ends), so batch is at most 256 (MSW_EINVAL above).
+ * Calls that overlap on one GPU share that array and race. */
+int msw_dot_noalloc(int n_cells, int n_edges, int batch, int prec, const void *xh, const void *xu,
+                    const void *yh, const void *yu, const double *wh, const double *wu, double *out,
+                    void *hip_stream);
+
 /* Per-member non-finite detector: flags[b] = 1 if any NaN/Inf in member b
  * of (h,u); flags is a device int[B] the caller zeroes. */
 int msw_nonfinite(msw_ctx *ctx, const void *h, const void *u, int *flags);

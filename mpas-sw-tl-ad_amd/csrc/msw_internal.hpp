@@ -226,6 +226,12 @@ template <typename T>
 void dot_batched(long n, int lb, const T* x, const T* y, const double* w, double* partial,
                  int nblk, double* out, int accumulate, hipStream_t s);
 int dot_blocks(long n);
+// Partials of the allocation-free inner product (msw_dot_noalloc) live in a
+// file-scope device array of msw_kernels.hip: dot_blocks(n) <= 2048 blocks of
+// up to 256 members.  dot_scratch returns this device's address of it
+// (looked up once per device, with that device current): 0 or a hipError_t.
+constexpr long kDotScratchDoubles = 2048L * 256L;
+int dot_scratch(double** out);
 
 // copies and conversions
 template <typename TD, typename TS>

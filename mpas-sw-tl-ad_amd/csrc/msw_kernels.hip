@@ -25,6 +25,9 @@
 #include "msw_internal.hpp"
 #include "msw_launch.hpp"
 
+#include <map>
+#include <mutex>
+
 namespace msw {
 
 namespace {
@@ -1712,6 +1715,33 @@ void reconstruct(int nC, int lb, const int* nEoC, const int* eoc, const double* 
 {
     LAUNCH_WU(k_reconstruct, T, lb, grid_for(static_cast<long>(nC) << lb), kBlock, s, nC, lb, nEoC, eoc, coeffs, trig,
               u, out5);
+}
+
+// Device memory of the allocation-free inner product msw_dot_noalloc: FP64
+// partials of up to 2048 blocks x 256 members (4 MiB), one copy per GPU, held
+// until the process ends.  Every call on a GPU shares it; calls that overlap
+// on one GPU race.  Each call writes every partial it reads, so nothing in it
+// has to start at zero.  Host code uses only the per-device address that
+// hipGetSymbolAddress returns, never the array's name.
+static __device__ __align__(256) double scratch[kDotScratchDoubles];
+
+int dot_scratch(double** out)
+{
+    static std::mutex mu;
+    static std::map<int, double*> addr;   // device -> address of its scratch
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return static_cast<int>(e);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(dev);
+    if (it == addr.end()) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return static_cast<int>(e);
+        it = addr.emplace(dev, static_cast<double*>(p)).first;
+    }
+    *out = it->second;
+    return 0;
 }
 
 int dot_blocks(long n)

@@ -1241,6 +1241,41 @@ int msw_dot(msw_ctx* ctx, const void* xh, const void* xu, const void* yh, const 
     return MSW_OK;
 }
 
+int msw_dot_noalloc(int n_cells, int n_edges, int batch, int prec, const void* xh, const void* xu,
+                    const void* yh, const void* yu, const double* wh, const double* wu, double* out,
+                    void* hip_stream)
+{
+    MSW_TRACE();
+    if (!out || (!xh && !xu) || n_cells < 0 || n_edges < 0) return fail(MSW_EINVAL, "bad argument");
+    if (prec != MSW_FP64 && prec != MSW_FP32) return fail(MSW_EINVAL, "prec must be MSW_FP64 or MSW_FP32");
+    if (!is_pow2(batch) || batch > 256)
+        return fail(MSW_EINVAL, "batch must be a power of two in [1,256] (the scratch holds 2048 x 256 "
+                                "FP64 partials)");
+    int lb = 0;
+    while ((1 << lb) < batch) ++lb;
+    // the block split msw_dot uses for a mesh of these sizes
+    const int nblk = dot_blocks(std::max<long>(n_cells, n_edges));
+    if (static_cast<long>(nblk) * batch > kDotScratchDoubles) return fail(MSW_EINVAL, "too many partials");
+    double* partial = nullptr;
+    const int e = dot_scratch(&partial);
+    if (e) return fail(MSW_EHIP, std::string("scratch address: ") + hipGetErrorString(static_cast<hipError_t>(e)));
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    bool acc = false;
+    auto run = [&](long n, const void* x, const void* y, const double* w) {
+        if (prec == MSW_FP64)
+            dot_batched<double>(n, lb, static_cast<const double*>(x), static_cast<const double*>(y), w, partial,
+                                nblk, out, acc ? 1 : 0, s);
+        else
+            dot_batched<float>(n, lb, static_cast<const float*>(x), static_cast<const float*>(y), w, partial,
+                               nblk, out, acc ? 1 : 0, s);
+        acc = true;
+    };
+    if (xh) run(n_cells, xh, yh, wh);
+    if (xu) run(n_edges, xu, yu, wu);
+    HIP_TRY(hipGetLastError());
+    return MSW_OK;
+}
+
 int msw_nonfinite(msw_ctx* ctx, const void* h, const void* u, int* flags)
 {
     MSW_TRACE();

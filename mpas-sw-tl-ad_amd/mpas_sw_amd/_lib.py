@@ -156,7 +156,7 @@ EXPORTS = (
     "msw_base_steps", "msw_tlm_steps", "msw_adj_step", "msw_adj_steps", "msw_diagnostics",
     "msw_diagnostics_adj",
     "msw_diag_names", "msw_diag_kinds", "msw_reconstruct", "msw_reconstruct_adj", "msw_invariants", "msw_dot", "msw_nonfinite",
-    "msw_permute", "msw_device_bytes", "msw_abi_version", "msw_last_error",
+    "msw_permute", "msw_device_bytes", "msw_abi_version", "msw_last_error", "msw_dot_noalloc",
 )
 
 OPT_TL_FUSED_FLUX = 1   # msw_option
@@ -208,6 +208,7 @@ def load(path=None):
         "msw_invariants": ([vp, vp, vp, vp], i32),
         "msw_dot": ([vp, vp, vp, vp, vp, vp, vp, vp], i32),
         "msw_nonfinite": ([vp, vp, vp, vp], i32),
+        "msw_dot_noalloc": ([i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp], i32),
         "msw_permute": ([vp, i32, i32, i32, i32, vp, vp], i32),
         "msw_device_bytes": ([vp], i64),
         "msw_abi_version": ([], i32),

@@ -163,6 +163,8 @@ def other_entry_points(prec, B, graphs):
     wh, wu = dev(8 * nC), dev(8 * nE)
     ok(lib.msw_dot(ctx, s["ht"], s["ut"], s["ht"], s["ut"], wh, wu, out), "msw_dot")
     ok(lib.msw_dot(ctx, s["ht"], None, None, None, None, None, out), "msw_dot h only")
+    ok(lib.msw_dot_noalloc(nC, nE, B, prec, s["ht"], s["ut"], s["ht"], s["ut"], wh, wu, out, None),
+       "msw_dot_noalloc")
     flags = dev(4 * B)
     ok(lib.msw_nonfinite(ctx, s["ht"], s["ut"], flags), "msw_nonfinite")
     tmp = dev(es * B * nE)

@@ -591,6 +591,14 @@ int dot_blocks(long n)
     return static_cast<int>(nb);
 }
 
+int dot_scratch(double** out)
+{
+    static void* p = nullptr;   // the library's file-scope array, registered once
+    if (!p && hipMalloc(&p, kDotScratchDoubles * sizeof(double)) != hipSuccess) return hipErrorOutOfMemory;
+    *out = static_cast<double*>(p);
+    return 0;
+}
+
 namespace {
 
 const size_t D = sizeof(double), I = sizeof(int);
