@@ -159,8 +159,12 @@ def run_sw_fwd(nmlname='namelist.sw.x1.10242'):
                   [ulist, vlist, hlist, unorm])
 
 
-def run_sw_tlm(pt=1.E-3, nmlname='namelist.sw.x1.10242'):
-    """src/mpas_sw_driver.py:311-332"""
+def run_sw_tlm(pt=1.E-3, nmlname='namelist.sw.x1.10242', write_output=True):
+    """src/mpas_sw_driver.py:311-332, with the output of the TLM demo
+    (demo/run-tlm.py:32-63): at every output interval, before the step, the
+    reconstructed TL velocity, h_tl and u_tl are recorded, and the record is
+    written to <output>.tlm.nc (f8; with write_output=False nothing is
+    written, as in the driver).  Returns the recorded arrays."""
     nml = namelist(nmlname=nmlname)
     read_configs(nml)
     read_dims(nml)
@@ -172,15 +176,32 @@ def run_sw_tlm(pt=1.E-3, nmlname='namelist.sw.x1.10242'):
     mpsw.h_tl[0] = mpsw.h[0] * pt
     clock_start, clock_end = clock_namelist(nml)
     today = clock_start
+    ulist, vlist, hlist, unorm = [], [], [], []
     while today <= clock_end:
+        if (today - clock_start).total_seconds() % nml.output_interval == 0:
+            print(today, mpsw.u[0].min(), mpsw.u[0].max())
+            ulist.append(mpsw.ureconstructzonal_tl[0].copy())
+            vlist.append(mpsw.ureconstructmeridional_tl[0].copy())
+            hlist.append(mpsw.h_tl[0, 0].copy())
+            unorm.append(mpsw.u_tl[0, 0].copy())
         mpsw.sw_rk4_tlm()
         today += timedelta(seconds=int(mpsw.config_dt))
+    if write_output:
+        _write_output(nml.file_output.replace('.nc', '.tlm.nc'), mpsw.latcell, mpsw.loncell, mpsw.latedge,
+                      mpsw.lonedge,
+                      [('ux_tl', 'd', ('nTime', 'nCell')), ('uy_tl', 'd', ('nTime', 'nCell')),
+                       ('h_tl', 'd', ('nTime', 'nCell')), ('u_tl', 'd', ('nTime', 'nEdge'))],
+                      [ulist, vlist, hlist, unorm])
+    return np.array(ulist), np.array(vlist), np.array(hlist), np.array(unorm)
 
 
-def run_sw_adj(nmlname='namelist.sw.x1.10242', seed_index=0):
+def run_sw_adj(nmlname='namelist.sw.x1.10242', seed_index=0, write_output=True):
     """The flow of demo/run-adj.py:16-80 with the nonlinear trajectory kept
     in memory: a forward sweep stores the state at every step, the adjoint
-    sweep walks it backwards.  Returns the adjoint (u_ad, h_ad) at the start."""
+    sweep walks it backwards.  At every output interval, before the step,
+    the reconstructed-velocity adjoints, h_ad and u_ad are recorded and
+    written to <output>.adj.nc in f4, as the demo does (:59-76).  Returns the
+    adjoint (u_ad, h_ad) at the start."""
     nml = namelist(nmlname=nmlname)
     read_configs(nml)
     read_dims(nml)
@@ -200,12 +221,25 @@ def run_sw_adj(nmlname='namelist.sw.x1.10242', seed_index=0):
     mpsw.h_ad[0, 0, seed_index] = 1.
     itimestep = len(traj_u) - 1
     today = clock_end
+    ulist, vlist, hlist, unorm = [], [], [], []
     while today >= clock_start:
+        print('{:%Y-%m-%d %H:%M}'.format(today), mpsw.u_ad.sum(), mpsw.h_ad.sum())
         mpsw.u[0, 0] = traj_u[itimestep]
         mpsw.h[0, 0] = traj_h[itimestep]
+        if (today - clock_start).total_seconds() % nml.output_interval == 0:
+            ulist.append(mpsw.ureconstructzonal_ad[0].copy())
+            vlist.append(mpsw.ureconstructmeridional_ad[0].copy())
+            hlist.append(mpsw.h_ad[0, 0].copy())
+            unorm.append(mpsw.u_ad[0, 0].copy())
         mpsw.sw_rk4_adj()
         itimestep -= 1
         today -= timedelta(seconds=int(mpsw.config_dt))
+    if write_output:
+        _write_output(nml.file_output.replace('.nc', '.adj.nc'), mpsw.latcell, mpsw.loncell, mpsw.latedge,
+                      mpsw.lonedge,
+                      [('ux_ad', 'f', ('nTime', 'nCell')), ('uy_ad', 'f', ('nTime', 'nCell')),
+                       ('h_ad', 'f', ('nTime', 'nCell')), ('u_ad', 'f', ('nTime', 'nEdge'))],
+                      [np.array(x, np.float32) for x in (ulist, vlist, hlist, unorm)])
     return mpsw.u_ad[0, 0].copy(), mpsw.h_ad[0, 0].copy()
 
 

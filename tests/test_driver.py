@@ -124,3 +124,65 @@ def test_run_sw_adj(tmp_path):
         o.rk4_adj()
     assert rel_l2(h_ad, o["h_ad1"]) < 1e-11
     assert rel_l2(u_ad, o["u_ad1"]) < 1e-11
+
+
+@pytest.mark.gpu
+def test_run_sw_tlm_writes_the_demo_output(tmp_path):
+    """run_sw_tlm with the TLM demo's output (demo/run-tlm.py:32-63): at each
+    output hour before the step, the reconstructed TL velocity, h_tl and u_tl
+    go to out.tlm.nc; the records equal the oracle's TLM at those times bit
+    for bit."""
+    print("driver run_sw_tlm L3 tc5 output", flush=True)
+    import importlib
+    import module_sw_mpas._facade as fac
+    importlib.reload(fac)
+    import module_sw_mpas
+    module_sw_mpas.mpas_sw_module = fac.mpas_sw_module
+    import mpas_sw_driver
+    importlib.reload(mpas_sw_driver)
+    from scipy.io import netcdf_file
+    nml = write_case(str(tmp_path), 3, 5, 900.0, "00_03:00:00")
+    mpas_sw_driver.run_sw_tlm(1e-3, nml)
+    with netcdf_file(os.path.join(str(tmp_path), "out.tlm.nc"), "r", mmap=False) as f:
+        rec = {k: np.array(f.variables[k][:]) for k in ("ux_tl", "uy_tl", "h_tl", "u_tl")}
+        assert f.variables["h_tl"].typecode() == "d"
+    assert rec["h_tl"].shape == (4, 642) and rec["u_tl"].shape == (4, 1920)
+    o = make_oracle(3, 5, 900.0)
+    o["h_tl1"][:] = o["h1"] * 1e-3
+    o["u_tl1"][:] = o["u1"] * 1e-3
+    # before the first step the TL reconstruction is still zero (only
+    # sw_rk4_tlm fills it, src/mpas_sw_module.f90:767)
+    assert not np.any(rec["ux_tl"][0]) and not np.any(rec["uy_tl"][0])
+    for k in range(4):
+        if k:
+            o.rk4_tlm(4)
+            np.testing.assert_array_equal(rec["ux_tl"][k], o["uReconstructZonal_tl"])
+            np.testing.assert_array_equal(rec["uy_tl"][k], o["uReconstructMeridional_tl"])
+        np.testing.assert_array_equal(rec["h_tl"][k], o["h_tl1"])
+        np.testing.assert_array_equal(rec["u_tl"][k], o["u_tl1"])
+
+
+@pytest.mark.gpu
+def test_run_sw_adj_writes_the_demo_output(tmp_path):
+    """run_sw_adj with the adjoint demo's output (demo/run-adj.py:41-76): f4
+    records of the reconstructed-velocity adjoints, h_ad and u_ad at each
+    output hour, last record (the start) equal to the returned adjoint."""
+    print("driver run_sw_adj L3 tc5 output", flush=True)
+    import importlib
+    import module_sw_mpas._facade as fac
+    importlib.reload(fac)
+    import module_sw_mpas
+    module_sw_mpas.mpas_sw_module = fac.mpas_sw_module
+    import mpas_sw_driver
+    importlib.reload(mpas_sw_driver)
+    from scipy.io import netcdf_file
+    nml = write_case(str(tmp_path), 3, 5, 900.0, "00_03:00:00")
+    u_ad, h_ad = mpas_sw_driver.run_sw_adj(nml, seed_index=3)
+    with netcdf_file(os.path.join(str(tmp_path), "out.adj.nc"), "r", mmap=False) as f:
+        rec = {k: np.array(f.variables[k][:]) for k in ("ux_ad", "uy_ad", "h_ad", "u_ad")}
+        assert f.variables["h_ad"].typecode() == "f"
+    assert rec["h_ad"].shape == (4, 642) and rec["u_ad"].shape == (4, 1920)
+    # the first record is the seed (end time); reconstruction adjoints stay zero
+    assert rec["h_ad"][0][3] == 1.0 and np.count_nonzero(rec["h_ad"][0]) == 1
+    assert not np.any(rec["ux_ad"]) and not np.any(rec["uy_ad"])
+    assert np.all(np.isfinite(rec["u_ad"]))
