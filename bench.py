@@ -194,9 +194,11 @@ def traffic_from_counters(fetch_dir, write_dir, nsteps, out_json, out_md, comman
 
 def packed_workload(steps, warmup):
     """Uneven batch: 64 level-5 + 16 level-6 + 4 level-7 + 1 level-9 instances
-    (4.59 M cells, alternating TC2 / TC6) in one packed index space, so every
-    stage kernel covers all of them in one launch; FP32 forward and FP32 TLM
-    members (FP64 base), dt 18.75 s.  Returns cell-steps/s of each."""
+    (4.59 M cells, alternating TC2 / TC6) in one packed index space; FP32
+    forward and FP32 TLM members (FP64 base), dt 18.75 s.  Timed two ways:
+    one launch per stage kernel for all instances (hipGraph per step), and
+    the persistent path, one cooperative launch per call whose passes take
+    (instance, patch) work items from a queue.  Returns cell-steps/s of each."""
     import torch
     from mpas_sw_amd.core import SWModel
     from mpas_sw_amd.ensemble import packed_instances
@@ -210,35 +212,41 @@ def packed_workload(steps, warmup):
     mesh, phys, offs, perm = packed_instances(specs)
     log(f"packed set-up: {time.time() - t:.1f}s; C={mesh.nCells} E={mesh.nEdges} V={mesh.nVertices}")
     m = SWModel(mesh, phys, 18.75, test_case=2, perm=perm, precision="fp32", graphs=True)
-    h = m.to_internal(phys["h"], "c")
-    u = m.to_internal(phys["u"], "e")
-    m.fwd(h, u, warmup)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    m.fwd(h, u, steps)
-    e1.record()
-    torch.cuda.synchronize()
-    t_fwd = e0.elapsed_time(e1) / 1e3
-    log(f"packed forward: {1e3 * t_fwd / steps:.3f} ms/step")
-    hb = m.to_internal(phys["h"], "c", member=False)
-    ub = m.to_internal(phys["u"], "e", member=False)
+    m.set_instances(offs)
     g = torch.Generator("cpu").manual_seed(SEED)
-    ht = m.to_internal(torch.randn(mesh.nCells, generator=g, dtype=torch.float64).numpy(), "c")
-    ut = m.to_internal(0.1 * torch.randn(mesh.nEdges, generator=g, dtype=torch.float64).numpy(), "e")
-    m.tlm(hb, ub, ht, ut, warmup)
-    torch.cuda.synchronize()
-    e0.record()
-    m.tlm(hb, ub, ht, ut, steps)
-    e1.record()
-    torch.cuda.synchronize()
-    t_tlm = e0.elapsed_time(e1) / 1e3
-    ok = int(m.nonfinite(h, u).sum().item()) == 0 and int(m.nonfinite(ht, ut).sum().item()) == 0
-    return {"workload": "uneven batch 64xL5 + 16xL6 + 4xL7 + 1xL9, TC2/TC6, dt 18.75 s, FP32 members",
-            "cells": mesh.nCells, "edges": mesh.nEdges, "vertices": mesh.nVertices, "steps": steps,
-            "fwd_cell_steps_per_s": mesh.nCells * steps / t_fwd, "fwd_ms_per_step": 1e3 * t_fwd / steps,
-            "tlm_cell_steps_per_s": mesh.nCells * steps / t_tlm, "tlm_ms_per_step": 1e3 * t_tlm / steps,
-            "finite": ok}
+    dh = torch.randn(mesh.nCells, generator=g, dtype=torch.float64).numpy()
+    du = 0.1 * torch.randn(mesh.nEdges, generator=g, dtype=torch.float64).numpy()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        fn(warmup)
+        torch.cuda.synchronize()
+        e0.record()
+        fn(steps)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+    out = {"workload": "uneven batch 64xL5 + 16xL6 + 4xL7 + 1xL9, TC2/TC6, dt 18.75 s, FP32 members",
+           "cells": mesh.nCells, "edges": mesh.nEdges, "vertices": mesh.nVertices, "instances": len(specs),
+           "steps": steps}
+    ok = True
+    for mode in ("stage_launches", "persistent"):
+        m.set_persistent(mode == "persistent")
+        h = m.to_internal(phys["h"], "c")
+        u = m.to_internal(phys["u"], "e")
+        t_fwd = timed(lambda n: m.fwd(h, u, n))
+        hb = m.to_internal(phys["h"], "c", member=False)
+        ub = m.to_internal(phys["u"], "e", member=False)
+        ht = m.to_internal(dh, "c")
+        ut = m.to_internal(du, "e")
+        t_tlm = timed(lambda n: m.tlm(hb, ub, ht, ut, n))
+        ok = ok and int(m.nonfinite(h, u).sum().item()) == 0 and int(m.nonfinite(ht, ut).sum().item()) == 0
+        log(f"packed {mode}: forward {1e3 * t_fwd / steps:.3f} ms/step, TLM {1e3 * t_tlm / steps:.3f} ms/step")
+        out[mode] = {"fwd_cell_steps_per_s": mesh.nCells * steps / t_fwd, "fwd_ms_per_step": 1e3 * t_fwd / steps,
+                     "tlm_cell_steps_per_s": mesh.nCells * steps / t_tlm, "tlm_ms_per_step": 1e3 * t_tlm / steps}
+    out["finite"] = ok
+    return out
 
 
 def gradient_workload(steps, snapshots):

@@ -129,9 +129,33 @@ int msw_set_graphs(msw_ctx *ctx, int enable);
  * MSW_OPT_BLOCK (default 256): threads per block of the stage kernels, 64,
  * 128, 256 or 512 (with 64 or more members a wavefront is one entity, so a
  * block covers block/64 consecutive entities).  Results do not depend on it;
- * SWModel.autotune times it after the stripe. */
-enum msw_option { MSW_OPT_TL_FUSED_FLUX = 1, MSW_OPT_XCD_STRIPE = 2, MSW_OPT_BLOCK = 3 };
+ * SWModel.autotune times it after the stripe.
+ *
+ * MSW_OPT_PERSISTENT (default 0): 1 runs msw_fwd_steps and msw_tlm_steps of a
+ * batch-1 context as ONE cooperative launch for the whole window: every pass
+ * of every RK stage takes (instance, patch) work items from a queue (patches
+ * of up to 256 entities inside one instance, see msw_set_instances) and the
+ * blocks meet at a grid barrier between passes.  Same results, bit for bit;
+ * the call waits for the launch.  No viscosity, drag or wind stress
+ * (MSW_EUNSUPPORTED); graphs are not used.
+ *
+ * MSW_OPT_BARRIER_POLLS (default 2^20): how many times a block polls a grid
+ * barrier of the persistent launch before every block leaves the kernel and
+ * the call returns MSW_EHIP. */
+enum msw_option {
+    MSW_OPT_TL_FUSED_FLUX = 1, MSW_OPT_XCD_STRIPE = 2, MSW_OPT_BLOCK = 3,
+    MSW_OPT_PERSISTENT = 4, MSW_OPT_BARRIER_POLLS = 5
+};
 int msw_set_option(msw_ctx *ctx, int option, int value);
+
+/* No counterpart in the reference.  The mesh of the context is a disjoint
+ * union of n instances (an uneven batch packed into one index space):
+ * instance i's cells, edges and vertices start at cell_start[i],
+ * edge_start[i], vertex_start[i] (internal order, increasing, the first 0).
+ * The persistent path cuts its work items at these boundaries; results do
+ * not depend on them.  Host arrays of n entries each. */
+int msw_set_instances(msw_ctx *ctx, int n, const int32_t *cell_start, const int32_t *edge_start,
+                      const int32_t *vertex_start);
 
 /* Replaces: sw_rk4 (src/mpas_sw_module.f90:528-635), nsteps times, on a
  * batch of B independent states h[C][B], u[E][B] of type `prec`. */

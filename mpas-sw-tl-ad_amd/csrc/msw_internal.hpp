@@ -248,6 +248,44 @@ void scatter_rows(long n, int lb, const int* perm, const T* src, T* dst, hipStre
 template <typename T>
 void nonfinite_flags(long n, int lb, const T* x, int* flags, hipStream_t s);
 
+// Arguments of the persistent path (one member per entity): a window of
+// nsteps forward (h, u: the member state) or TLM steps (bh, bu: the FP64
+// base state; h, u: the TL state) in one cooperative launch.  Work items are
+// patches (first entity, count <= 256) of vertices [np_v], then cells
+// [np_c], then edges [np_e]; queue (2 x 8 counters), bar and err start at zero.
+struct PersistArgs {
+    DMesh m;
+    DConfig c;
+    RKCoeffs k;
+    int nsteps;
+    const int2* patch;
+    int np_v, np_c, np_e;
+    BaseStage st[4];          // base stage fields (st[0].h/u = bh/bu)
+    double* bh;
+    double* bu;
+    double* bacc;             // [C+E]
+    double* bend;             // [C+E]
+    double* traj_h;           // optional [nsteps][C], [nsteps][E]
+    double* traj_u;
+    void* h;                  // member (forward) or TL state [C], [E]
+    void* u;
+    void* acc;                // [C+E]
+    void* pA;                 // [C+E]
+    void* pB;                 // [C+E]
+    void* pvv;                // [V]
+    void* ke;                 // [C]
+    void* pc;                 // [C]
+    void* he;                 // [E]
+    void* pe;                 // [E]
+    unsigned long long* queue;
+    unsigned long long* bar;
+    int* err;
+    unsigned spin_limit;      // polls of a grid barrier before the launch gives up
+};
+// Launch it (cooperatively, every block resident): 0 or a hipError_t.
+template <typename T>
+int persistent_steps(const PersistArgs& a, bool tlm, hipStream_t s, int* grid);
+
 }  // namespace msw
 
 namespace msw {

@@ -1527,6 +1527,517 @@ __global__ __launch_bounds__(kMaxBlock) void k_nonfinite(long n, int lb, const T
     }
 }
 
+// ---------------------------------------------------------------------------
+// Persistent path for one member per entity (B = 1), for uneven batches of
+// mesh instances packed into one index space: one cooperative launch runs a
+// whole window of forward or TLM steps.  Every pass of every RK stage walks
+// a work queue of (instance, patch) items -- patches of up to kPatch
+// consecutive entities of one kind that never straddle two instances -- and
+// the blocks meet at a grid barrier between passes.  Each item computes
+// what the per-stage kernels above compute for it, with the same operations
+// in the same order, so the results are the same bits; the TL stage's
+// coefficient products are formed from the neighbour records in the item
+// (in FP64, the products k_tl_coef stores) instead of rows in memory.
+
+constexpr int kPatch = 256;   // entities per patch = threads per block
+
+template <typename T>
+__device__ __forceinline__ void pv_vertex_item(const DMesh& m, const T* h, const T* u, T* pvv, T* hv, T* vort,
+                                               int iv)
+{
+    int ie[3], ic[3];
+    T ue[3], hc[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { ie[i] = m.eovR[3 * iv + i]; ic[i] = m.cov[3 * iv + i]; }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { ue[i] = u[ie[i]]; hc[i] = h[ic[i]]; }
+    T circ = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) circ = circ + T(m.sdcV[3 * iv + i]) * ue[i];
+    const T at = T(m.areaT[iv]);
+    const T vo = circ / at;
+    T hh = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) hh = hh + hc[i] * T(m.kite[3 * iv + i]);
+    hh = hh / at;
+    stn(pvv + iv, (T(m.fV[iv]) + vo) / hh);
+    if (hv) stn(hv + iv, hh);
+    if (vort) stn(vort + iv, vo);
+}
+
+template <typename T>
+__device__ __forceinline__ void pv_cell_item(const DMesh& m, const T* u, const T* pvv, T* ke, T* pc, int ic)
+{
+    int ie[kC6], iv[kC6];
+    T ue[kC6], pv[kC6];
+#pragma unroll
+    for (int i = 0; i < kC6; ++i) { ie[i] = m.eoc[kC6 * ic + i]; iv[i] = m.vocR[kC6 * ic + i]; }
+#pragma unroll
+    for (int i = 0; i < kC6; ++i) { ue[i] = u[ie[i]]; pv[i] = pvv[iv[i]]; }
+    const T area = T(m.areaC[ic]);
+    T k = T(0);
+#pragma unroll
+    for (int i = 0; i < kC6; ++i) k = k + T(m.keC[kC6 * ic + i]) * (ue[i] * ue[i]);
+    k = k / area;
+    T p = T(0);
+#pragma unroll
+    for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
+    stn(ke + ic, k);
+    stn(pc + ic, p);
+}
+
+template <typename T>
+__device__ __forceinline__ void pv_edge_item(const DMesh& m, const DConfig& c, const T* h, const T* u,
+                                             const T* pvv, const T* pc, T* he, T* pe, T* vout, T* gpt_out,
+                                             T* gpn_out, double* rec, int e)
+{
+    const T apvm = T(c.apvm), dt = T(c.dt);
+    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+    const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
+    int ix[kEoE];
+    T ux[kEoE];
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) ux[j] = u[ix[j]];
+    const T h1 = h[c1], h2 = h[c2], p1 = pvv[v1], p2 = pvv[v2], q1 = pc[c1], q2 = pc[c2];
+    const T ue = u[e];
+    T vv = T(0);
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) vv = vv + T(m.woe[kE2 * e + j]) * ux[j];
+    const T gt = (p2 - p1) / T(m.dv[e]);
+    const T gn = (q2 - q1) / T(m.dc[e]);
+    T p = T(0.5) * p1 + T(0.5) * p2;
+    p = p - apvm * vv * dt * gt;
+    p = p - apvm * ue * dt * gn;
+    stn(he + e, T(0.5) * (h1 + h2));
+    stn(pe + e, p);
+    if (vout) stn(vout + e, vv);
+    if (gpt_out) stn(gpt_out + e, gt);
+    if (gpn_out) stn(gpn_out + e, gn);
+    if (rec) {
+        double2* r = reinterpret_cast<double2*>(rec + 4L * e);
+        r[0] = make_double2(static_cast<double>(p), static_cast<double>(ue));
+        r[1] = make_double2(static_cast<double>(T(0.5) * (h1 + h2)), static_cast<double>(gt));
+    }
+}
+
+// tend_u and its RK update (k_fwd_tend's edge branch); rec: the neighbours'
+// pv_edge, u, h_edge from the packed records (the base) instead of u, pe, he
+template <typename T>
+__device__ __forceinline__ void tend_edge_item(const DMesh& m, const DConfig& c, int stage, T w, T sw,
+                                               const T* h, const T* u, const T* he, const T* pe, const T* ke,
+                                               const double* rec, T* xu, T* accu, T* pu, int e)
+{
+    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+    int ix[kEoE];
+    T ux[kEoE], px[kEoE], hx[kEoE];
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
+    if (rec) {
+#pragma unroll
+        for (int j = 0; j < kEoE; ++j) {
+            const double4 r4 = *reinterpret_cast<const double4*>(rec + 4L * ix[j]);
+            px[j] = T(r4.x);
+            ux[j] = T(r4.y);
+            hx[j] = T(r4.z);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kEoE; ++j) { ux[j] = u[ix[j]]; px[j] = pe[ix[j]]; hx[j] = he[ix[j]]; }
+    }
+    const T pee = pe[e];
+    const T k1 = ke[c1], k2 = ke[c2], h1 = h[c1], h2 = h[c2];
+    T q = T(0);
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) {
+        const T workpv = T(0.5) * (pee + px[j]);
+        q = q + T(m.woe[kE2 * e + j]) * ux[j] * workpv * hx[j];
+    }
+    const T tend = q - (k2 - k1 + T(kGravity) * (h2 + T(m.hs[c2]) - h1 - T(m.hs[c1]))) / T(m.dc[e]);
+    rk_update(stage, w, sw, tend, xu, accu, pu, e, 1, 0, c.frozen_u != 0);
+}
+
+// tend_h and its RK update (k_fwd_tend's cell branch)
+template <typename T>
+__device__ __forceinline__ void tend_cell_item(const DMesh& m, int stage, T w, T sw, const T* u, const T* he,
+                                               T* xh, T* acch, T* ph, int ic)
+{
+    int ie[kC6];
+    T ue[kC6], hh[kC6];
+#pragma unroll
+    for (int k = 0; k < kC6; ++k) ie[k] = m.eocR[kC6 * ic + k];
+#pragma unroll
+    for (int k = 0; k < kC6; ++k) { ue[k] = u[ie[k]]; hh[k] = he[ie[k]]; }
+    T th = T(0);
+#pragma unroll
+    for (int k = 0; k < kC6; ++k) {
+        const T flux = ue[k] * T(m.dvCR[kC6 * ic + k]) * hh[k];
+        th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
+    }
+    th = th / T(m.areaC[ic]);
+    rk_update(stage, w, sw, th, xh, acch, ph, ic, 1, 0, false);
+}
+
+template <typename T>
+__device__ __forceinline__ void tl_vertex_item(const DMesh& m, const BaseStage& bs, const T* h_tl, const T* u_tl,
+                                               T* pvv_tl, int iv)
+{
+    int ie[3], ic[3];
+    T ue[3], hc[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { ie[i] = m.eovR[3 * iv + i]; ic[i] = m.cov[3 * iv + i]; }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { ue[i] = u_tl[ie[i]]; hc[i] = h_tl[ic[i]]; }
+    T ct = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ct = ct + T(m.sdcV[3 * iv + i]) * ue[i];
+    const T at = T(m.areaT[iv]);
+    const T vt = ct / at;
+    T ht = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ht = ht + hc[i] * T(m.kite[3 * iv + i]);
+    ht = ht / at;
+    const T hv = T(bs.hv[iv]);
+    stn(pvv_tl + iv, vt / hv - ht * (T(m.fV[iv]) + T(bs.vort[iv])) / (hv * hv));
+}
+
+// k_tl_cell with its two coefficients (2*u and 0.25*dcEdge*dvEdge per edge
+// slot, as k_tl_coef forms them) computed here
+template <typename T>
+__device__ __forceinline__ void tl_cell_item(const DMesh& m, const BaseStage& bs, const T* u_tl, const T* pvv_tl,
+                                             T* ke_tl, T* pc_tl, int ic)
+{
+    int ie[kC6], iv[kC6];
+    T ue[kC6], pv[kC6];
+#pragma unroll
+    for (int i = 0; i < kC6; ++i) { ie[i] = m.eoc[kC6 * ic + i]; iv[i] = m.vocR[kC6 * ic + i]; }
+#pragma unroll
+    for (int i = 0; i < kC6; ++i) { ue[i] = u_tl[ie[i]]; pv[i] = pvv_tl[iv[i]]; }
+    const T area = T(m.areaC[ic]);
+    T k = T(0);
+#pragma unroll
+    for (int i = 0; i < kC6; ++i) {
+        const int es = ie[i];
+        const double r24 = 0.25 * m.dc[es] * m.dv[es];
+        const double r12 = 2.0 * bs.u[es];
+        const T mask = T(fabs(m.sgnC[kC6 * ic + i]));
+        k = k + mask * (T(r24) * (T(r12) * ue[i]));
+    }
+    k = k / area;
+    T p = T(0);
+#pragma unroll
+    for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
+    stn(ke_tl + ic, k);
+    stn(pc_tl + ic, p);
+}
+
+template <typename T>
+__device__ __forceinline__ void tl_edge_item(const DMesh& m, const DConfig& c, const BaseStage& bs, const T* h_tl,
+                                             const T* u_tl, const T* pvv_tl, const T* pc_tl, T* pe_tl, T* he_tl,
+                                             int e)
+{
+    const T apvm = T(c.apvm), dt = T(c.dt);
+    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+    const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
+    int ix[kEoE];
+    T ux[kEoE];
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) ux[j] = u_tl[ix[j]];
+    const T p1 = pvv_tl[v1], p2 = pvv_tl[v2], q1 = pc_tl[c1], q2 = pc_tl[c2], h1 = h_tl[c1], h2 = h_tl[c2];
+    const T ue = u_tl[e];
+    T vt = T(0);
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) vt = vt + T(m.woe[kE2 * e + j]) * ux[j];
+    const T gtt = (p2 - p1) / T(m.dv[e]);
+    const T gnt = (q2 - q1) / T(m.dc[e]);
+    T p = T(0.5) * p1 + T(0.5) * p2;
+    p = p - apvm * vt * dt * T(bs.gpt[e]) - apvm * T(bs.v[e]) * dt * gtt;
+    p = p - apvm * ue * dt * T(bs.gpn[e]) - apvm * T(bs.u[e]) * dt * gnt;
+    stn(pe_tl + e, p);
+    const T het = T(0.5) * (h1 + h2);
+    stn(he_tl + e, c.tl_fused ? ue * T(bs.he[e]) + T(bs.u[e]) * het : het);
+}
+
+// k_tl_tend's edge branch with its edgesOnEdge coefficients formed from the
+// neighbour records as k_tl_coef forms them
+template <typename T, bool FUSED>
+__device__ __forceinline__ void tl_tend_edge_item(const DMesh& m, const DConfig& c, int stage, T w, T sw,
+                                                  const BaseStage& bs, const T* h_tl, const T* u_tl, const T* pe_tl,
+                                                  const T* he_tl, const T* ke_tl, T* xu, T* accu, T* pu, int e)
+{
+    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+    int ix[kEoE];
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
+    const T pte = pe_tl[e];
+    const T k1 = ke_tl[c1], k2 = ke_tl[c2], h1 = h_tl[c1], h2 = h_tl[c2];
+    const double peb = bs.pe[e];
+    T qt = T(0);
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) {
+        const double4 ro = *reinterpret_cast<const double4*>(bs.rec + 4L * ix[j]);   // pe u he gpt
+        const double wo = m.woe[kE2 * e + j];
+        const double wpv = 0.5 * (peb + ro.x);
+        const double kk = wo * ro.y;
+        if constexpr (FUSED) {
+            qt = qt + T(wo * wpv) * he_tl[ix[j]] + T(0.5 * (kk * ro.z)) * (pte + pe_tl[ix[j]]);
+        } else {
+            const T wpvt = T(0.5) * (pte + pe_tl[ix[j]]);
+            const T heb = T(ro.z);
+            qt = qt + T(wo) * u_tl[ix[j]] * T(wpv) * heb + T(kk) * wpvt * heb + T(kk * wpv) * he_tl[ix[j]];
+        }
+    }
+    const T tend = qt - (k2 - k1 + T(kGravity) * (h2 - h1)) / T(m.dc[e]);
+    rk_update(stage, w, sw, tend, xu, accu, pu, e, 1, 0, c.frozen_u != 0);
+}
+
+template <typename T, bool FUSED>
+__device__ __forceinline__ void tl_tend_cell_item(const DMesh& m, int stage, T w, T sw, const BaseStage& bs,
+                                                  const T* u_tl, const T* he_tl, T* xh, T* acch, T* ph, int ic)
+{
+    T th = T(0);
+#pragma unroll
+    for (int k = 0; k < kC6; ++k) {
+        const int e = m.eocR[kC6 * ic + k];
+        if constexpr (FUSED) {
+            th = th + T(-m.sgnCR[kC6 * ic + k]) * (T(m.dv[e]) * he_tl[e]);
+        } else {
+            const T flux = u_tl[e] * T(m.dv[e]) * T(bs.he[e]) + T(bs.u[e] * m.dv[e]) * he_tl[e];
+            th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
+        }
+    }
+    th = th / T(m.areaC[ic]);
+    rk_update(stage, w, sw, th, xh, acch, ph, ic, 1, 0, false);
+}
+
+// Grid barrier of the persistent launch: every block adds one to `bar` and
+// waits until all gridDim.x blocks of this phase have, at most `limit`
+// polls; a block that gives up sets `err`, and every block that sees `err`
+// returns false, so all blocks leave the kernel instead of waiting forever.
+__device__ bool grid_barrier(unsigned long long* bar, int* err, unsigned long long target, unsigned limit)
+{
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();   // this block's writes of the pass, to every XCD
+        __hip_atomic_fetch_add(bar, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        int ok = 1;
+        unsigned polls = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
+            if (++polls > limit) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(8);
+        }
+        __threadfence();   // the other blocks' writes, before this block reads them
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// One pass: its work items are the patches of up to three entity kinds
+// (0 vertex, 1 cell, 2 edge); `item(kind, entity)` processes one entity.
+// The items are split into 8 queues, queue x holding the x-th contiguous
+// eighth of every kind's patches, so that the blocks of XCD x (hardware
+// block i runs on XCD i mod 8) work on one region of the space-filling curve
+// and find its neighbours in their own L2.  A block takes kChunk items per
+// atomic from its XCD's queue, then from the others in turn (work stealing)
+// until all are empty.  Passes alternate between two sets of queue counters;
+// block 0 clears the set of the previous pass, which the next barrier
+// publishes before that set is used again.
+constexpr int kChunk = 4;
+constexpr int kQueues = 8;
+
+template <typename F>
+__device__ __forceinline__ bool persistent_pass(const PersistArgs& a, const int (&kinds)[3], int nk,
+                                                unsigned long long& phase, F&& item)
+{
+    __shared__ unsigned long long s_q;
+    const int par = static_cast<int>(phase & 1ull);
+    unsigned long long* cnt = a.queue + par * kQueues;
+    if (blockIdx.x == 0 && threadIdx.x < kQueues)
+        __hip_atomic_store(a.queue + (1 - par) * kQueues + threadIdx.x, 0ull, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    const int np[3] = {a.np_v, a.np_c, a.np_e};
+    const int home = static_cast<int>(blockIdx.x) & (kQueues - 1);
+    for (int d = 0; d < kQueues; ++d) {
+        const int x = (home + d) & (kQueues - 1);
+        // this queue's share of each kind: patches [lo, hi) of that kind
+        int lo[3], len[3];
+        int total = 0;
+        for (int k = 0; k < nk; ++k) {
+            const int n = np[kinds[k]];
+            lo[k] = static_cast<int>((static_cast<long>(n) * x) / kQueues);
+            len[k] = static_cast<int>((static_cast<long>(n) * (x + 1)) / kQueues) - lo[k];
+            total += len[k];
+        }
+        for (;;) {
+            if (threadIdx.x == 0)
+                s_q = __hip_atomic_fetch_add(cnt + x, static_cast<unsigned long long>(kChunk), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            const unsigned long long q = s_q;
+            __syncthreads();
+            if (q >= static_cast<unsigned long long>(total)) break;
+            // the chunk's patches: kind and entity of this thread in each
+            int kk[kChunk], ent[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+                const int qq = static_cast<int>(q) + j;
+                kk[j] = -1;
+                if (qq < total) {
+                    int k = 0, r = qq;
+                    while (r >= len[k]) { r -= len[k]; ++k; }
+                    const int kind = kinds[k];
+                    const int off = kind == 0 ? 0 : (kind == 1 ? np[0] : np[0] + np[1]);
+                    const int2 pt = a.patch[off + lo[k] + r];
+                    if (static_cast<int>(threadIdx.x) < pt.y) {
+                        kk[j] = kind;
+                        ent[j] = pt.x + static_cast<int>(threadIdx.x);
+                    }
+                }
+            }
+            // entities of one pass never read what the pass writes, so the
+            // items of a chunk are independent
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j)
+                if (kk[j] >= 0) item(kk[j], ent[j]);
+        }
+    }
+    ++phase;
+    return grid_barrier(a.bar, a.err, phase * gridDim.x, a.spin_limit);
+}
+
+// Forward RK4 of one member per entity (enqueue_fwd_step's passes):
+// vertex, cell, edge, then tendency + update of edges and cells per stage.
+template <typename T>
+__global__ __launch_bounds__(kPatch) void k_persist_fwd(PersistArgs a)
+{
+    const DMesh& m = a.m;
+    const DConfig& c = a.c;
+    T* h = static_cast<T*>(a.h);
+    T* u = static_cast<T*>(a.u);
+    T* acc = static_cast<T*>(a.acc);
+    T* pvv = static_cast<T*>(a.pvv);
+    T* ke = static_cast<T*>(a.ke);
+    T* pc = static_cast<T*>(a.pc);
+    T* he = static_cast<T*>(a.he);
+    T* pe = static_cast<T*>(a.pe);
+    const size_t C = m.nC;
+    unsigned long long phase = 0;
+    const int kV[3] = {0, 0, 0}, kC[3] = {1, 0, 0}, kE[3] = {2, 0, 0}, kEC[3] = {2, 1, 0};
+    for (int step = 0; step < a.nsteps; ++step) {
+        const T* rh = h;
+        const T* ru = u;
+        for (int s = 0; s < 4; ++s) {
+            T* nb = static_cast<T*>((s % 2 == 0) ? a.pA : a.pB);
+            const T w = T(a.k.w[s]), sw = T(a.k.sw[s]);
+            if (!persistent_pass(a, kV, 1, phase, [&](int, int i) {
+                    pv_vertex_item<T>(m, rh, ru, pvv, nullptr, nullptr, i); }))
+                return;
+            if (!persistent_pass(a, kC, 1, phase, [&](int, int i) { pv_cell_item<T>(m, ru, pvv, ke, pc, i); }))
+                return;
+            if (!persistent_pass(a, kE, 1, phase, [&](int, int i) {
+                    pv_edge_item<T>(m, c, rh, ru, pvv, pc, he, pe, nullptr, nullptr, nullptr, nullptr, i); }))
+                return;
+            if (!persistent_pass(a, kEC, 2, phase, [&](int kind, int i) {
+                    if (kind == 2)
+                        tend_edge_item<T>(m, c, s, w, sw, rh, ru, he, pe, ke, nullptr, u, acc + C, nb + C, i);
+                    else
+                        tend_cell_item<T>(m, s, w, sw, ru, he, h, acc, nb, i);
+                }))
+                return;
+            rh = nb;
+            ru = nb + C;
+        }
+    }
+}
+
+// Tangent-linear RK4 of one member per entity with its FP64 base trajectory
+// (enqueue_tlm_step's base and TL kernels), interleaved in five passes per
+// stage: base vertex | base cell + TL vertex | base edge + TL cell | base
+// tendency + update + TL edge | TL tendency + update.  With a trajectory the
+// first pass of each step also stores the base state at the step start.
+template <typename T, bool FUSED>
+__global__ __launch_bounds__(kPatch) void k_persist_tlm(PersistArgs a)
+{
+    const DMesh& m = a.m;
+    const DConfig& c = a.c;
+    T* htl = static_cast<T*>(a.h);
+    T* utl = static_cast<T*>(a.u);
+    T* acc = static_cast<T*>(a.acc);
+    T* pvv_tl = static_cast<T*>(a.pvv);
+    T* ke_tl = static_cast<T*>(a.ke);
+    T* pc_tl = static_cast<T*>(a.pc);
+    T* pe_tl = static_cast<T*>(a.pe);
+    T* he_tl = static_cast<T*>(a.he);
+    const size_t C = m.nC;
+    unsigned long long phase = 0;
+    const int kV[3] = {0, 0, 0}, kVCE[3] = {0, 1, 2}, kCV[3] = {1, 0, 0}, kEC[3] = {2, 1, 0};
+    for (int step = 0; step < a.nsteps; ++step) {
+        const T* th = htl;
+        const T* tu = utl;
+        for (int s = 0; s < 4; ++s) {
+            const BaseStage& b = a.st[s];
+            double* nh = s < 3 ? a.st[s + 1].h : a.bend;
+            T* nb = static_cast<T*>((s % 2 == 0) ? a.pA : a.pB);
+            const T w = T(a.k.w[s]), sw = T(a.k.sw[s]);
+            const bool store = s == 0 && a.traj_h != nullptr;
+            const size_t tro = static_cast<size_t>(step);
+            if (!persistent_pass(a, store ? kVCE : kV, store ? 3 : 1, phase, [&](int kind, int i) {
+                    if (kind == 0)
+                        pv_vertex_item<double>(m, b.h, b.u, b.pvv, b.hv, b.vort, i);
+                    else if (kind == 1)
+                        a.traj_h[tro * C + i] = a.bh[i];
+                    else
+                        a.traj_u[tro * m.nE + i] = a.bu[i];
+                }))
+                return;
+            if (!persistent_pass(a, kCV, 2, phase, [&](int kind, int i) {
+                    if (kind == 1)
+                        pv_cell_item<double>(m, b.u, b.pvv, b.ke, b.pc, i);
+                    else
+                        tl_vertex_item<T>(m, b, th, tu, pvv_tl, i);
+                }))
+                return;
+            if (!persistent_pass(a, kEC, 2, phase, [&](int kind, int i) {
+                    if (kind == 2)
+                        pv_edge_item<double>(m, c, b.h, b.u, b.pvv, b.pc, b.he, b.pe, b.v, b.gpt, b.gpn, b.rec, i);
+                    else
+                        tl_cell_item<T>(m, b, tu, pvv_tl, ke_tl, pc_tl, i);
+                }))
+                return;
+            if (!persistent_pass(a, kEC, 2, phase, [&](int kind, int i) {
+                    if (kind == 2) {
+                        tend_edge_item<double>(m, c, s, a.k.w[s], a.k.sw[s], b.h, b.u, b.he, b.pe, b.ke, b.rec,
+                                               a.bu, a.bacc + C, nh + C, i);
+                        tl_edge_item<T>(m, c, b, th, tu, pvv_tl, pc_tl, pe_tl, he_tl, i);
+                    } else {
+                        tend_cell_item<double>(m, s, a.k.w[s], a.k.sw[s], b.u, b.he, a.bh, a.bacc, nh, i);
+                    }
+                }))
+                return;
+            if (!persistent_pass(a, kEC, 2, phase, [&](int kind, int i) {
+                    if (kind == 2)
+                        tl_tend_edge_item<T, FUSED>(m, c, s, w, sw, b, th, tu, pe_tl, he_tl, ke_tl, utl, acc + C,
+                                                    nb + C, i);
+                    else
+                        tl_tend_cell_item<T, FUSED>(m, s, w, sw, b, tu, he_tl, htl, acc, nb, i);
+                }))
+                return;
+            th = nb;
+            tu = nb + C;
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1795,6 +2306,29 @@ void nonfinite_flags(long n, int lb, const T* x, int* flags, hipStream_t s)
 {
     hipLaunchKernelGGL(k_nonfinite<T>, dim3(grid_for(n << lb)), dim3(kBlock), 0, s, n, lb, x, flags);
 }
+
+template <typename T>
+int persistent_steps(const PersistArgs& a0, bool tlm, hipStream_t s, int* grid_out)
+{
+    PersistArgs a = a0;
+    const void* fn = tlm ? (a.c.tl_fused ? reinterpret_cast<const void*>(&k_persist_tlm<T, true>)
+                                         : reinterpret_cast<const void*>(&k_persist_tlm<T, false>))
+                         : reinterpret_cast<const void*>(&k_persist_fwd<T>);
+    int dev = 0, ncu = 0, per = 0, coop = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, kPatch, 0);
+    if (e != hipSuccess) return static_cast<int>(e);
+    if (!coop || per < 1) return static_cast<int>(hipErrorCooperativeLaunchTooLarge);
+    const int grid = ncu * per;
+    if (grid_out) *grid_out = grid;
+    void* args[] = {&a};
+    return static_cast<int>(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPatch), args, 0, s));
+}
+
+template int persistent_steps<double>(const PersistArgs&, bool, hipStream_t, int*);
+template int persistent_steps<float>(const PersistArgs&, bool, hipStream_t, int*);
 
 #define INST(T)                                                                                     \
     template void fwd_diag<T>(const DMesh&, const DConfig&, int, const T*, const T*, T*, T*, T*, T*, \

@@ -114,6 +114,14 @@ struct msw_ctx {
     void* me2 = nullptr;              // [E]
     void* me3 = nullptr;              // [E]
     double* inv = nullptr;            // [2C + 3V + 2E + 2C] invariants scratch, made on first use
+    // persistent path (MSW_OPT_PERSISTENT): work items and counters
+    bool persistent = false;
+    unsigned spin_limit = 1u << 20;
+    std::vector<int> inst_c{0}, inst_e{0}, inst_v{0};   // instance starts (internal order)
+    const int2* patches = nullptr;
+    int np_v = 0, np_c = 0, np_e = 0;
+    unsigned long long* pcount = nullptr;   // queue counters [2][8], barrier, error flag
+    int* hflag = nullptr;                   // pinned host copy of the error flag
     double* dpartial = nullptr;       // dot partials
     int dot_nblk = 0;
 
@@ -144,6 +152,7 @@ struct msw_ctx {
             if (e) (void)hipEventDestroy(e);
         for (auto& b : bufs)
             if (b.p) (void)hipFree(b.p);
+        if (hflag) (void)hipHostFree(hflag);
     }
 
     template <typename T>
@@ -730,6 +739,34 @@ int enqueue_adj_step_piped(msw_ctx* c, int par, bool next, T* lamh, T* lamu)
     });
 }
 
+// Patches of up to 256 consecutive entities of each kind that never cross an
+// instance boundary, uploaded as the persistent path's work items.
+int build_patches(msw_ctx* c)
+{
+    std::vector<int2> p;
+    const std::vector<int>* starts[3] = {&c->inst_v, &c->inst_c, &c->inst_e};
+    const int totals[3] = {c->nV, c->nC, c->nE};
+    int counts[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+        const std::vector<int>& st = *starts[k];
+        for (size_t i = 0; i < st.size(); ++i) {
+            const int a = st[i], b = i + 1 < st.size() ? st[i + 1] : totals[k];
+            for (int q = a; q < b; q += 256) {
+                p.push_back(make_int2(q, std::min(256, b - q)));
+                ++counts[k];
+            }
+        }
+    }
+    const int2* dev = nullptr;
+    int rc = c->upload(&dev, p);
+    if (rc) return rc;
+    c->patches = dev;
+    c->np_v = counts[0];
+    c->np_c = counts[1];
+    c->np_e = counts[2];
+    return MSW_OK;
+}
+
 int check_cfg(msw_ctx* c)
 {
     const msw_config& g = c->cfg;
@@ -812,6 +849,68 @@ int run_steps(msw_ctx* c, int kind, const void* p0, const void* p1, const void* 
               int nsteps, F&& body, const std::function<int(int)>* pre = nullptr)
 {
     return run_steps_var(c, kind, p0, p1, p2, p3, nsteps, [&](int) { return body(); }, pre, nullptr);
+}
+
+// A window of forward (tlm = false; h, u: member state) or TLM steps (bh, bu:
+// base state; h, u: TL state) through the persistent kernel, then a check of
+// its error flag (this waits for the launch).
+int run_persistent(msw_ctx* c, bool tlm, double* bh, double* bu, void* h, void* u, int nsteps, double* traj_h,
+                   double* traj_u)
+{
+    if (c->B != 1) return fail(MSW_EUNSUPPORTED, "the persistent path runs one member per entity (batch 1)");
+    const DConfig& d = c->dcfg;
+    if (d.visc2 > 0.0 || d.visc4 > 0.0 || d.wind || d.drag)
+        return fail(MSW_EUNSUPPORTED, "the persistent path has no viscosity, drag or wind-stress terms");
+    if (nsteps == 0) return MSW_OK;
+    int rc = MSW_OK;
+    if (!c->patches && (rc = build_patches(c))) return rc;
+    if (!c->pcount && (rc = c->alloc(&c->pcount, 18))) return rc;
+    if (!c->hflag) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->hflag), sizeof(int), hipHostMallocDefault));
+    const size_t C = c->nC;
+    PersistArgs a{};
+    a.m = c->dm;
+    a.c = c->dcfg;
+    a.k = rk_coeffs(c->cfg.dt);
+    a.nsteps = nsteps;
+    a.patch = c->patches;
+    a.np_v = c->np_v;
+    a.np_c = c->np_c;
+    a.np_e = c->np_e;
+    for (int s = 0; s < 4; ++s) a.st[s] = c->st[s];
+    a.st[0].h = bh;
+    a.st[0].u = bu;
+    a.bh = bh;
+    a.bu = bu;
+    a.bacc = c->bacc;
+    a.bend = c->bend;
+    a.traj_h = traj_h;
+    a.traj_u = traj_u;
+    a.h = h;
+    a.u = u;
+    a.acc = c->macc;
+    a.pA = c->mpA;
+    a.pB = c->mpB;
+    a.pvv = c->mv1;
+    a.ke = c->mc1;
+    a.pc = c->mc2;
+    a.he = c->me2;
+    a.pe = c->me1;
+    a.queue = c->pcount;   // 2 x 8 queue counters
+    a.bar = c->pcount + 16;
+    a.err = reinterpret_cast<int*>(c->pcount + 17);
+    a.spin_limit = c->spin_limit;
+    (void)C;
+    HIP_TRY(hipMemsetAsync(c->pcount, 0, 18 * sizeof(unsigned long long), c->stream));
+    int grid = 0;
+    const int e = c->prec == MSW_FP64 ? persistent_steps<double>(a, tlm, c->stream, &grid)
+                                      : persistent_steps<float>(a, tlm, c->stream, &grid);
+    if (e) return fail(MSW_EHIP, std::string("persistent launch: ") + hipGetErrorString(static_cast<hipError_t>(e)));
+    HIP_TRY(hipMemcpyAsync(c->hflag, a.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (*c->hflag)
+        return fail(MSW_EHIP, "persistent launch: a grid barrier waited longer than its limit (" +
+                                  std::to_string(c->spin_limit) + " polls) and every block left the kernel");
+    return MSW_OK;
 }
 
 }  // namespace
@@ -950,14 +1049,38 @@ int msw_set_option(msw_ctx* ctx, int option, int value)
                                 "(one contiguous range per XCD)");
     if (option == MSW_OPT_BLOCK && value != 64 && value != 128 && value != 256 && value != 512)
         return fail(MSW_EINVAL, "MSW_OPT_BLOCK takes 64, 128, 256 or 512 threads");
-    if (option != MSW_OPT_TL_FUSED_FLUX && option != MSW_OPT_XCD_STRIPE && option != MSW_OPT_BLOCK)
+    if (option == MSW_OPT_BARRIER_POLLS && value < 1) return fail(MSW_EINVAL, "MSW_OPT_BARRIER_POLLS must be positive");
+    if (option != MSW_OPT_TL_FUSED_FLUX && option != MSW_OPT_XCD_STRIPE && option != MSW_OPT_BLOCK &&
+        option != MSW_OPT_PERSISTENT && option != MSW_OPT_BARRIER_POLLS)
         return fail(MSW_EINVAL, "unknown option");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (option == MSW_OPT_TL_FUSED_FLUX) ctx->tl_fused = value != 0;
     else if (option == MSW_OPT_XCD_STRIPE) ctx->dm.stripe_log2 = value;
-    else ctx->dm.block = value;
+    else if (option == MSW_OPT_BLOCK) ctx->dm.block = value;
+    else if (option == MSW_OPT_PERSISTENT) ctx->persistent = value != 0;
+    else ctx->spin_limit = static_cast<unsigned>(value);
     set_dcfg(ctx);
     ctx->clear_graphs();
+    return MSW_OK;
+}
+
+int msw_set_instances(msw_ctx* ctx, int n, const int32_t* cell_start, const int32_t* edge_start,
+                      const int32_t* vertex_start)
+{
+    if (!ctx || n < 1 || !cell_start || !edge_start || !vertex_start) return fail(MSW_EINVAL, "bad argument");
+    const int32_t* starts[3] = {cell_start, edge_start, vertex_start};
+    const int totals[3] = {ctx->nC, ctx->nE, ctx->nV};
+    for (int k = 0; k < 3; ++k) {
+        if (starts[k][0] != 0) return fail(MSW_EINVAL, "the first instance must start at entity 0");
+        for (int i = 1; i < n; ++i)
+            if (starts[k][i] <= starts[k][i - 1] || starts[k][i] >= totals[k])
+                return fail(MSW_EINVAL, "instance starts must increase and stay inside the mesh");
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->inst_c.assign(cell_start, cell_start + n);
+    ctx->inst_e.assign(edge_start, edge_start + n);
+    ctx->inst_v.assign(vertex_start, vertex_start + n);
+    ctx->patches = nullptr;   // rebuilt at the next persistent call (the old table stays allocated)
     return MSW_OK;
 }
 
@@ -982,6 +1105,7 @@ int msw_fwd_steps(msw_ctx* ctx, void* h, void* u, int nsteps)
     if (!members_aligned(ctx, h, u)) return fail(MSW_EINVAL, "h/u must be 16-byte aligned");
     int rc = check_cfg(ctx);
     if (rc) return rc;
+    if (ctx->persistent) return run_persistent(ctx, false, nullptr, nullptr, h, u, nsteps, nullptr, nullptr);
     if (ctx->prec == MSW_FP64)
         return run_steps(ctx, 1, h, u, nullptr, nullptr, nsteps,
                          [&] { enqueue_fwd_step<double>(ctx, static_cast<double*>(h), static_cast<double*>(u)); return MSW_OK; });
@@ -1010,6 +1134,7 @@ int msw_tlm_steps(msw_ctx* ctx, double* h, double* u, void* h_tl, void* u_tl, in
     if (!members_aligned(ctx, h_tl, u_tl)) return fail(MSW_EINVAL, "h_tl/u_tl must be 16-byte aligned");
     int rc = check_cfg(ctx);
     if (rc) return rc;
+    if (ctx->persistent) return run_persistent(ctx, true, h, u, h_tl, u_tl, nsteps, traj_h, traj_u);
     const size_t C = ctx->nC, E = ctx->nE;
     std::function<int(int)> pre = [&](int i) { return store_state(ctx, traj_h + i * C, traj_u + i * E, h, u); };
     const std::function<int(int)>* pp = traj_h ? &pre : nullptr;

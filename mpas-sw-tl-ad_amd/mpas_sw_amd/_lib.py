@@ -157,11 +157,14 @@ EXPORTS = (
     "msw_diagnostics_adj",
     "msw_diag_names", "msw_diag_kinds", "msw_reconstruct", "msw_reconstruct_adj", "msw_invariants", "msw_dot", "msw_nonfinite",
     "msw_permute", "msw_device_bytes", "msw_abi_version", "msw_last_error", "msw_dot_noalloc",
+    "msw_set_instances",
 )
 
 OPT_TL_FUSED_FLUX = 1   # msw_option
 OPT_XCD_STRIPE = 2
 OPT_BLOCK = 3
+OPT_PERSISTENT = 4
+OPT_BARRIER_POLLS = 5
 
 _lib = None
 
@@ -209,6 +212,7 @@ def load(path=None):
         "msw_dot": ([vp, vp, vp, vp, vp, vp, vp, vp], i32),
         "msw_nonfinite": ([vp, vp, vp, vp], i32),
         "msw_dot_noalloc": ([i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+        "msw_set_instances": ([vp, i32, vp, vp, vp], i32),
         "msw_permute": ([vp, i32, i32, i32, i32, vp, vp], i32),
         "msw_device_bytes": ([vp], i64),
         "msw_abi_version": ([], i32),

@@ -179,6 +179,22 @@ class SWModel:
         self.set_option(_lib.OPT_BLOCK, best[1])
         return times
 
+    def set_instances(self, offsets):
+        """The mesh is a disjoint union of instances (``mesh.union_mesh``):
+        offsets[k] = (first cell, first edge, first vertex) of instance k,
+        which the internal order keeps (``mesh.union_sfc_permutation``).  The
+        persistent launch (``set_persistent``) cuts its work items there."""
+        o = np.asarray(offsets, dtype=np.int32).reshape(-1, 3)
+        cs, es, vs = (np.ascontiguousarray(o[:, i]) for i in range(3))
+        _lib.check(self.lib.msw_set_instances(self._h, len(o), cs.ctypes.data, es.ctypes.data, vs.ctypes.data))
+
+    def set_persistent(self, on: bool = True, barrier_polls: int | None = None):
+        """Run fwd/tlm of a batch-1 model as one cooperative launch per call
+        (MSW_OPT_PERSISTENT); results equal the per-stage launches bit for bit."""
+        _lib.check(self.lib.msw_set_option(self._h, _lib.OPT_PERSISTENT, int(bool(on))))
+        if barrier_polls is not None:
+            _lib.check(self.lib.msw_set_option(self._h, _lib.OPT_BARRIER_POLLS, int(barrier_polls)))
+
     def set_mesh_scaling(self, del2=None, del4=None):
         """Per-edge factors of the del2 and del4 terms (meshScalingDel2/4 of
         compute_mesh_scaling, [E] reference order); None for both: all 1."""

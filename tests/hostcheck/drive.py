@@ -187,6 +187,39 @@ def other_entry_points(prec, B, graphs):
     no_violations(f"entry points prec={prec} B={B} graphs={graphs}")
 
 
+def persistent_path():
+    """The persistent launch of a batch-1 context split into instances: its
+    buffers, the error-flag read-back, and what it refuses."""
+    dev = Dev()
+    for prec in (FP64, FP32):
+        ctx = create(prec, 1)
+        s = state(dev, prec, 1, n=3)
+        cs = (C.c_int32 * 3)(0, 200, 400)
+        es = (C.c_int32 * 3)(0, 600, 1200)
+        vs = (C.c_int32 * 3)(0, 400, 800)
+        ok(lib.msw_set_instances(ctx, 3, cs, es, vs), "msw_set_instances")
+        ok(lib.msw_set_option(ctx, _lib.OPT_PERSISTENT, 1), "persistent")
+        ok(lib.msw_fwd_steps(ctx, s["ht"], s["ut"], 3), "persistent forward")
+        ok(lib.msw_tlm_steps(ctx, s["h"], s["u"], s["ht"], s["ut"], 3, s["th"], s["tu"]), "persistent TLM")
+        ok(lib.msw_tlm_steps(ctx, s["h"], s["u"], s["ht"], s["ut"], 2, None, None), "persistent TLM, no trajectory")
+        ok(lib.msw_adj_steps(ctx, s["th"], s["tu"], s["ht"], s["ut"], 3), "adjoint after a persistent TLM")
+        bad = (C.c_int32 * 3)(0, 700, 600)
+        if lib.msw_set_instances(ctx, 3, cs, bad, vs) != _lib.MSW_EINVAL:
+            fail("msw_set_instances accepted decreasing starts")
+        ok(lib.msw_set_config(ctx, C.byref(config(visc2=0.1))), "viscosity on")
+        if lib.msw_fwd_steps(ctx, s["ht"], s["ut"], 1) != _lib.MSW_EUNSUPPORTED:
+            fail("the persistent path accepted a viscosity")
+        ok(lib.msw_destroy(ctx), "destroy")
+    ctx = create(FP64, 64)
+    ok(lib.msw_set_option(ctx, _lib.OPT_PERSISTENT, 1), "persistent")
+    s = state(dev, FP64, 64, n=1)
+    if lib.msw_fwd_steps(ctx, s["ht"], s["ut"], 1) != _lib.MSW_EUNSUPPORTED:
+        fail("the persistent path accepted 64 members")
+    ok(lib.msw_destroy(ctx), "destroy")
+    dev.free()
+    no_violations("persistent path")
+
+
 def errors_reach_the_caller():
     """An injected failure of every copy, of a launch and of every allocation
     in msw_create comes back as a status, and the context keeps working."""
@@ -286,6 +319,8 @@ for prec, B in ((FP64, 1), (FP64, 64), (FP32, 64)):
     for graphs in (False, True):
         other_entry_points(prec, B, graphs)
         print(f"ok entry points prec={prec} B={B} graphs={graphs}", flush=True)
+persistent_path()
+print("ok persistent path", flush=True)
 errors_reach_the_caller()
 print("ok error paths", flush=True)
 graph_cache_is_bounded()

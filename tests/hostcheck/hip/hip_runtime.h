@@ -15,6 +15,7 @@ enum hipError_t {
     hipErrorInvalidConfiguration = 9,
     hipErrorInvalidResourceHandle = 400,
     hipErrorLaunchFailure = 719,
+    hipErrorCooperativeLaunchTooLarge = 720,
     hipErrorStreamCaptureUnsupported = 900,
     hipErrorStreamCaptureInvalidated = 901,
     hipErrorStreamCaptureUnjoined = 904,
@@ -39,6 +40,7 @@ enum hipStreamCaptureMode {
 #define hipStreamDefault 0x0
 #define hipStreamNonBlocking 0x1
 #define hipEventDefault 0x0
+#define hipHostMallocDefault 0x0
 #define hipEventDisableTiming 0x2
 
 struct ihipStream_t;
@@ -74,3 +76,10 @@ hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t s);
 hipError_t hipGetLastError(void);
 const char* hipGetErrorString(hipError_t e);
 hipError_t hipDeviceSynchronize(void);
+hipError_t hipHostMalloc(void** p, size_t bytes, unsigned flags);
+hipError_t hipHostFree(void* p);
+
+struct int2 {
+    int x, y;
+};
+inline int2 make_int2(int x, int y) { return int2{x, y}; }

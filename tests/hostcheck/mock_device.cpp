@@ -552,6 +552,19 @@ hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t st)
     return hipSuccess;
 }
 
+hipError_t hipHostMalloc(void** p, size_t bytes, unsigned)
+{
+    if (fail_now(2)) return hipErrorOutOfMemory;
+    *p = std::calloc(1, std::max<size_t>(bytes, 1));   // host memory: not in the device registry
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+
+hipError_t hipHostFree(void* p)
+{
+    std::free(p);
+    return hipSuccess;
+}
+
 hipError_t hipGetLastError(void)
 {
     const hipError_t e = sticky;
@@ -1033,6 +1046,48 @@ void diag_extra(const DMesh& m, const double* u, const double* vort, double* cir
                        W(div, C * D, "divergence"), W(vc, C * D, "vorticity_cell")});
     launch(s, "k_diag_extra", b);
 }
+
+template <typename T>
+int persistent_steps(const PersistArgs& a, bool tlm, hipStream_t s, int* grid)
+{
+    const DMesh& m = a.m;
+    check_shape(m, "persistent_steps");
+    const size_t C = m.nC, E = m.nE, V = m.nV, n = static_cast<size_t>(a.nsteps);
+    std::vector<Buf> b;
+    mesh_reads(m, b);
+    b.insert(b.end(), {R(a.patch, static_cast<size_t>(a.np_v + a.np_c + a.np_e) * sizeof(int2), "patches"),
+                       R(a.queue, 16 * 8, "queue"), W(a.queue, 16 * 8, "queue"), R(a.bar, 8, "barrier"), W(a.bar, 8, "barrier"),
+                       R(a.err, 4, "error flag"), W(a.err, 4, "error flag"),
+                       MR(static_cast<const T*>(a.h), C, 0, "h"), MW(static_cast<T*>(a.h), C, 0, "h"),
+                       MR(static_cast<const T*>(a.u), E, 0, "u"), MW(static_cast<T*>(a.u), E, 0, "u"),
+                       MW(static_cast<T*>(a.acc), C + E, 0, "acc"), MW(static_cast<T*>(a.pA), C + E, 0, "stage A"),
+                       MW(static_cast<T*>(a.pB), C + E, 0, "stage B"), MW(static_cast<T*>(a.pvv), V, 0, "pv_vertex"),
+                       MW(static_cast<T*>(a.ke), C, 0, "ke"), MW(static_cast<T*>(a.pc), C, 0, "pv_cell"),
+                       MW(static_cast<T*>(a.he), E, 0, "h_edge"), MW(static_cast<T*>(a.pe), E, 0, "pv_edge")});
+    if (a.np_v < 1 || a.np_c < 1 || a.np_e < 1) violation("persistent_steps: an entity kind has no work items");
+    if (tlm) {
+        b.insert(b.end(), {R(a.bh, C * D, "base h"), W(a.bh, C * D, "base h"), R(a.bu, E * D, "base u"),
+                           W(a.bu, E * D, "base u"), W(a.bacc, (C + E) * D, "base acc"), W(a.bend, (C + E) * D, "base end")});
+        for (int st = 0; st < 4; ++st) {
+            const BaseStage& q = a.st[st];
+            b.insert(b.end(), {W(q.h, C * D, "base stage h"), W(q.u, E * D, "base stage u"), W(q.he, E * D, "base h_edge"),
+                               W(q.pe, E * D, "base pv_edge"), W(q.v, E * D, "base v"), W(q.gpt, E * D, "base gradPVt"),
+                               W(q.gpn, E * D, "base gradPVn"), W(q.ke, C * D, "base ke"), W(q.pc, C * D, "base pv_cell"),
+                               W(q.pvv, V * D, "base pv_vertex"), W(q.hv, V * D, "base h_vertex"),
+                               W(q.vort, V * D, "base vorticity"), {q.rec, 4 * E * D, true, "base records", false, 32}});
+        }
+        if (a.traj_h) {
+            b.push_back(W(a.traj_h, n * C * D, "trajectory h"));
+            b.push_back(W(a.traj_u, n * E * D, "trajectory u"));
+        }
+    }
+    if (grid) *grid = 1024;
+    launch(s, "k_persist", b);
+    return 0;
+}
+
+template int persistent_steps<double>(const PersistArgs&, bool, hipStream_t, int*);
+template int persistent_steps<float>(const PersistArgs&, bool, hipStream_t, int*);
 
 #define INST(T)                                                                                                  \
     template void fwd_diag<T>(const DMesh&, const DConfig&, int, const T*, const T*, T*, T*, T*, T*, T*, T*, T*,  \

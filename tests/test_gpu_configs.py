@@ -595,7 +595,9 @@ def test_uneven_batch_full_size_fp32_forward_and_tlm():
     7 + 1 x level 9 (4 587 690 cells), alternating TC2/TC6, dt 18.75 s, FP32
     forward and FP32 TLM members (FP64 base) in one packed index space, 8
     steps.  Every instance finite; the FP64 base's mass (msw_invariants over
-    the packed state) conserved to 1e-13; the first instance of each level
+    the packed state) conserved to 1e-13; the persistent launch (one
+    cooperative launch per call over (instance, patch) work items) equal to
+    the per-stage launches bit for bit; the first instance of each level
     (and a TC6 one) equal bit for bit to that instance run alone; the FP32
     forward of one instance per level within 1e-5 relative L2 of the FP64
     oracle (levels 5, 6, 7) or of the FP64 GPU forward (level 9, which the
@@ -631,7 +633,20 @@ def test_uneven_batch_full_size_fp32_forward_and_tlm():
     got = [m.to_reference(x, k) for x, k in ((hf, "c"), (uf, "e"), (ht, "c"), (ut, "e"))]
     for g in got:
         assert np.all(np.isfinite(g))
-    del m, hf, uf, h, u, ht, ut
+    # the same window as one persistent launch per call over (instance, patch) items
+    m.set_instances(offs)
+    m.set_persistent(True)
+    hf2, uf2 = m.to_internal(phys["h"], "c"), m.to_internal(phys["u"], "e")
+    m.fwd(hf2, uf2, n)
+    h2 = m.to_internal(phys["h"], "c", member=False)
+    u2 = m.to_internal(phys["u"], "e", member=False)
+    ht2, ut2 = m.to_internal(dh[:, 0], "c"), m.to_internal(du[:, 0], "e")
+    m.tlm(h2, u2, ht2, ut2, n)
+    torch.cuda.synchronize()
+    for a, b in ((hf, hf2), (uf, uf2), (h, h2), (u, u2), (ht, ht2), (ut, ut2)):
+        assert torch.equal(a, b)
+    print(f"uneven batch: persistent launch equal to the stage launches after {time.time() - t0:.0f} s", flush=True)
+    del m, hf, uf, h, u, ht, ut, hf2, uf2, h2, u2, ht2, ut2
     torch.cuda.empty_cache()
     for i in (0, 1, 64, 80, 84):
         (level, tc), (oc, oe, ov) = specs[i], offs[i]

@@ -842,3 +842,69 @@ def test_diagnostics_adjoint_matches_oracle(B, prec, tol):
         oh, ou = o.diag_adj(h0, u0)
         assert rel_l2(HA[:, b] - base_h[:, b], oh) < tol, b
         assert rel_l2(UA[:, b] - base_u[:, b], ou) < tol, b
+
+
+@pytest.mark.parametrize("prec,fused", [("fp32", True), ("fp32", False), ("fp64", True)])
+def test_persistent_launch_equals_stage_launches(prec, fused):
+    """The uneven-batch layout (16 x level 2 + 4 x level 3 + 1 x level 4,
+    alternating TC2/TC6) with one member per entity: the persistent launch
+    (one cooperative launch per call, (instance, patch) work queue, grid
+    barriers between passes) gives the per-stage launches' forward, TLM and
+    stored base trajectory bit for bit, with and without instance
+    boundaries; the adjoint sweep over its trajectory equals the one over
+    the per-stage trajectory."""
+    print(f"persistent vs stage launches, packed L2-L4, {prec}, fused {fused}", flush=True)
+    from mpas_sw_amd.core import SWModel
+    from mpas_sw_amd.ensemble import packed_instances
+    levels = [2] * 16 + [3] * 4 + [4]
+    specs = [(lv, 2 if i % 2 == 0 else 6) for i, lv in enumerate(levels)]
+    dt, n = 300.0, 6
+    mesh, phys, offs, perm = packed_instances(specs)
+    dh, du = perturbations(mesh, 1)
+    runs = []
+    for mode in ("stages", "persistent", "persistent+instances"):
+        m = SWModel(mesh, phys, dt, test_case=2, perm=perm, precision=prec, graphs=True,
+                    tl_fused_flux=fused)
+        if mode != "stages":
+            m.set_persistent(True)
+        if mode == "persistent+instances":
+            m.set_instances(offs)
+        hf, uf = m.to_internal(phys["h"], "c"), m.to_internal(phys["u"], "e")
+        m.fwd(hf, uf, n)
+        h = m.to_internal(phys["h"], "c", member=False)
+        u = m.to_internal(phys["u"], "e", member=False)
+        ht, ut = m.to_internal(dh[:, 0], "c"), m.to_internal(du[:, 0], "e")
+        th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+        tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+        m.tlm(h, u, ht, ut, n, th, tu)
+        ah, au = ht.clone(), ut.clone()
+        m.set_persistent(False)
+        m.adj(th, tu, ah, au, n)
+        torch.cuda.synchronize()
+        runs.append([x.clone() for x in (hf, uf, h, u, ht, ut, th, tu, ah, au)])
+        del m
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert torch.equal(a, b)
+
+
+def test_persistent_barrier_gives_up_without_hanging():
+    """A grid barrier of the persistent launch that may poll only once gives
+    up: every block leaves the kernel and the call returns MSW_EHIP; the same
+    context then runs normally and matches the per-stage launches."""
+    print("persistent launch, barrier poll limit 1", flush=True)
+    from mpas_sw_amd._lib import MSWError
+    mesh, phys = case(LEVEL, 2)
+    m = model(LEVEL, 2, 600.0, precision="fp32")
+    m.set_persistent(True, barrier_polls=1)
+    h, u = m.to_internal(phys["h"], "c"), m.to_internal(phys["u"], "e")
+    with pytest.raises(MSWError, match="grid barrier"):
+        m.fwd(h, u, 4)
+    m.set_persistent(True, barrier_polls=1 << 20)
+    h, u = m.to_internal(phys["h"], "c"), m.to_internal(phys["u"], "e")
+    m.fwd(h, u, 4)
+    m.set_persistent(False)
+    h2, u2 = m.to_internal(phys["h"], "c"), m.to_internal(phys["u"], "e")
+    m.fwd(h2, u2, 4)
+    torch.cuda.synchronize()
+    assert torch.equal(h, h2) and torch.equal(u, u2)
