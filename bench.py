@@ -47,6 +47,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def library_sha256():
+    """SHA-256 of the HIP library this process loads (ties counter summaries
+    to the code they measured)."""
+    import hashlib
+    from mpas_sw_amd import _lib
+    with open(_lib.LIB_PATH, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def git_head():
+    try:
+        r = subprocess.run(["git", "-C", ROOT, "rev-parse", "--short=12", "HEAD"], capture_output=True, text=True)
+        return r.stdout.strip() or None
+    except OSError:
+        return None
+
+
 def perturbations(mesh, nmem):
     import numpy as np
     import torch
@@ -172,6 +189,7 @@ def traffic_from_counters(fetch_dir, write_dir, nsteps, out_json, out_md, comman
     total = sum(d["read"] + d["write"] for d in per.values())
     res = {"hbm_bytes_per_step": total / nsteps, "window_steps": nsteps,
            "fetch_scale": fs, "write_scale": ws, "command": command,
+           "library_sha256": library_sha256(), "commit": git_head(),
            "kernels": {k: {"read_bytes_per_call": v["read"] / max(v["calls"], 1),
                            "write_bytes_per_call": v["write"] / max(v["calls"], 1),
                            "calls": v["calls"]} for k, v in per.items()}}
@@ -180,6 +198,8 @@ def traffic_from_counters(fetch_dir, write_dir, nsteps, out_json, out_md, comman
     with open(out_md, "w") as f:
         f.write("# HBM traffic of the batched TLM+ADJ step (level 7, TC6, 64 members, FP64)\n\n")
         f.write(f"Commands (from the repository root):\n\n```\n{command}\n```\n\n")
+        f.write(f"Library: `libmpas_sw_amd.so` SHA-256 `{res['library_sha256']}` (commit {res['commit']}); "
+                "bench.py reports this traffic only for a library with the same hash.\n\n")
         f.write(f"Calibration: the 1 GiB device copy reads FETCH_SIZE = {fraw:.0f} KiB and "
                 f"WRITE_SIZE = {wraw:.0f} KiB, so FETCH_SIZE is scaled by {fs:.3f} and "
                 f"WRITE_SIZE by {ws:.3f}.\n\n")
@@ -411,8 +431,14 @@ def main():
     }
     prof = os.path.join(ROOT, "profiles", "traffic_tlm_adj_l7_b64_fp64.json")
     if os.path.isfile(prof):
+        # counter bytes measured on this very library only; otherwise left out
         with open(prof) as f:
-            roofline["traffic"] = json.load(f).get("hbm_bytes_per_step")
+            tr = json.load(f)
+        if tr.get("library_sha256") == library_sha256():
+            roofline["traffic"] = tr.get("hbm_bytes_per_step")
+            roofline["traffic_source"] = f"profiles/traffic_tlm_adj_l7_b64_fp64.json (commit {tr.get('commit')})"
+        else:
+            roofline["traffic_source"] = "not measured for this library build (counter summary is of another build)"
 
     cpu = None
     if not args.no_cpu:
@@ -446,6 +472,7 @@ def main():
         "batched_vs_plain": None if plain is None else res["rate"] / plain["rate"],
         "fp32": None if f32 is None else {
             "value": f32["rate"], "ms_per_step": 1e3 * f32["seconds"] / K,
+            "xcd_stripe_log2": f32["stripe_log2"], "block": f32["block"],
             "roofline_frac": round(BYTES_PER_CELL_STEP["f32"] * mesh.nCells * BATCH * K /
                                    f32["seconds"] / 1e9 / HBM_PEAK_GBS, 4)},
         "tc6_physical_f": None if physf is None else {
