@@ -153,7 +153,7 @@ def calibration_copy(nbytes=1 << 30, reps=3):
 
 def traffic_from_counters(fetch_dir, write_dir, nsteps, out_json, out_md, command):
     """HBM bytes per TLM+ADJ step from two rocprofv3 --pmc runs of
-    `bench.py --steps S --warmup W --no-cpu --no-plain --no-fp32 --no-tune --calibrate`
+    `bench.py --steps S --warmup W --no-cpu --no-plain --no-fp32 --no-physical-f --no-tune --calibrate`
     (FETCH_SIZE, then WRITE_SIZE; counters in KiB per dispatch).  Each counter
     is scaled by what it reads for the 1 GiB calibration copy; the library's
     kernels over the whole run are summed and divided by the S + W window
@@ -357,13 +357,14 @@ def main():
                     help="time the level-8 checkpointed gradient (FP64 and FP32 adjoint) and exit")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
-                         "`--no-cpu --no-plain --no-fp32 --calibrate` into profiles/ and exit")
+                         "`--no-cpu --no-plain --no-fp32 --no-physical-f --calibrate` into profiles/ and exit")
     args = ap.parse_args()
     if args.traffic_from:
+        legs = "--no-cpu --no-plain --no-fp32 --no-physical-f --no-tune --calibrate"
         cmd = ("rocprofv3 --pmc FETCH_SIZE --output-format csv -d prof/fetch -o run -- python3 bench.py "
-               f"--steps {args.steps} --warmup {args.warmup} --no-cpu --no-plain --no-fp32 --no-tune --calibrate\n"
+               f"--steps {args.steps} --warmup {args.warmup} {legs}\n"
                "rocprofv3 --pmc WRITE_SIZE --output-format csv -d prof/write -o run -- python3 bench.py "
-               f"--steps {args.steps} --warmup {args.warmup} --no-cpu --no-plain --no-fp32 --no-tune --calibrate\n"
+               f"--steps {args.steps} --warmup {args.warmup} {legs}\n"
                f"python3 bench.py --steps {args.steps} --warmup {args.warmup} --traffic-from "
                "prof/fetch prof/write")
         r = traffic_from_counters(args.traffic_from[0], args.traffic_from[1], args.steps + args.warmup,
