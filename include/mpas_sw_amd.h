@@ -111,6 +111,24 @@ int msw_set_forcing(msw_ctx *ctx, const double *u_src);
 int msw_set_mesh_scaling(msw_ctx *ctx, const double *del2, const double *del4);
 /* Use a captured hipGraph per step for the step functions (default off). */
 int msw_set_graphs(msw_ctx *ctx, int enable);
+/* Replaces: the module arrays tend_h, tend_u, tend_h_tl, tend_u_tl
+ * (src/mpas_sw_module.f90:188-195), which the last RK stage leaves holding
+ * its tendencies after sw_rk4 (:577-586), sw_rk4_tlm (:700-712) and
+ * sw_rk4_adj (its recomputed forward step, :838-848).  When set, the step
+ * entry points write the last stage's tendencies here at every step, so
+ * after a call they hold those of the last step run (for msw_adj_steps the
+ * earliest step of the window, which its reverse sweep runs last):
+ *   msw_fwd_steps: tend_h [nCells][B], tend_u [nEdges][B], the members'
+ *     tendencies in the member precision;
+ *   msw_base_steps, msw_tlm_steps, msw_adj_step, msw_adj_steps: tend_h
+ *     [nCells], tend_u [nEdges], FP64, of the base state;
+ *   msw_tlm_steps: tend_h_tl [nCells][B], tend_u_tl [nEdges][B], the
+ *     members' TL tendencies in the member precision.
+ * Device pointers in the internal order; NULL leaves that array unwritten
+ * (the default for all four).  While any is set the steps run as stage
+ * launches (MSW_OPT_PERSISTENT does not apply).  Waits for the creation
+ * stream; captured step graphs are dropped. */
+int msw_set_tend_output(msw_ctx *ctx, void *tend_h, void *tend_u, void *tend_h_tl, void *tend_u_tl);
 
 /* Options without a counterpart in the reference.
  * MSW_OPT_TL_FUSED_FLUX (default 0): 1 forms the TL tendency of the TLM step

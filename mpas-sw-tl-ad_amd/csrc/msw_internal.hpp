@@ -138,7 +138,7 @@ RKCoeffs rk_coeffs(double dt);
 // Nonlinear stage: diagnostics at state (h,u) into `st` (which may alias
 // `st.h/st.u` = (h,u)), then tendency and the RK4 update of stage `s`:
 //   acc = (s==0 ? x : acc) + w[s]*tend ;  pnext = x + sw[s]*tend (s<3)
-//   s==3: x = acc + w[3]*tend
+//   s==3: x = acc + w[3]*tend, and tend into (toh, tou) when they are set
 template <typename T>
 void fwd_diag(const DMesh& m, const DConfig& c, int lb, const T* h, const T* u,
               T* pvv, T* hv, T* vort, T* ke, T* pc, T* he, T* pe, T* v, T* gpt, T* gpn,
@@ -147,7 +147,7 @@ template <typename T>
 void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
                         const T* h, const T* u, const T* he, const T* pe, const T* ke,
                         const Dissip<T>& dp, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
-                        const double* rec, hipStream_t s);
+                        const double* rec, T* toh, T* tou, hipStream_t s);
 
 // Scratch of the del2/del4 terms for one field set (members or base):
 // div [C], dsu [E], dsd [C], dsv [V]; vort [V] comes from the vertex pass.
@@ -170,13 +170,14 @@ void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef
 void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s);
 
 // Tangent-linear stage on top of base fields `b` of the same stage.
-// vort_tl [V] and ds are used only when a viscosity is on.
+// vort_tl [V] and ds are used only when a viscosity is on; the last stage
+// (stage 3) leaves its TL tendencies in (toh, tou) when they are set.
 template <typename T>
 void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
               const BaseStage& b, const T* h_tl, const T* u_tl,
               T* pvv_tl, T* ke_tl, T* pc_tl, T* pe_tl, T* he_tl, T* vort_tl,
               const DissipScratch<T>& ds, const AdjCoef& q,
-              T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s);
+              T* xh, T* xu, T* acch, T* accu, T* ph, T* pu, T* toh, T* tou, hipStream_t s);
 
 // Adjoint of one RK4 stage: pa = F'(p_s)^T g, then the epilogue
 //   s>0: accx = (first ? pa : accx + pa) ; g = w[s-1]*lam + sw[s-1]*pa

@@ -306,10 +306,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd_edge(DMesh m, DConfig c, int 
     }
 }
 
-// RK4 bookkeeping of one stage for one value (src/mpas_sw_module.f90:590-611)
+// RK4 bookkeeping of one stage for one value (src/mpas_sw_module.f90:590-611).
+// tout (optional): where the last stage leaves its tendency, as the module
+// arrays tend_h/tend_u hold it after a step (:578-586)
 template <typename T>
 __device__ __forceinline__ void rk_update(int stage, T w, T sw, T tend, T* x, T* acc, T* pnext,
-                                          int row, int B, int o, bool frozen)
+                                          int row, int B, int o, bool frozen, T* tout = nullptr)
 {
     x = rowp(x, row, B);
     acc = rowp(acc, row, B);
@@ -324,6 +326,7 @@ __device__ __forceinline__ void rk_update(int stage, T w, T sw, T tend, T* x, T*
         stn(pnext + o, frozen ? xv : xv + sw * tend);
     } else {
         if (!frozen) stn(x + o, ldn(acc + o) + w * tend);
+        if (tout) stn(rowp(tout, row, B) + o, tend);
     }
 }
 
@@ -465,7 +468,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd_tend(DMesh m, DConfig c, int 
                                                      const T* __restrict__ pe,
                                                      const T* __restrict__ ke, Dissip<T> dp,
                                                      T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
-                                                     const double* __restrict__ rec)
+                                                     const double* __restrict__ rec,
+                                                     T* __restrict__ toh, T* __restrict__ tou)
 {
     const int B = 1 << lb;
     EDGE_CELL_ITEM(is_edge, t)
@@ -509,7 +513,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd_tend(DMesh m, DConfig c, int 
                 tend = tend_forcing(m, c, dp, e, c1, c2, B, b, tend, ld(u, e, B, b), ld(he, e, B, b),
                                     k1, k2);
             rk_update(stage, w, sw, tend, xu, accu, pu, e, B, b,
-                      c.frozen_u != 0);
+                      c.frozen_u != 0, tou);
         } else {
             const int ic = entity_of<WU>(t, lb);
             int ie[kC6];
@@ -525,7 +529,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd_tend(DMesh m, DConfig c, int 
                 th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
             }
             th = th / T(m.areaC[ic]);
-            rk_update(stage, w, sw, th, xh, acch, ph, ic, B, b, false);
+            rk_update(stage, w, sw, th, xh, acch, ph, ic, B, b, false, toh);
         }
     }
 }
@@ -735,7 +739,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_tend(DMesh m, DConfig c, int l
                                                     const T* __restrict__ pe_tl,
                                                     const T* __restrict__ he_tl,
                                                     const T* __restrict__ ke_tl, Dissip<T> dp,
-                                                    T* xh, T* xu, T* acch, T* accu, T* ph, T* pu)
+                                                    T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
+                                                    T* __restrict__ toh, T* __restrict__ tou)
 {
     const int B = 1 << lb;
     EDGE_CELL_ITEM(is_edge, t)
@@ -778,7 +783,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_tend(DMesh m, DConfig c, int l
             if (c.visc2 > 0.0 || c.visc4 > 0.0 || c.wind || c.drag)
                 tend = tend_forcing_tl(m, c, dp, bs, e, c1, c2, B, b, tend, ld(u_tl, e, B, b),
                                        FUSED ? T(0.5) * (h1 + h2) : ld(he_tl, e, B, b), k1, k2);
-            rk_update(stage, w, sw, tend, xu, accu, pu, e, B, b, c.frozen_u != 0);
+            rk_update(stage, w, sw, tend, xu, accu, pu, e, B, b, c.frozen_u != 0, tou);
         } else {
             const int ic = entity_of<WU>(t, lb);
             int ie[kC6];
@@ -803,7 +808,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_tend(DMesh m, DConfig c, int l
                 }
             }
             th = th / T(m.areaC[ic]);
-            rk_update(stage, w, sw, th, xh, acch, ph, ic, B, b, false);
+            rk_update(stage, w, sw, th, xh, acch, ph, ic, B, b, false, toh);
         }
     }
 }
@@ -2074,10 +2079,11 @@ template <typename T>
 void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
                         const T* h, const T* u, const T* he, const T* pe, const T* ke,
                         const Dissip<T>& dp, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
-                        const double* rec, hipStream_t s)
+                        const double* rec, T* toh, T* tou, hipStream_t s)
 {
     LAUNCH_WU(k_fwd_tend, T, lb, grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2, m.block), m.block, s, m, c, lb, stage, T(k.w[stage]),
-              T(k.sw[stage]), h, u, he, pe, ke, dp, xh, xu, acch, accu, ph, pu, lb == 0 ? rec : nullptr);
+              T(k.sw[stage]), h, u, he, pe, ke, dp, xh, xu, acch, accu, ph, pu, lb == 0 ? rec : nullptr,
+              stage == 3 ? toh : nullptr, stage == 3 ? tou : nullptr);
 }
 
 template <typename T>
@@ -2122,7 +2128,7 @@ template <typename T>
 void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k,
               const BaseStage& b, const T* h_tl, const T* u_tl, T* pvv_tl, T* ke_tl, T* pc_tl,
               T* pe_tl, T* he_tl, T* vort_tl, const DissipScratch<T>& ds, const AdjCoef& q, T* xh,
-              T* xu, T* acch, T* accu, T* ph, T* pu, hipStream_t s)
+              T* xu, T* acch, T* accu, T* ph, T* pu, T* toh, T* tou, hipStream_t s)
 {
     const bool visc = c.visc2 > 0.0 || c.visc4 > 0.0;
     LAUNCH_WU(k_tl_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2, m.block), m.block, s, m, lb, b.hv, b.vort,
@@ -2136,7 +2142,8 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
     // gets its own register allocation
     const int g = grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2, m.block);
 #define TL_TEND_ARGS s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), b, q.e48, q.c32, h_tl, u_tl, \
-                     pe_tl, he_tl, ke_tl, dp, xh, xu, acch, accu, ph, pu
+                     pe_tl, he_tl, ke_tl, dp, xh, xu, acch, accu, ph, pu, stage == 3 ? toh : nullptr, \
+                     stage == 3 ? tou : nullptr
     if (lb >= 6) {
         if (c.tl_fused)
             hipLaunchKernelGGL((k_tl_tend<T, true, true>), dim3(g), dim3(m.block), 0, TL_TEND_ARGS);
@@ -2336,13 +2343,13 @@ template int persistent_steps<float>(const PersistArgs&, bool, hipStream_t, int*
     template void fwd_tend_update_he<T>(const DMesh&, const DConfig&, int, int, const RKCoeffs&,    \
                                         const T*, const T*, const T*, const T*, const T*,           \
                                         const Dissip<T>&, T*, T*, T*, T*, T*, T*, const double*,    \
-                                        hipStream_t);                                               \
+                                        T*, T*, hipStream_t);                                       \
     template Dissip<T> dissip_fields<T>(const DMesh&, const DConfig&, int, const T*, const T*,       \
                                         const DissipScratch<T>&, hipStream_t);                      \
     template void tl_stage<T>(const DMesh&, const DConfig&, int, int, const RKCoeffs&,              \
                               const BaseStage&, const T*, const T*, T*, T*, T*, T*, T*, T*,         \
                               const DissipScratch<T>&, const AdjCoef&, T*, T*, T*, T*, T*, T*,      \
-                              hipStream_t);                                                         \
+                              T*, T*, hipStream_t);                                                 \
     template void adj_stage<T>(const DMesh&, const DConfig&, int, int, bool, const RKCoeffs&,       \
                                const BaseStage&, T*, T*, T*, T*, T*, T*, T*, T*, T*, T*,            \
                                const DissipScratch<T>&, const AdjCoef&, T*, T*, T*, T*,             \

@@ -124,6 +124,9 @@ struct msw_ctx {
     int* hflag = nullptr;                   // pinned host copy of the error flag
     double* dpartial = nullptr;       // dot partials
     int dot_nblk = 0;
+    // msw_set_tend_output: where the last RK stage leaves its tendencies
+    // (h, u, h_tl, u_tl); null: not kept
+    void* tout[4]{};
 
     // graphs, keyed by (kind * 16 + variant, the pointers they bake in);
     // at most kMaxGraphs are kept, the least recently launched is dropped
@@ -511,13 +514,15 @@ DissipScratch<T> member_ds(msw_ctx* c)
 DissipScratch<double> base_ds(msw_ctx* c) { return {c->bdiv, c->bdsu, c->bdsd, c->bdsv}; }
 
 // base (B = 1) tendency + RK update of one stage, with the dissipation terms
+// (the last stage leaves its tendency in tout[0..1] when they are set)
 void base_tend(msw_ctx* c, const BaseStage& b, int s, const RKCoeffs& k, double* xh, double* xu,
                double* nh, hipStream_t st)
 {
     const Dissip<double> dp = dissip_fields<double>(c->dm, c->dcfg, 0, b.u, b.vort, base_ds(c), st);
     const size_t C = c->nC;
     fwd_tend_update_he<double>(c->dm, c->dcfg, 0, s, k, b.h, b.u, b.he, b.pe, b.ke, dp, xh, xu,
-                               c->bacc, c->bacc + C, nh, nh + C, b.rec, st);
+                               c->bacc, c->bacc + C, nh, nh + C, b.rec, static_cast<double*>(c->tout[0]),
+                               static_cast<double*>(c->tout[1]), st);
 }
 
 // The side stream joins the main stream's work (and a capture in progress).
@@ -574,7 +579,8 @@ void enqueue_fwd_step(msw_ctx* c, T* h, T* u)
         const Dissip<T> dp = dissip_fields<T>(m, c->dcfg, c->lb, ru, vort, member_ds<T>(c), c->stream);
         T* nb = (s % 2 == 0) ? pA : pB;
         fwd_tend_update_he<T>(m, c->dcfg, c->lb, s, k, rh, ru, he, pe, ke, dp, h, u, acc,
-                              acc + C * B, nb, nb + C * B, nullptr, c->stream);
+                              acc + C * B, nb, nb + C * B, nullptr, static_cast<T*>(c->tout[0]),
+                              static_cast<T*>(c->tout[1]), c->stream);
         rh = nb;
         ru = nb + C * B;
     }
@@ -641,7 +647,8 @@ int enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
         T* nb = (s % 2 == 0) ? pA : pB;
         tl_stage<T>(m, c->dcfg, c->lb, s, k, c->st[s], th, tu, pvv_tl, ke_tl, pc_tl, pe_tl, he_tl,
                     static_cast<T*>(c->xvort), member_ds<T>(c), c->qs[s], htl, utl, acc,
-                    acc + C * B, nb, nb + C * B, c->stream);
+                    acc + C * B, nb, nb + C * B, static_cast<T*>(c->tout[2]), static_cast<T*>(c->tout[3]),
+                    c->stream);
         th = nb;
         tu = nb + C * B;
     }
@@ -664,7 +671,7 @@ int side_adj_base(msw_ctx* c, BaseStage* st, AdjCoef* qs, double* x, bool want_e
         BaseStage& b = st[s];
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
                          b.gpt, b.gpn, b.rec, ss);
-        if (s < 3 || want_end) {
+        if (s < 3 || want_end || c->tout[0] || c->tout[1]) {
             // stage 3 writes x_new into its "x" argument: give it the end buffer
             double* nh = s < 3 ? st[s + 1].h : c->bend;
             double* wxh = s < 3 ? x : c->bend;
@@ -1084,11 +1091,29 @@ int msw_set_instances(msw_ctx* ctx, int n, const int32_t* cell_start, const int3
     return MSW_OK;
 }
 
+int msw_set_tend_output(msw_ctx* ctx, void* tend_h, void* tend_u, void* tend_h_tl, void* tend_u_tl)
+{
+    if (!ctx) return fail(MSW_EINVAL, "NULL ctx");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->tout[0] = tend_h;
+    ctx->tout[1] = tend_u;
+    ctx->tout[2] = tend_h_tl;
+    ctx->tout[3] = tend_u_tl;
+    ctx->clear_graphs();   // the pointers are launch arguments of the captured steps
+    return MSW_OK;
+}
+
 int msw_set_graphs(msw_ctx* ctx, int enable)
 {
     if (!ctx) return fail(MSW_EINVAL, "NULL ctx");
     ctx->use_graphs = enable != 0;
     return MSW_OK;
+}
+
+// a tendency output is set: the steps run as stage launches, which write it
+static bool tend_kept(const msw_ctx* ctx)
+{
+    return ctx->tout[0] || ctx->tout[1] || ctx->tout[2] || ctx->tout[3];
 }
 
 // FP64 member arrays of a 64-member-or-larger context move as 16-byte pairs
@@ -1105,7 +1130,8 @@ int msw_fwd_steps(msw_ctx* ctx, void* h, void* u, int nsteps)
     if (!members_aligned(ctx, h, u)) return fail(MSW_EINVAL, "h/u must be 16-byte aligned");
     int rc = check_cfg(ctx);
     if (rc) return rc;
-    if (ctx->persistent) return run_persistent(ctx, false, nullptr, nullptr, h, u, nsteps, nullptr, nullptr);
+    if (ctx->persistent && !tend_kept(ctx))
+        return run_persistent(ctx, false, nullptr, nullptr, h, u, nsteps, nullptr, nullptr);
     if (ctx->prec == MSW_FP64)
         return run_steps(ctx, 1, h, u, nullptr, nullptr, nsteps,
                          [&] { enqueue_fwd_step<double>(ctx, static_cast<double*>(h), static_cast<double*>(u)); return MSW_OK; });
@@ -1134,7 +1160,8 @@ int msw_tlm_steps(msw_ctx* ctx, double* h, double* u, void* h_tl, void* u_tl, in
     if (!members_aligned(ctx, h_tl, u_tl)) return fail(MSW_EINVAL, "h_tl/u_tl must be 16-byte aligned");
     int rc = check_cfg(ctx);
     if (rc) return rc;
-    if (ctx->persistent) return run_persistent(ctx, true, h, u, h_tl, u_tl, nsteps, traj_h, traj_u);
+    if (ctx->persistent && !tend_kept(ctx))
+        return run_persistent(ctx, true, h, u, h_tl, u_tl, nsteps, traj_h, traj_u);
     const size_t C = ctx->nC, E = ctx->nE;
     std::function<int(int)> pre = [&](int i) { return store_state(ctx, traj_h + i * C, traj_u + i * E, h, u); };
     const std::function<int(int)>* pp = traj_h ? &pre : nullptr;

@@ -298,7 +298,14 @@ class _Module:
                         visc4=v["config_h_mom_eddy_visc4"],
                         wind_stress=bool(v["config_wind_stress"]),
                         bottom_drag=bool(v["config_bottom_drag"]))
-            gpu = {"model": m, "key": key}
+            # the last RK stage's tendencies, which the module arrays tend_h,
+            # tend_u, tend_h_tl, tend_u_tl hold after each step
+            # (src/mpas_sw_module.f90:188-195, :577-586, :700-712, :838-848)
+            import torch
+            tend = tuple(torch.zeros(n, dtype=torch.float64, device=m.device)
+                         for n in (m.nC, m.nE, m.nC, m.nE))
+            m.set_tend_output(*tend)
+            gpu = {"model": m, "key": key, "tend": tend}
             object.__setattr__(self, "_gpu", gpu)
         elif gpu["key"] != key:
             gpu["model"].set_config(dt=v["config_dt"], apvm=v["config_apvm_upwinding"],
@@ -326,6 +333,16 @@ class _Module:
 
     def _up(self, m, arr, kind):
         return m.to_internal(np.asarray(arr), kind, member=False)
+
+    def _tend_back(self, m, names):
+        """Copy the tendencies the last step left on the GPU into the module
+        arrays `names` (those that are allocated)."""
+        th, tu, tht, tut = self._gpu["tend"]
+        src = {"tend_h": (th, "c"), "tend_u": (tu, "e"), "tend_h_tl": (tht, "c"), "tend_u_tl": (tut, "e")}
+        for n in names:
+            if n in self._arrays:
+                t, kind = src[n]
+                self._arrays[n][0, :] = m.to_reference(t, kind)
 
     def _set_diag_state(self, tag, h, u, u_tl=None):
         object.__setattr__(self, "_diag_state", (tag, h, u, u_tl))
@@ -401,6 +418,7 @@ class _Module:
         A["h"][0, 0, :] = hn; A["h"][1, 0, :] = hn
         A["u"][0, 0, :] = un; A["u"][1, 0, :] = un
         A["tracers"][1] = A["tracers"][0]
+        self._tend_back(m, ("tend_h", "tend_u"))
         self._set_diag_state("fwd", h, u)
 
     def sw_rk4_tlm(self):
@@ -421,6 +439,7 @@ class _Module:
             A[name][1, 0, :] = x
         A["tracers"][1] = A["tracers"][0]
         A["tracers_tl"][1] = A["tracers_tl"][0]
+        self._tend_back(m, ("tend_h", "tend_u", "tend_h_tl", "tend_u_tl"))
         self._set_diag_state("tlm", h, u, ut)
 
     def sw_rk4_adj(self):
@@ -473,6 +492,7 @@ class _Module:
         A["u"][1, 0, :] = m.to_reference(ue, "e")
         for n in ("tend_h_ad", "tend_u_ad"):
             A[n][...] = 0.0
+        self._tend_back(m, ("tend_h", "tend_u"))
         self._set_diag_state("adj", h, u)
 
     def link_tlm_adj(self):

@@ -157,7 +157,7 @@ EXPORTS = (
     "msw_diagnostics_adj",
     "msw_diag_names", "msw_diag_kinds", "msw_reconstruct", "msw_reconstruct_adj", "msw_invariants", "msw_dot", "msw_nonfinite",
     "msw_permute", "msw_device_bytes", "msw_abi_version", "msw_last_error", "msw_dot_noalloc",
-    "msw_set_instances",
+    "msw_set_instances", "msw_set_tend_output",
 )
 
 OPT_TL_FUSED_FLUX = 1   # msw_option
@@ -213,6 +213,7 @@ def load(path=None):
         "msw_nonfinite": ([vp, vp, vp, vp], i32),
         "msw_dot_noalloc": ([i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp], i32),
         "msw_set_instances": ([vp, i32, vp, vp, vp], i32),
+        "msw_set_tend_output": ([vp, vp, vp, vp, vp], i32),
         "msw_permute": ([vp, i32, i32, i32, i32, vp, vp], i32),
         "msw_device_bytes": ([vp], i64),
         "msw_abi_version": ([], i32),

@@ -188,6 +188,20 @@ class SWModel:
         cs, es, vs = (np.ascontiguousarray(o[:, i]) for i in range(3))
         _lib.check(self.lib.msw_set_instances(self._h, len(o), cs.ctypes.data, es.ctypes.data, vs.ctypes.data))
 
+    def set_tend_output(self, tend_h=None, tend_u=None, tend_h_tl=None, tend_u_tl=None):
+        """Keep the last RK stage's tendencies of every step (the module arrays
+        tend_h, tend_u, tend_h_tl, tend_u_tl of the reference), internal order:
+        fwd() writes the members' into tend_h/tend_u ([C, B], [E, B] of the
+        member dtype); base(), tlm(), adj_step() and adj() write the FP64 base
+        state's ([C], [E]) there; tlm() writes the members' TL tendencies into
+        tend_h_tl/tend_u_tl.  None leaves an array unwritten; all None stops."""
+        arrs = (tend_h, tend_u, tend_h_tl, tend_u_tl)
+        for a in arrs:
+            if a is not None and (not a.is_cuda or not a.is_contiguous()):
+                raise ValueError("tendency outputs must be contiguous device tensors")
+        self._tend_out = arrs   # the library keeps the pointers
+        _lib.check(self.lib.msw_set_tend_output(self._h, *[None if a is None else a.data_ptr() for a in arrs]))
+
     def set_persistent(self, on: bool = True, barrier_polls: int | None = None):
         """Run fwd/tlm of a batch-1 model as one cooperative launch per call
         (MSW_OPT_PERSISTENT); results equal the per-stage launches bit for bit."""

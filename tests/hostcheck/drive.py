@@ -187,6 +187,36 @@ def other_entry_points(prec, B, graphs):
     no_violations(f"entry points prec={prec} B={B} graphs={graphs}")
 
 
+def tend_outputs(prec, B, graphs):
+    """msw_set_tend_output with buffers of exactly the documented sizes: the
+    members' tendencies for msw_fwd_steps, the FP64 base tendencies and the
+    members' TL tendencies for the other steps (persistent option on: the
+    steps must still run as stage launches that write them)."""
+    es = 8 if prec == FP64 else 4
+    ctx = create(prec, B)
+    ok(lib.msw_set_graphs(ctx, int(graphs)), "graphs")
+    if B == 1:
+        ok(lib.msw_set_option(ctx, _lib.OPT_PERSISTENT, 1), "persistent")
+    dev = Dev()
+    s = state(dev, prec, B, n=3)
+    mh, mu = dev(es * B * nC), dev(es * B * nE)
+    ok(lib.msw_set_tend_output(ctx, mh, mu, None, None), "tend output, members")
+    ok(lib.msw_fwd_steps(ctx, s["ht"], s["ut"], 2), "msw_fwd_steps kept tendencies")
+    bh, bu, th, tu = dev(8 * nC), dev(8 * nE), dev(es * B * nC), dev(es * B * nE)
+    ok(lib.msw_set_tend_output(ctx, bh, bu, th, tu), "tend output, base and TL")
+    for cfg in (config(), config(visc2=0.1, visc4=0.1, wind_stress=1, bottom_drag=1), config(test_case=1)):
+        ok(lib.msw_set_config(ctx, C.byref(cfg)), "set_config")
+        ok(lib.msw_base_steps(ctx, s["h"], s["u"], 2, None, None), "msw_base_steps kept tendencies")
+        tlm_adj(ctx, s)
+        tlm_adj(ctx, s)
+        ok(lib.msw_adj_step(ctx, s["h"], s["u"], s["ht"], s["ut"], None, None), "msw_adj_step kept tendencies")
+    ok(lib.msw_set_tend_output(ctx, None, None, None, None), "tend output off")
+    tlm_adj(ctx, s)
+    ok(lib.msw_destroy(ctx), "destroy")
+    dev.free()
+    no_violations(f"tendency outputs prec={prec} B={B} graphs={graphs}")
+
+
 def persistent_path():
     """The persistent launch of a batch-1 context split into instances: its
     buffers, the error-flag read-back, and what it refuses."""
@@ -319,6 +349,10 @@ for prec, B in ((FP64, 1), (FP64, 64), (FP32, 64)):
     for graphs in (False, True):
         other_entry_points(prec, B, graphs)
         print(f"ok entry points prec={prec} B={B} graphs={graphs}", flush=True)
+for prec, B in ((FP64, 1), (FP32, 1), (FP64, 64), (FP32, 64)):
+    for graphs in (False, True):
+        tend_outputs(prec, B, graphs)
+        print(f"ok tendency outputs prec={prec} B={B} graphs={graphs}", flush=True)
 persistent_path()
 print("ok persistent path", flush=True)
 errors_reach_the_caller()

@@ -743,7 +743,7 @@ void fwd_diag(const DMesh& m, const DConfig& c, int lb, const T* h, const T* u, 
 template <typename T>
 void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k, const T* h,
                         const T* u, const T* he, const T* pe, const T* ke, const Dissip<T>& dp, T* xh, T* xu,
-                        T* acch, T* accu, T* ph, T* pu, const double* rec, hipStream_t s)
+                        T* acch, T* accu, T* ph, T* pu, const double* rec, T* toh, T* tou, hipStream_t s)
 {
     check_shape(m, "fwd_tend");
     const size_t C = m.nC, E = m.nE;
@@ -756,6 +756,9 @@ void fwd_tend_update_he(const DMesh& m, const DConfig& c, int lb, int stage, con
                        MR(ke, C, lb, "ke")});
     rk_bufs(b, stage, xu, accu, pu, E, lb, c.frozen_u != 0, "u");
     rk_bufs(b, stage, xh, acch, ph, C, lb, false, "h");
+    // the last stage's tendencies, when kept (one value per lane, any alignment)
+    if (stage == 3 && toh) b.push_back(Buf{toh, (C << lb) * sizeof(T), true, "tend_h", false, 0u});
+    if (stage == 3 && tou) b.push_back(Buf{tou, (E << lb) * sizeof(T), true, "tend_u", false, 0u});
     launch(s, "k_fwd_tend", b);
     (void)k;
 }
@@ -825,7 +828,7 @@ template <typename T>
 void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k, const BaseStage& bs,
               const T* h_tl, const T* u_tl, T* pvv_tl, T* ke_tl, T* pc_tl, T* pe_tl, T* he_tl, T* vort_tl,
               const DissipScratch<T>& ds, const AdjCoef& q, T* xh, T* xu, T* acch, T* accu, T* ph, T* pu,
-              hipStream_t s)
+              T* toh, T* tou, hipStream_t s)
 {
     check_shape(m, "tl_stage");
     const size_t C = m.nC, E = m.nE, V = m.nV;
@@ -858,6 +861,8 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
                          MR(he_tl, E, lb, "h_edge_tl"), MR(ke_tl, C, lb, "ke_tl")});
     rk_bufs(b4, stage, xu, accu, pu, E, lb, c.frozen_u != 0, "u_tl");
     rk_bufs(b4, stage, xh, acch, ph, C, lb, false, "h_tl");
+    if (stage == 3 && toh) b4.push_back(Buf{toh, (C << lb) * sizeof(T), true, "tend_h_tl", false, 0u});
+    if (stage == 3 && tou) b4.push_back(Buf{tou, (E << lb) * sizeof(T), true, "tend_u_tl", false, 0u});
     launch(s, "k_tl_tend", b4);
     (void)k;
 }
@@ -1094,12 +1099,12 @@ template int persistent_steps<float>(const PersistArgs&, bool, hipStream_t, int*
                               T*, T*, T*, double*, hipStream_t);                                                 \
     template void fwd_tend_update_he<T>(const DMesh&, const DConfig&, int, int, const RKCoeffs&, const T*,       \
                                         const T*, const T*, const T*, const T*, const Dissip<T>&, T*, T*, T*, T*, \
-                                        T*, T*, const double*, hipStream_t);                                     \
+                                        T*, T*, const double*, T*, T*, hipStream_t);                             \
     template Dissip<T> dissip_fields<T>(const DMesh&, const DConfig&, int, const T*, const T*,                    \
                                         const DissipScratch<T>&, hipStream_t);                                   \
     template void tl_stage<T>(const DMesh&, const DConfig&, int, int, const RKCoeffs&, const BaseStage&,         \
                               const T*, const T*, T*, T*, T*, T*, T*, T*, const DissipScratch<T>&,               \
-                              const AdjCoef&, T*, T*, T*, T*, T*, T*, hipStream_t);                              \
+                              const AdjCoef&, T*, T*, T*, T*, T*, T*, T*, T*, hipStream_t);                      \
     template void adj_stage<T>(const DMesh&, const DConfig&, int, int, bool, const RKCoeffs&, const BaseStage&,  \
                                T*, T*, T*, T*, T*, T*, T*, T*, T*, T*, const DissipScratch<T>&, const AdjCoef&,  \
                                T*, T*, T*, T*, hipStream_t);                                                     \

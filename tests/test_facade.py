@@ -114,6 +114,66 @@ def test_facade_steps_on_gpu():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("forcing", [False, True])
+def test_facade_tendency_arrays_match_oracle(forcing):
+    """mpsw.tend_h/tend_u (and _tl) after sw_rk4, sw_rk4_tlm and sw_rk4_adj
+    hold the last RK stage's tendencies, as the reference's module arrays do
+    (src/mpas_sw_module.f90:577-586, :700-712, :838-848; the tendency
+    adjoint leaves them alone), and tend_*_ad are zero: equal to the oracle's
+    module arrays bit for bit, level 3 TC5, also with del2, del4, bottom drag
+    and wind stress on."""
+    print(f"facade tendency arrays L3 tc5 forcing={forcing}", flush=True)
+    mesh, phys = case(3, 5)
+    mpsw = fresh_module()
+    load_mesh(mpsw, mesh)
+    mpsw.config_test_case = 5
+    mpsw.config_dt = 900.0
+    kw = {}
+    if forcing:
+        usrc = 0.1 * np.cos(mesh.latEdge) * np.cos(mesh.angleEdge) + 0.02 * np.sin(3.0 * mesh.lonEdge)
+        kw = dict(visc2=2.0e5, visc4=1.0e15, wind_stress=True, bottom_drag=True, u_src=usrc)
+        mpsw.config_h_mom_eddy_visc2 = kw["visc2"]
+        mpsw.config_h_mom_eddy_visc4 = kw["visc4"]
+        mpsw.config_wind_stress = True
+        mpsw.config_bottom_drag = True
+    mpsw.sw_test_case_5()
+    mpsw.var_allocation_tlm()
+    mpsw.var_allocation_adj()
+    if forcing:
+        mpsw.u_src[0] = usrc
+    mpsw.sw_mpas_init_block()
+    o = make_oracle(3, 5, 900.0, **kw)
+    assert not np.any(mpsw.tend_u)                    # zero after var_allocation
+    for _ in range(2):
+        mpsw.sw_rk4()
+    o.rk4(2)
+    np.testing.assert_array_equal(mpsw.u[0, 0], o["u1"])
+    assert np.any(o["tend_u"] != 0)
+    np.testing.assert_array_equal(mpsw.tend_h[0], o["tend_h"])
+    np.testing.assert_array_equal(mpsw.tend_u[0], o["tend_u"])
+    dh, du = perturbations(mesh, 1)
+    mpsw.h_tl[0, 0] = dh[:, 0]
+    mpsw.u_tl[0, 0] = du[:, 0]
+    o["h_tl1"][:] = dh[:, 0]
+    o["u_tl1"][:] = du[:, 0]
+    mpsw.sw_rk4_tlm()
+    o.rk4_tlm()
+    np.testing.assert_array_equal(mpsw.u_tl[0, 0], o["u_tl1"])
+    for n in ("tend_h", "tend_u", "tend_h_tl", "tend_u_tl"):
+        np.testing.assert_array_equal(getattr(mpsw, n)[0], o[n], err_msg=n)
+    o["h_ad1"][:] = dh[:, 0]
+    o["u_ad1"][:] = du[:, 0]
+    mpsw.h_ad[0, 0] = dh[:, 0]
+    mpsw.u_ad[0, 0] = du[:, 0]
+    mpsw.sw_rk4_adj()
+    o.rk4_adj()
+    for n in ("tend_h", "tend_u"):
+        np.testing.assert_array_equal(getattr(mpsw, n)[0], o[n], err_msg=n + " after sw_rk4_adj")
+    for n in ("tend_h_ad", "tend_u_ad"):
+        assert not np.any(getattr(mpsw, n)) and not np.any(o[n]), n
+
+
+@pytest.mark.gpu
 def test_facade_tl_reconstruction_against_reference_fortran():
     """mpsw.ureconstruct{zonal,meridional}_tl after sw_rk4_tlm, which the
     reference's TLM demo writes at every output time (demo/run-tlm.py:38-39),

@@ -908,3 +908,65 @@ def test_persistent_barrier_gives_up_without_hanging():
     m.fwd(h2, u2, 4)
     torch.cuda.synchronize()
     assert torch.equal(h, h2) and torch.equal(u, u2)
+
+
+@pytest.mark.parametrize("batch", [4, 64])
+def test_model_tendency_outputs_per_member(batch):
+    """SWModel.set_tend_output with several members: msw_fwd_steps leaves each
+    member's last-stage tendencies, msw_tlm_steps the FP64 base's and each
+    member's TL tendencies, equal to the oracle run of that member alone bit
+    for bit (level 3, TC6, the pair form at 64 members); msw_adj_steps over a
+    3-step window leaves the base tendencies of its earliest step, and
+    switching the outputs off leaves the arrays as they were."""
+    print(f"tendency outputs B={batch} L3 tc6", flush=True)
+    dt = 600.0
+    mesh, phys = case(3, 6)
+    m = model(3, 6, dt, batch=batch)
+    dh, du = perturbations(mesh, batch)
+    dev = m.device
+    mh = torch.zeros(m.nC, batch, dtype=torch.float64, device=dev)
+    mu = torch.zeros(m.nE, batch, dtype=torch.float64, device=dev)
+    # forward: member j starts from the TC6 state plus its perturbation
+    h = m.to_internal(phys["h"][:, None] + dh, "c")
+    u = m.to_internal(phys["u"][:, None] + du, "e")
+    m.set_tend_output(mh, mu)
+    m.fwd(h, u, 2)
+    TH, TU = m.to_reference(mh, "c"), m.to_reference(mu, "e")
+    for j in sorted({0, 1, batch - 1}):
+        o = make_oracle(3, 6, dt)
+        o["h1"][:] = phys["h"] + dh[:, j]
+        o["u1"][:] = phys["u"] + du[:, j]
+        o.rk4(2)
+        np.testing.assert_array_equal(TH[:, j], o["tend_h"], err_msg=f"member {j}")
+        np.testing.assert_array_equal(TU[:, j], o["tend_u"], err_msg=f"member {j}")
+    # TLM: FP64 base tendencies and each member's TL tendencies
+    bh = torch.zeros(m.nC, dtype=torch.float64, device=dev)
+    bu = torch.zeros(m.nE, dtype=torch.float64, device=dev)
+    m.set_tend_output(bh, bu, mh, mu)
+    hb = m.to_internal(phys["h"], "c", member=False)
+    ub = m.to_internal(phys["u"], "e", member=False)
+    ht = m.to_internal(dh, "c")
+    ut = m.to_internal(du, "e")
+    traj_h = torch.empty(3, m.nC, dtype=torch.float64, device=dev)
+    traj_u = torch.empty(3, m.nE, dtype=torch.float64, device=dev)
+    m.tlm(hb, ub, ht, ut, 3, traj_h=traj_h, traj_u=traj_u)
+    TH, TU = m.to_reference(mh, "c"), m.to_reference(mu, "e")
+    for j in sorted({0, batch - 1}):
+        o = make_oracle(3, 6, dt)
+        o["h_tl1"][:] = dh[:, j]
+        o["u_tl1"][:] = du[:, j]
+        o.rk4_tlm(3)
+        np.testing.assert_array_equal(TH[:, j], o["tend_h_tl"], err_msg=f"member {j}")
+        np.testing.assert_array_equal(TU[:, j], o["tend_u_tl"], err_msg=f"member {j}")
+    np.testing.assert_array_equal(m.to_reference(bh, "c"), o["tend_h"])
+    np.testing.assert_array_equal(m.to_reference(bu, "e"), o["tend_u"])
+    # adjoint sweep over the window: its last (earliest) step's base tendencies
+    m.adj(traj_h, traj_u, ht, ut, 3)
+    o = make_oracle(3, 6, dt)
+    o.rk4(1)
+    np.testing.assert_array_equal(m.to_reference(bh, "c"), o["tend_h"])
+    np.testing.assert_array_equal(m.to_reference(bu, "e"), o["tend_u"])
+    m.set_tend_output()
+    keep = bh.clone()
+    m.tlm(hb, ub, ht, ut, 1)
+    assert torch.equal(bh, keep)
