@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import case, make_oracle
+from helpers import case, make_oracle, rel_l2
 
 NML = """&sw_model
     config_test_case = {tc}
@@ -166,7 +166,10 @@ def test_run_sw_tlm_writes_the_demo_output(tmp_path):
 def test_run_sw_adj_writes_the_demo_output(tmp_path):
     """run_sw_adj with the adjoint demo's output (demo/run-adj.py:41-76): f4
     records of the reconstructed-velocity adjoints, h_ad and u_ad at each
-    output hour, last record (the start) equal to the returned adjoint."""
+    output hour, taken before that hour's adjoint step; the same flow run
+    with the oracle (forward storing each step's start state, then the
+    adjoint sweep over them, seeded at cell and edge 3) gives the same
+    records to f4 rounding and the same start adjoint to 1e-11."""
     print("driver run_sw_adj L3 tc5 output", flush=True)
     import importlib
     import module_sw_mpas._facade as fac
@@ -186,3 +189,28 @@ def test_run_sw_adj_writes_the_demo_output(tmp_path):
     assert rec["h_ad"][0][3] == 1.0 and np.count_nonzero(rec["h_ad"][0]) == 1
     assert not np.any(rec["ux_ad"]) and not np.any(rec["uy_ad"])
     assert np.all(np.isfinite(rec["u_ad"]))
+    # the oracle through the driver's loops: 13 forward steps (the driver's
+    # N+1), then the adjoint from the last stored state back to the first,
+    # a record at every 4th step (3600 s / 900 s)
+    o = make_oracle(3, 5, 900.0)
+    th, tu = [], []
+    for _ in range(13):
+        th.append(o["h1"].copy())
+        tu.append(o["u1"].copy())
+        o.rk4()
+    o["h_ad1"][:] = 0.0
+    o["u_ad1"][:] = 0.0
+    o["h_ad1"][3] = 1.0
+    o["u_ad1"][3] = 1.0
+    k = 0
+    for it in range(12, -1, -1):
+        o["h1"][:] = th[it]
+        o["u1"][:] = tu[it]
+        if it % 4 == 0:
+            for name, key in (("h_ad", "h_ad1"), ("u_ad", "u_ad1")):
+                want = o[key].astype(np.float32).astype(np.float64)
+                assert rel_l2(rec[name][k], want) < 1e-6, (name, k)
+            k += 1
+        o.rk4_adj()
+    assert k == 4
+    assert rel_l2(h_ad, o["h_ad1"]) < 1e-11 and rel_l2(u_ad, o["u_ad1"]) < 1e-11
