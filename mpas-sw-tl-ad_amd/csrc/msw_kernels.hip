@@ -1548,7 +1548,7 @@ constexpr int kPatch = 256;   // entities per patch = threads per block
 
 template <typename T>
 __device__ __forceinline__ void pv_vertex_item(const DMesh& m, const T* h, const T* u, T* pvv, T* hv, T* vort,
-                                               int iv)
+                                               int iv, bool live)
 {
     int ie[3], ic[3];
     T ue[3], hc[3];
@@ -1565,13 +1565,15 @@ __device__ __forceinline__ void pv_vertex_item(const DMesh& m, const T* h, const
 #pragma unroll
     for (int i = 0; i < 3; ++i) hh = hh + hc[i] * T(m.kite[3 * iv + i]);
     hh = hh / at;
+    if (!live) return;
     stn(pvv + iv, (T(m.fV[iv]) + vo) / hh);
     if (hv) stn(hv + iv, hh);
     if (vort) stn(vort + iv, vo);
 }
 
 template <typename T>
-__device__ __forceinline__ void pv_cell_item(const DMesh& m, const T* u, const T* pvv, T* ke, T* pc, int ic)
+__device__ __forceinline__ void pv_cell_item(const DMesh& m, const T* u, const T* pvv, T* ke, T* pc, int ic,
+                                             bool live)
 {
     int ie[kC6], iv[kC6];
     T ue[kC6], pv[kC6];
@@ -1587,6 +1589,7 @@ __device__ __forceinline__ void pv_cell_item(const DMesh& m, const T* u, const T
     T p = T(0);
 #pragma unroll
     for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
+    if (!live) return;
     stn(ke + ic, k);
     stn(pc + ic, p);
 }
@@ -1594,7 +1597,7 @@ __device__ __forceinline__ void pv_cell_item(const DMesh& m, const T* u, const T
 template <typename T>
 __device__ __forceinline__ void pv_edge_item(const DMesh& m, const DConfig& c, const T* h, const T* u,
                                              const T* pvv, const T* pc, T* he, T* pe, T* vout, T* gpt_out,
-                                             T* gpn_out, double* rec, int e)
+                                             T* gpn_out, double* rec, int e, bool live)
 {
     const T apvm = T(c.apvm), dt = T(c.dt);
     const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
@@ -1615,6 +1618,7 @@ __device__ __forceinline__ void pv_edge_item(const DMesh& m, const DConfig& c, c
     T p = T(0.5) * p1 + T(0.5) * p2;
     p = p - apvm * vv * dt * gt;
     p = p - apvm * ue * dt * gn;
+    if (!live) return;
     stn(he + e, T(0.5) * (h1 + h2));
     stn(pe + e, p);
     if (vout) stn(vout + e, vv);
@@ -1632,7 +1636,7 @@ __device__ __forceinline__ void pv_edge_item(const DMesh& m, const DConfig& c, c
 template <typename T>
 __device__ __forceinline__ void tend_edge_item(const DMesh& m, const DConfig& c, int stage, T w, T sw,
                                                const T* h, const T* u, const T* he, const T* pe, const T* ke,
-                                               const double* rec, T* xu, T* accu, T* pu, int e)
+                                               const double* rec, T* xu, T* accu, T* pu, int e, bool live)
 {
     const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
     int ix[kEoE];
@@ -1660,13 +1664,13 @@ __device__ __forceinline__ void tend_edge_item(const DMesh& m, const DConfig& c,
         q = q + T(m.woe[kE2 * e + j]) * ux[j] * workpv * hx[j];
     }
     const T tend = q - (k2 - k1 + T(kGravity) * (h2 + T(m.hs[c2]) - h1 - T(m.hs[c1]))) / T(m.dc[e]);
-    rk_update(stage, w, sw, tend, xu, accu, pu, e, 1, 0, c.frozen_u != 0);
+    if (live) rk_update(stage, w, sw, tend, xu, accu, pu, e, 1, 0, c.frozen_u != 0);
 }
 
 // tend_h and its RK update (k_fwd_tend's cell branch)
 template <typename T>
 __device__ __forceinline__ void tend_cell_item(const DMesh& m, int stage, T w, T sw, const T* u, const T* he,
-                                               T* xh, T* acch, T* ph, int ic)
+                                               T* xh, T* acch, T* ph, int ic, bool live)
 {
     int ie[kC6];
     T ue[kC6], hh[kC6];
@@ -1681,12 +1685,12 @@ __device__ __forceinline__ void tend_cell_item(const DMesh& m, int stage, T w, T
         th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
     }
     th = th / T(m.areaC[ic]);
-    rk_update(stage, w, sw, th, xh, acch, ph, ic, 1, 0, false);
+    if (live) rk_update(stage, w, sw, th, xh, acch, ph, ic, 1, 0, false);
 }
 
 template <typename T>
 __device__ __forceinline__ void tl_vertex_item(const DMesh& m, const BaseStage& bs, const T* h_tl, const T* u_tl,
-                                               T* pvv_tl, int iv)
+                                               T* pvv_tl, int iv, bool live)
 {
     int ie[3], ic[3];
     T ue[3], hc[3];
@@ -1704,14 +1708,14 @@ __device__ __forceinline__ void tl_vertex_item(const DMesh& m, const BaseStage& 
     for (int i = 0; i < 3; ++i) ht = ht + hc[i] * T(m.kite[3 * iv + i]);
     ht = ht / at;
     const T hv = T(bs.hv[iv]);
-    stn(pvv_tl + iv, vt / hv - ht * (T(m.fV[iv]) + T(bs.vort[iv])) / (hv * hv));
+    if (live) stn(pvv_tl + iv, vt / hv - ht * (T(m.fV[iv]) + T(bs.vort[iv])) / (hv * hv));
 }
 
 // k_tl_cell with its two coefficients (2*u and 0.25*dcEdge*dvEdge per edge
 // slot, as k_tl_coef forms them) computed here
 template <typename T>
 __device__ __forceinline__ void tl_cell_item(const DMesh& m, const BaseStage& bs, const T* u_tl, const T* pvv_tl,
-                                             T* ke_tl, T* pc_tl, int ic)
+                                             T* ke_tl, T* pc_tl, int ic, bool live)
 {
     int ie[kC6], iv[kC6];
     T ue[kC6], pv[kC6];
@@ -1733,6 +1737,7 @@ __device__ __forceinline__ void tl_cell_item(const DMesh& m, const BaseStage& bs
     T p = T(0);
 #pragma unroll
     for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
+    if (!live) return;
     stn(ke_tl + ic, k);
     stn(pc_tl + ic, p);
 }
@@ -1740,7 +1745,7 @@ __device__ __forceinline__ void tl_cell_item(const DMesh& m, const BaseStage& bs
 template <typename T>
 __device__ __forceinline__ void tl_edge_item(const DMesh& m, const DConfig& c, const BaseStage& bs, const T* h_tl,
                                              const T* u_tl, const T* pvv_tl, const T* pc_tl, T* pe_tl, T* he_tl,
-                                             int e)
+                                             int e, bool live)
 {
     const T apvm = T(c.apvm), dt = T(c.dt);
     const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
@@ -1761,8 +1766,9 @@ __device__ __forceinline__ void tl_edge_item(const DMesh& m, const DConfig& c, c
     T p = T(0.5) * p1 + T(0.5) * p2;
     p = p - apvm * vt * dt * T(bs.gpt[e]) - apvm * T(bs.v[e]) * dt * gtt;
     p = p - apvm * ue * dt * T(bs.gpn[e]) - apvm * T(bs.u[e]) * dt * gnt;
-    stn(pe_tl + e, p);
     const T het = T(0.5) * (h1 + h2);
+    if (!live) return;
+    stn(pe_tl + e, p);
     stn(he_tl + e, c.tl_fused ? ue * T(bs.he[e]) + T(bs.u[e]) * het : het);
 }
 
@@ -1771,7 +1777,8 @@ __device__ __forceinline__ void tl_edge_item(const DMesh& m, const DConfig& c, c
 template <typename T, bool FUSED>
 __device__ __forceinline__ void tl_tend_edge_item(const DMesh& m, const DConfig& c, int stage, T w, T sw,
                                                   const BaseStage& bs, const T* h_tl, const T* u_tl, const T* pe_tl,
-                                                  const T* he_tl, const T* ke_tl, T* xu, T* accu, T* pu, int e)
+                                                  const T* he_tl, const T* ke_tl, T* xu, T* accu, T* pu, int e,
+                                                  bool live)
 {
     const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
     int ix[kEoE];
@@ -1796,12 +1803,13 @@ __device__ __forceinline__ void tl_tend_edge_item(const DMesh& m, const DConfig&
         }
     }
     const T tend = qt - (k2 - k1 + T(kGravity) * (h2 - h1)) / T(m.dc[e]);
-    rk_update(stage, w, sw, tend, xu, accu, pu, e, 1, 0, c.frozen_u != 0);
+    if (live) rk_update(stage, w, sw, tend, xu, accu, pu, e, 1, 0, c.frozen_u != 0);
 }
 
 template <typename T, bool FUSED>
 __device__ __forceinline__ void tl_tend_cell_item(const DMesh& m, int stage, T w, T sw, const BaseStage& bs,
-                                                  const T* u_tl, const T* he_tl, T* xh, T* acch, T* ph, int ic)
+                                                  const T* u_tl, const T* he_tl, T* xh, T* acch, T* ph, int ic,
+                                                  bool live)
 {
     T th = T(0);
 #pragma unroll
@@ -1815,7 +1823,147 @@ __device__ __forceinline__ void tl_tend_cell_item(const DMesh& m, int stage, T w
         }
     }
     th = th / T(m.areaC[ic]);
-    rk_update(stage, w, sw, th, xh, acch, ph, ic, 1, 0, false);
+    if (live) rk_update(stage, w, sw, th, xh, acch, ph, ic, 1, 0, false);
+}
+
+// Chunks of items.  No entity of a pass reads what the pass writes, so the
+// arrays a chunk function reads and writes are declared __restrict__: the
+// compiler may then issue the gathers of all of a thread's kChunk entities
+// before the first store, instead of walking them one after another with a
+// full memory latency each.  An entity with live false is entity 0, computed
+// and not stored.
+constexpr int kChunk = 4;
+using ChunkI = int[kChunk];
+using ChunkB = bool[kChunk];
+
+template <typename T>
+__device__ __forceinline__ void pv_vertex_chunk(const DMesh& m, const T* __restrict__ h, const T* __restrict__ u,
+                                                T* __restrict__ pvv, T* __restrict__ hv, T* __restrict__ vort,
+                                                const ChunkI& ent, const ChunkB& lv)
+{
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) pv_vertex_item<T>(m, h, u, pvv, hv, vort, ent[j], lv[j]);
+}
+
+template <typename T>
+__device__ __forceinline__ void pv_cell_chunk(const DMesh& m, const T* __restrict__ u, const T* __restrict__ pvv,
+                                              T* __restrict__ ke, T* __restrict__ pc, const ChunkI& ent,
+                                              const ChunkB& lv)
+{
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) pv_cell_item<T>(m, u, pvv, ke, pc, ent[j], lv[j]);
+}
+
+template <typename T>
+__device__ __forceinline__ void pv_edge_chunk(const DMesh& m, const DConfig& c, const T* __restrict__ h,
+                                              const T* __restrict__ u, const T* __restrict__ pvv,
+                                              const T* __restrict__ pc, T* __restrict__ he, T* __restrict__ pe,
+                                              T* __restrict__ vout, T* __restrict__ gpt, T* __restrict__ gpn,
+                                              double* __restrict__ rec, const ChunkI& ent, const ChunkB& lv)
+{
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) pv_edge_item<T>(m, c, h, u, pvv, pc, he, pe, vout, gpt, gpn, rec, ent[j], lv[j]);
+}
+
+template <typename T>
+__device__ __forceinline__ void tend_edge_chunk(const DMesh& m, const DConfig& c, int stage, T w, T sw,
+                                                const T* __restrict__ h, const T* __restrict__ u,
+                                                const T* __restrict__ he, const T* __restrict__ pe,
+                                                const T* __restrict__ ke, const double* __restrict__ rec,
+                                                T* __restrict__ xu, T* __restrict__ accu, T* __restrict__ pu,
+                                                const ChunkI& ent, const ChunkB& lv)
+{
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) tend_edge_item<T>(m, c, stage, w, sw, h, u, he, pe, ke, rec, xu, accu, pu, ent[j], lv[j]);
+}
+
+template <typename T>
+__device__ __forceinline__ void tend_cell_chunk(const DMesh& m, int stage, T w, T sw, const T* __restrict__ u,
+                                                const T* __restrict__ he, T* __restrict__ xh, T* __restrict__ acch,
+                                                T* __restrict__ ph, const ChunkI& ent, const ChunkB& lv)
+{
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) tend_cell_item<T>(m, stage, w, sw, u, he, xh, acch, ph, ent[j], lv[j]);
+}
+
+template <typename T>
+__device__ __forceinline__ void tl_vertex_chunk(const DMesh& m, const BaseStage& bs, const T* __restrict__ h_tl,
+                                                const T* __restrict__ u_tl, T* __restrict__ pvv_tl,
+                                                const ChunkI& ent, const ChunkB& lv)
+{
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) tl_vertex_item<T>(m, bs, h_tl, u_tl, pvv_tl, ent[j], lv[j]);
+}
+
+template <typename T>
+__device__ __forceinline__ void tl_cell_chunk(const DMesh& m, const BaseStage& bs, const T* __restrict__ u_tl,
+                                              const T* __restrict__ pvv_tl, T* __restrict__ ke_tl,
+                                              T* __restrict__ pc_tl, const ChunkI& ent, const ChunkB& lv)
+{
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) tl_cell_item<T>(m, bs, u_tl, pvv_tl, ke_tl, pc_tl, ent[j], lv[j]);
+}
+
+template <typename T, bool FUSED>
+__device__ __forceinline__ void tl_tend_edge_chunk(const DMesh& m, const DConfig& c, int stage, T w, T sw,
+                                                   const BaseStage& bs, const T* __restrict__ h_tl,
+                                                   const T* __restrict__ u_tl, const T* __restrict__ pe_tl,
+                                                   const T* __restrict__ he_tl, const T* __restrict__ ke_tl,
+                                                   T* __restrict__ xu, T* __restrict__ accu, T* __restrict__ pu,
+                                                   const ChunkI& ent, const ChunkB& lv)
+{
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+        tl_tend_edge_item<T, FUSED>(m, c, stage, w, sw, bs, h_tl, u_tl, pe_tl, he_tl, ke_tl, xu, accu, pu, ent[j], lv[j]);
+}
+
+template <typename T, bool FUSED>
+__device__ __forceinline__ void tl_tend_cell_chunk(const DMesh& m, int stage, T w, T sw, const BaseStage& bs,
+                                                   const T* __restrict__ u_tl, const T* __restrict__ he_tl,
+                                                   T* __restrict__ xh, T* __restrict__ acch, T* __restrict__ ph,
+                                                   const ChunkI& ent, const ChunkB& lv)
+{
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+        tl_tend_cell_item<T, FUSED>(m, stage, w, sw, bs, u_tl, he_tl, xh, acch, ph, ent[j], lv[j]);
+}
+
+template <typename T>
+__device__ __forceinline__ void tl_edge_chunk(const DMesh& m, const DConfig& c, const BaseStage& bs,
+                                              const T* __restrict__ h_tl, const T* __restrict__ u_tl,
+                                              const T* __restrict__ pvv_tl, const T* __restrict__ pc_tl,
+                                              T* __restrict__ pe_tl, T* __restrict__ he_tl, const ChunkI& ent,
+                                              const ChunkB& lv)
+{
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) tl_edge_item<T>(m, c, bs, h_tl, u_tl, pvv_tl, pc_tl, pe_tl, he_tl, ent[j], lv[j]);
+}
+
+// A chunk whose entities are all of kind ka goes to fa(ent, live), all of
+// kind kb to fb(ent, live) (the usual case: a queue lists each kind's
+// patches in one run); a chunk that straddles two kinds goes one entity at
+// a time to one(kind, entity).  kb = -2: a pass of one kind.
+template <typename FA, typename FB, typename G>
+__device__ __forceinline__ void chunk_of(const ChunkI& kk, const ChunkI& ent, int ka, FA&& fa, int kb, FB&& fb,
+                                         G&& one)
+{
+    bool la[kChunk], lb[kChunk], onlya = true, onlyb = true;
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+        la[j] = kk[j] == ka;
+        lb[j] = kk[j] == kb;
+        onlya = onlya && (la[j] || kk[j] < 0);
+        onlyb = onlyb && (lb[j] || kk[j] < 0);
+    }
+    if (onlya) {
+        fa(ent, la);
+    } else if (onlyb) {
+        fb(ent, lb);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j)
+            if (kk[j] >= 0) one(kk[j], ent[j]);
+    }
 }
 
 // Grid barrier of the persistent launch: every block adds one to `bar` and
@@ -1848,7 +1996,9 @@ __device__ bool grid_barrier(unsigned long long* bar, int* err, unsigned long lo
 }
 
 // One pass: its work items are the patches of up to three entity kinds
-// (0 vertex, 1 cell, 2 edge); `item(kind, entity)` processes one entity.
+// (0 vertex, 1 cell, 2 edge); `chunk(kind, entity)` processes this thread's
+// entities of one chunk of patches: kind[j] is the kind of the j-th (-1: no
+// entity, and then entity[j] is 0, a valid index of every kind).
 // The items are split into 8 queues, queue x holding the x-th contiguous
 // eighth of every kind's patches, so that the blocks of XCD x (hardware
 // block i runs on XCD i mod 8) work on one region of the space-filling curve
@@ -1857,12 +2007,11 @@ __device__ bool grid_barrier(unsigned long long* bar, int* err, unsigned long lo
 // until all are empty.  Passes alternate between two sets of queue counters;
 // block 0 clears the set of the previous pass, which the next barrier
 // publishes before that set is used again.
-constexpr int kChunk = 4;
 constexpr int kQueues = 8;
 
 template <typename F>
 __device__ __forceinline__ bool persistent_pass(const PersistArgs& a, const int (&kinds)[3], int nk,
-                                                unsigned long long& phase, F&& item)
+                                                unsigned long long& phase, F&& chunk)
 {
     __shared__ unsigned long long s_q;
     const int par = static_cast<int>(phase & 1ull);
@@ -1897,6 +2046,7 @@ __device__ __forceinline__ bool persistent_pass(const PersistArgs& a, const int 
             for (int j = 0; j < kChunk; ++j) {
                 const int qq = static_cast<int>(q) + j;
                 kk[j] = -1;
+                ent[j] = 0;
                 if (qq < total) {
                     int k = 0, r = qq;
                     while (r >= len[k]) { r -= len[k]; ++k; }
@@ -1911,9 +2061,7 @@ __device__ __forceinline__ bool persistent_pass(const PersistArgs& a, const int 
             }
             // entities of one pass never read what the pass writes, so the
             // items of a chunk are independent
-#pragma unroll
-            for (int j = 0; j < kChunk; ++j)
-                if (kk[j] >= 0) item(kk[j], ent[j]);
+            chunk(kk, ent);
         }
     }
     ++phase;
@@ -1944,19 +2092,37 @@ __global__ __launch_bounds__(kPatch) void k_persist_fwd(PersistArgs a)
         for (int s = 0; s < 4; ++s) {
             T* nb = static_cast<T*>((s % 2 == 0) ? a.pA : a.pB);
             const T w = T(a.k.w[s]), sw = T(a.k.sw[s]);
-            if (!persistent_pass(a, kV, 1, phase, [&](int, int i) {
-                    pv_vertex_item<T>(m, rh, ru, pvv, nullptr, nullptr, i); }))
+            if (!persistent_pass(a, kV, 1, phase, [&](const ChunkI& kk, const ChunkI& en) {
+                    chunk_of(kk, en, 0, [&](const ChunkI& e, const ChunkB& lv) {
+                        pv_vertex_chunk<T>(m, rh, ru, pvv, nullptr, nullptr, e, lv); }, -2, [&](const ChunkI&, const ChunkB&) {},
+                        [&](int, int i) { pv_vertex_item<T>(m, rh, ru, pvv, nullptr, nullptr, i, true); });
+                }))
                 return;
-            if (!persistent_pass(a, kC, 1, phase, [&](int, int i) { pv_cell_item<T>(m, ru, pvv, ke, pc, i); }))
+            if (!persistent_pass(a, kC, 1, phase, [&](const ChunkI& kk, const ChunkI& en) {
+                    chunk_of(kk, en, 1, [&](const ChunkI& e, const ChunkB& lv) {
+                        pv_cell_chunk<T>(m, ru, pvv, ke, pc, e, lv); }, -2, [&](const ChunkI&, const ChunkB&) {},
+                        [&](int, int i) { pv_cell_item<T>(m, ru, pvv, ke, pc, i, true); });
+                }))
                 return;
-            if (!persistent_pass(a, kE, 1, phase, [&](int, int i) {
-                    pv_edge_item<T>(m, c, rh, ru, pvv, pc, he, pe, nullptr, nullptr, nullptr, nullptr, i); }))
+            if (!persistent_pass(a, kE, 1, phase, [&](const ChunkI& kk, const ChunkI& en) {
+                    chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
+                        pv_edge_chunk<T>(m, c, rh, ru, pvv, pc, he, pe, nullptr, nullptr, nullptr, nullptr, e, lv); },
+                        -2, [&](const ChunkI&, const ChunkB&) {},
+                        [&](int, int i) {
+                            pv_edge_item<T>(m, c, rh, ru, pvv, pc, he, pe, nullptr, nullptr, nullptr, nullptr, i, true); });
+                }))
                 return;
-            if (!persistent_pass(a, kEC, 2, phase, [&](int kind, int i) {
-                    if (kind == 2)
-                        tend_edge_item<T>(m, c, s, w, sw, rh, ru, he, pe, ke, nullptr, u, acc + C, nb + C, i);
-                    else
-                        tend_cell_item<T>(m, s, w, sw, ru, he, h, acc, nb, i);
+            if (!persistent_pass(a, kEC, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
+                    chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
+                        tend_edge_chunk<T>(m, c, s, w, sw, rh, ru, he, pe, ke, nullptr, u, acc + C, nb + C, e, lv); },
+                        1, [&](const ChunkI& e, const ChunkB& lv) {
+                        tend_cell_chunk<T>(m, s, w, sw, ru, he, h, acc, nb, e, lv); },
+                        [&](int kind, int i) {
+                            if (kind == 2)
+                                tend_edge_item<T>(m, c, s, w, sw, rh, ru, he, pe, ke, nullptr, u, acc + C, nb + C, i, true);
+                            else
+                                tend_cell_item<T>(m, s, w, sw, ru, he, h, acc, nb, i, true);
+                        });
                 }))
                 return;
             rh = nb;
@@ -1996,45 +2162,76 @@ __global__ __launch_bounds__(kPatch) void k_persist_tlm(PersistArgs a)
             const T w = T(a.k.w[s]), sw = T(a.k.sw[s]);
             const bool store = s == 0 && a.traj_h != nullptr;
             const size_t tro = static_cast<size_t>(step);
-            if (!persistent_pass(a, store ? kVCE : kV, store ? 3 : 1, phase, [&](int kind, int i) {
-                    if (kind == 0)
-                        pv_vertex_item<double>(m, b.h, b.u, b.pvv, b.hv, b.vort, i);
-                    else if (kind == 1)
-                        a.traj_h[tro * C + i] = a.bh[i];
-                    else
-                        a.traj_u[tro * m.nE + i] = a.bu[i];
+            if (!persistent_pass(a, store ? kVCE : kV, store ? 3 : 1, phase, [&](const ChunkI& kk, const ChunkI& en) {
+                    chunk_of(kk, en, 0, [&](const ChunkI& e, const ChunkB& lv) {
+                        pv_vertex_chunk<double>(m, b.h, b.u, b.pvv, b.hv, b.vort, e, lv); }, -2, [&](const ChunkI&, const ChunkB&) {},
+                        [&](int kind, int i) {
+                            if (kind == 0)
+                                pv_vertex_item<double>(m, b.h, b.u, b.pvv, b.hv, b.vort, i, true);
+                            else if (kind == 1)
+                                a.traj_h[tro * C + i] = a.bh[i];
+                            else
+                                a.traj_u[tro * m.nE + i] = a.bu[i];
+                        });
                 }))
                 return;
-            if (!persistent_pass(a, kCV, 2, phase, [&](int kind, int i) {
-                    if (kind == 1)
-                        pv_cell_item<double>(m, b.u, b.pvv, b.ke, b.pc, i);
-                    else
-                        tl_vertex_item<T>(m, b, th, tu, pvv_tl, i);
+            if (!persistent_pass(a, kCV, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
+                    chunk_of(kk, en, 1, [&](const ChunkI& e, const ChunkB& lv) {
+                        pv_cell_chunk<double>(m, b.u, b.pvv, b.ke, b.pc, e, lv); },
+                        0, [&](const ChunkI& e, const ChunkB& lv) { tl_vertex_chunk<T>(m, b, th, tu, pvv_tl, e, lv); },
+                        [&](int kind, int i) {
+                            if (kind == 1)
+                                pv_cell_item<double>(m, b.u, b.pvv, b.ke, b.pc, i, true);
+                            else
+                                tl_vertex_item<T>(m, b, th, tu, pvv_tl, i, true);
+                        });
                 }))
                 return;
-            if (!persistent_pass(a, kEC, 2, phase, [&](int kind, int i) {
-                    if (kind == 2)
-                        pv_edge_item<double>(m, c, b.h, b.u, b.pvv, b.pc, b.he, b.pe, b.v, b.gpt, b.gpn, b.rec, i);
-                    else
-                        tl_cell_item<T>(m, b, tu, pvv_tl, ke_tl, pc_tl, i);
+            if (!persistent_pass(a, kEC, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
+                    chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
+                        pv_edge_chunk<double>(m, c, b.h, b.u, b.pvv, b.pc, b.he, b.pe, b.v, b.gpt, b.gpn, b.rec, e, lv); },
+                        1, [&](const ChunkI& e, const ChunkB& lv) {
+                        tl_cell_chunk<T>(m, b, tu, pvv_tl, ke_tl, pc_tl, e, lv); },
+                        [&](int kind, int i) {
+                            if (kind == 2)
+                                pv_edge_item<double>(m, c, b.h, b.u, b.pvv, b.pc, b.he, b.pe, b.v, b.gpt, b.gpn, b.rec,
+                                                     i, true);
+                            else
+                                tl_cell_item<T>(m, b, tu, pvv_tl, ke_tl, pc_tl, i, true);
+                        });
                 }))
                 return;
-            if (!persistent_pass(a, kEC, 2, phase, [&](int kind, int i) {
-                    if (kind == 2) {
-                        tend_edge_item<double>(m, c, s, a.k.w[s], a.k.sw[s], b.h, b.u, b.he, b.pe, b.ke, b.rec,
-                                               a.bu, a.bacc + C, nh + C, i);
-                        tl_edge_item<T>(m, c, b, th, tu, pvv_tl, pc_tl, pe_tl, he_tl, i);
-                    } else {
-                        tend_cell_item<double>(m, s, a.k.w[s], a.k.sw[s], b.u, b.he, a.bh, a.bacc, nh, i);
-                    }
+            if (!persistent_pass(a, kEC, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
+                    chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
+                        tend_edge_chunk<double>(m, c, s, a.k.w[s], a.k.sw[s], b.h, b.u, b.he, b.pe, b.ke, b.rec,
+                                                a.bu, a.bacc + C, nh + C, e, lv);
+                        tl_edge_chunk<T>(m, c, b, th, tu, pvv_tl, pc_tl, pe_tl, he_tl, e, lv); },
+                        1, [&](const ChunkI& e, const ChunkB& lv) {
+                        tend_cell_chunk<double>(m, s, a.k.w[s], a.k.sw[s], b.u, b.he, a.bh, a.bacc, nh, e, lv); },
+                        [&](int kind, int i) {
+                            if (kind == 2) {
+                                tend_edge_item<double>(m, c, s, a.k.w[s], a.k.sw[s], b.h, b.u, b.he, b.pe, b.ke, b.rec,
+                                                       a.bu, a.bacc + C, nh + C, i, true);
+                                tl_edge_item<T>(m, c, b, th, tu, pvv_tl, pc_tl, pe_tl, he_tl, i, true);
+                            } else {
+                                tend_cell_item<double>(m, s, a.k.w[s], a.k.sw[s], b.u, b.he, a.bh, a.bacc, nh, i, true);
+                            }
+                        });
                 }))
                 return;
-            if (!persistent_pass(a, kEC, 2, phase, [&](int kind, int i) {
-                    if (kind == 2)
-                        tl_tend_edge_item<T, FUSED>(m, c, s, w, sw, b, th, tu, pe_tl, he_tl, ke_tl, utl, acc + C,
-                                                    nb + C, i);
-                    else
-                        tl_tend_cell_item<T, FUSED>(m, s, w, sw, b, tu, he_tl, htl, acc, nb, i);
+            if (!persistent_pass(a, kEC, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
+                    chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
+                        tl_tend_edge_chunk<T, FUSED>(m, c, s, w, sw, b, th, tu, pe_tl, he_tl, ke_tl, utl, acc + C,
+                                                     nb + C, e, lv); },
+                        1, [&](const ChunkI& e, const ChunkB& lv) {
+                        tl_tend_cell_chunk<T, FUSED>(m, s, w, sw, b, tu, he_tl, htl, acc, nb, e, lv); },
+                        [&](int kind, int i) {
+                            if (kind == 2)
+                                tl_tend_edge_item<T, FUSED>(m, c, s, w, sw, b, th, tu, pe_tl, he_tl, ke_tl, utl,
+                                                            acc + C, nb + C, i, true);
+                            else
+                                tl_tend_cell_item<T, FUSED>(m, s, w, sw, b, tu, he_tl, htl, acc, nb, i, true);
+                        });
                 }))
                 return;
             th = nb;
