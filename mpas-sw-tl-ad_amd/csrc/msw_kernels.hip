@@ -1546,24 +1546,38 @@ __global__ __launch_bounds__(kMaxBlock) void k_nonfinite(long n, int lb, const T
 
 constexpr int kPatch = 256;   // entities per patch = threads per block
 
+// Each item is a gather (index and neighbour loads) and a finish (the
+// arithmetic and the stores), so that a chunk can issue the gathers of
+// several entities before finishing any (see the chunk functions below).
 template <typename T>
-__device__ __forceinline__ void pv_vertex_item(const DMesh& m, const T* h, const T* u, T* pvv, T* hv, T* vort,
-                                               int iv, bool live)
-{
-    int ie[3], ic[3];
+struct VertexG {
     T ue[3], hc[3];
+};
+
+template <typename T>
+__device__ __forceinline__ VertexG<T> pv_vertex_gather(const DMesh& m, const T* h, const T* u, int iv)
+{
+    VertexG<T> g;
+    int ie[3], ic[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) { ie[i] = m.eovR[3 * iv + i]; ic[i] = m.cov[3 * iv + i]; }
 #pragma unroll
-    for (int i = 0; i < 3; ++i) { ue[i] = u[ie[i]]; hc[i] = h[ic[i]]; }
+    for (int i = 0; i < 3; ++i) { g.ue[i] = u[ie[i]]; g.hc[i] = h[ic[i]]; }
+    return g;
+}
+
+template <typename T>
+__device__ __forceinline__ void pv_vertex_finish(const DMesh& m, const VertexG<T>& g, T* pvv, T* hv, T* vort, int iv,
+                                                 bool live)
+{
     T circ = T(0);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) circ = circ + T(m.sdcV[3 * iv + i]) * ue[i];
+    for (int i = 0; i < 3; ++i) circ = circ + T(m.sdcV[3 * iv + i]) * g.ue[i];
     const T at = T(m.areaT[iv]);
     const T vo = circ / at;
     T hh = T(0);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) hh = hh + hc[i] * T(m.kite[3 * iv + i]);
+    for (int i = 0; i < 3; ++i) hh = hh + g.hc[i] * T(m.kite[3 * iv + i]);
     hh = hh / at;
     if (!live) return;
     stn(pvv + iv, (T(m.fV[iv]) + vo) / hh);
@@ -1572,26 +1586,99 @@ __device__ __forceinline__ void pv_vertex_item(const DMesh& m, const T* h, const
 }
 
 template <typename T>
-__device__ __forceinline__ void pv_cell_item(const DMesh& m, const T* u, const T* pvv, T* ke, T* pc, int ic,
-                                             bool live)
+__device__ __forceinline__ void pv_vertex_item(const DMesh& m, const T* h, const T* u, T* pvv, T* hv, T* vort,
+                                               int iv, bool live)
 {
-    int ie[kC6], iv[kC6];
+    pv_vertex_finish(m, pv_vertex_gather(m, h, u, iv), pvv, hv, vort, iv, live);
+}
+
+template <typename T>
+struct CellG {
     T ue[kC6], pv[kC6];
+};
+
+template <typename T>
+__device__ __forceinline__ CellG<T> pv_cell_gather(const DMesh& m, const T* u, const T* pvv, int ic)
+{
+    CellG<T> g;
+    int ie[kC6], iv[kC6];
 #pragma unroll
     for (int i = 0; i < kC6; ++i) { ie[i] = m.eoc[kC6 * ic + i]; iv[i] = m.vocR[kC6 * ic + i]; }
 #pragma unroll
-    for (int i = 0; i < kC6; ++i) { ue[i] = u[ie[i]]; pv[i] = pvv[iv[i]]; }
+    for (int i = 0; i < kC6; ++i) { g.ue[i] = u[ie[i]]; g.pv[i] = pvv[iv[i]]; }
+    return g;
+}
+
+template <typename T>
+__device__ __forceinline__ void pv_cell_finish(const DMesh& m, const CellG<T>& g, T* ke, T* pc, int ic, bool live)
+{
     const T area = T(m.areaC[ic]);
     T k = T(0);
 #pragma unroll
-    for (int i = 0; i < kC6; ++i) k = k + T(m.keC[kC6 * ic + i]) * (ue[i] * ue[i]);
+    for (int i = 0; i < kC6; ++i) k = k + T(m.keC[kC6 * ic + i]) * (g.ue[i] * g.ue[i]);
     k = k / area;
     T p = T(0);
 #pragma unroll
-    for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
+    for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * g.pv[i] / area;
     if (!live) return;
     stn(ke + ic, k);
     stn(pc + ic, p);
+}
+
+template <typename T>
+__device__ __forceinline__ void pv_cell_item(const DMesh& m, const T* u, const T* pvv, T* ke, T* pc, int ic,
+                                             bool live)
+{
+    pv_cell_finish(m, pv_cell_gather(m, u, pvv, ic), ke, pc, ic, live);
+}
+
+template <typename T>
+struct EdgeG {
+    T ux[kEoE];
+    T h1, h2, p1, p2, q1, q2, ue;
+};
+
+template <typename T>
+__device__ __forceinline__ EdgeG<T> pv_edge_gather(const DMesh& m, const T* h, const T* u, const T* pvv,
+                                                   const T* pc, int e)
+{
+    EdgeG<T> g;
+    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+    const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
+    int ix[kEoE];
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) g.ux[j] = u[ix[j]];
+    g.h1 = h[c1]; g.h2 = h[c2]; g.p1 = pvv[v1]; g.p2 = pvv[v2]; g.q1 = pc[c1]; g.q2 = pc[c2];
+    g.ue = u[e];
+    return g;
+}
+
+template <typename T>
+__device__ __forceinline__ void pv_edge_finish(const DMesh& m, const DConfig& c, const EdgeG<T>& g, T* he, T* pe,
+                                               T* vout, T* gpt_out, T* gpn_out, double* rec, int e, bool live)
+{
+    const T apvm = T(c.apvm), dt = T(c.dt);
+    T vv = T(0);
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) vv = vv + T(m.woe[kE2 * e + j]) * g.ux[j];
+    const T gt = (g.p2 - g.p1) / T(m.dv[e]);
+    const T gn = (g.q2 - g.q1) / T(m.dc[e]);
+    T p = T(0.5) * g.p1 + T(0.5) * g.p2;
+    p = p - apvm * vv * dt * gt;
+    p = p - apvm * g.ue * dt * gn;
+    if (!live) return;
+    stn(he + e, T(0.5) * (g.h1 + g.h2));
+    stn(pe + e, p);
+    if (vout) stn(vout + e, vv);
+    if (gpt_out) stn(gpt_out + e, gt);
+    if (gpn_out) stn(gpn_out + e, gn);
+    if (rec) {
+        double2* r = reinterpret_cast<double2*>(rec + 4L * e);
+        r[0] = make_double2(static_cast<double>(p), static_cast<double>(g.ue));
+        r[1] = make_double2(static_cast<double>(T(0.5) * (g.h1 + g.h2)), static_cast<double>(gt));
+    }
 }
 
 template <typename T>
@@ -1599,89 +1686,97 @@ __device__ __forceinline__ void pv_edge_item(const DMesh& m, const DConfig& c, c
                                              const T* pvv, const T* pc, T* he, T* pe, T* vout, T* gpt_out,
                                              T* gpn_out, double* rec, int e, bool live)
 {
-    const T apvm = T(c.apvm), dt = T(c.dt);
-    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
-    const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
-    int ix[kEoE];
-    T ux[kEoE];
-#pragma unroll
-    for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
-#pragma unroll
-    for (int j = 0; j < kEoE; ++j) ux[j] = u[ix[j]];
-    const T h1 = h[c1], h2 = h[c2], p1 = pvv[v1], p2 = pvv[v2], q1 = pc[c1], q2 = pc[c2];
-    const T ue = u[e];
-    T vv = T(0);
-#pragma unroll
-    for (int j = 0; j < kEoE; ++j) vv = vv + T(m.woe[kE2 * e + j]) * ux[j];
-    const T gt = (p2 - p1) / T(m.dv[e]);
-    const T gn = (q2 - q1) / T(m.dc[e]);
-    T p = T(0.5) * p1 + T(0.5) * p2;
-    p = p - apvm * vv * dt * gt;
-    p = p - apvm * ue * dt * gn;
-    if (!live) return;
-    stn(he + e, T(0.5) * (h1 + h2));
-    stn(pe + e, p);
-    if (vout) stn(vout + e, vv);
-    if (gpt_out) stn(gpt_out + e, gt);
-    if (gpn_out) stn(gpn_out + e, gn);
-    if (rec) {
-        double2* r = reinterpret_cast<double2*>(rec + 4L * e);
-        r[0] = make_double2(static_cast<double>(p), static_cast<double>(ue));
-        r[1] = make_double2(static_cast<double>(T(0.5) * (h1 + h2)), static_cast<double>(gt));
-    }
+    pv_edge_finish(m, c, pv_edge_gather(m, h, u, pvv, pc, e), he, pe, vout, gpt_out, gpn_out, rec, e, live);
 }
 
-// tend_u and its RK update (k_fwd_tend's edge branch); rec: the neighbours'
+// tend_u and its RK update (k_fwd_tend's edge branch); REC: the neighbours'
 // pv_edge, u, h_edge from the packed records (the base) instead of u, pe, he
+template <typename T>
+struct TendEG {
+    T ux[kEoE], px[kEoE], hx[kEoE];
+    T pee, k1, k2, h1, h2;
+};
+
+template <typename T, bool REC>
+__device__ __forceinline__ TendEG<T> tend_edge_gather(const DMesh& m, const T* h, const T* u, const T* he,
+                                                      const T* pe, const T* ke, const double* rec, int e)
+{
+    TendEG<T> g;
+    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+    int ix[kEoE];
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
+    if constexpr (REC) {
+#pragma unroll
+        for (int j = 0; j < kEoE; ++j) {
+            const double4 r4 = *reinterpret_cast<const double4*>(rec + 4L * ix[j]);
+            g.px[j] = T(r4.x);
+            g.ux[j] = T(r4.y);
+            g.hx[j] = T(r4.z);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kEoE; ++j) { g.ux[j] = u[ix[j]]; g.px[j] = pe[ix[j]]; g.hx[j] = he[ix[j]]; }
+    }
+    g.pee = pe[e];
+    g.k1 = ke[c1]; g.k2 = ke[c2]; g.h1 = h[c1]; g.h2 = h[c2];
+    return g;
+}
+
+template <typename T>
+__device__ __forceinline__ void tend_edge_finish(const DMesh& m, const DConfig& c, int stage, T w, T sw,
+                                                 const TendEG<T>& g, T* xu, T* accu, T* pu, int e, bool live)
+{
+    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+    T q = T(0);
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) {
+        const T workpv = T(0.5) * (g.pee + g.px[j]);
+        q = q + T(m.woe[kE2 * e + j]) * g.ux[j] * workpv * g.hx[j];
+    }
+    const T tend = q - (g.k2 - g.k1 + T(kGravity) * (g.h2 + T(m.hs[c2]) - g.h1 - T(m.hs[c1]))) / T(m.dc[e]);
+    if (live) rk_update(stage, w, sw, tend, xu, accu, pu, e, 1, 0, c.frozen_u != 0);
+}
+
 template <typename T>
 __device__ __forceinline__ void tend_edge_item(const DMesh& m, const DConfig& c, int stage, T w, T sw,
                                                const T* h, const T* u, const T* he, const T* pe, const T* ke,
                                                const double* rec, T* xu, T* accu, T* pu, int e, bool live)
 {
-    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
-    int ix[kEoE];
-    T ux[kEoE], px[kEoE], hx[kEoE];
-#pragma unroll
-    for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
-    if (rec) {
-#pragma unroll
-        for (int j = 0; j < kEoE; ++j) {
-            const double4 r4 = *reinterpret_cast<const double4*>(rec + 4L * ix[j]);
-            px[j] = T(r4.x);
-            ux[j] = T(r4.y);
-            hx[j] = T(r4.z);
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < kEoE; ++j) { ux[j] = u[ix[j]]; px[j] = pe[ix[j]]; hx[j] = he[ix[j]]; }
-    }
-    const T pee = pe[e];
-    const T k1 = ke[c1], k2 = ke[c2], h1 = h[c1], h2 = h[c2];
-    T q = T(0);
-#pragma unroll
-    for (int j = 0; j < kEoE; ++j) {
-        const T workpv = T(0.5) * (pee + px[j]);
-        q = q + T(m.woe[kE2 * e + j]) * ux[j] * workpv * hx[j];
-    }
-    const T tend = q - (k2 - k1 + T(kGravity) * (h2 + T(m.hs[c2]) - h1 - T(m.hs[c1]))) / T(m.dc[e]);
-    if (live) rk_update(stage, w, sw, tend, xu, accu, pu, e, 1, 0, c.frozen_u != 0);
+    if (rec)
+        tend_edge_finish(m, c, stage, w, sw, tend_edge_gather<T, true>(m, h, u, he, pe, ke, rec, e), xu, accu, pu, e,
+                         live);
+    else
+        tend_edge_finish(m, c, stage, w, sw, tend_edge_gather<T, false>(m, h, u, he, pe, ke, rec, e), xu, accu, pu,
+                         e, live);
 }
 
 // tend_h and its RK update (k_fwd_tend's cell branch)
 template <typename T>
-__device__ __forceinline__ void tend_cell_item(const DMesh& m, int stage, T w, T sw, const T* u, const T* he,
-                                               T* xh, T* acch, T* ph, int ic, bool live)
-{
-    int ie[kC6];
+struct TendCG {
     T ue[kC6], hh[kC6];
+};
+
+template <typename T>
+__device__ __forceinline__ TendCG<T> tend_cell_gather(const DMesh& m, const T* u, const T* he, int ic)
+{
+    TendCG<T> g;
+    int ie[kC6];
 #pragma unroll
     for (int k = 0; k < kC6; ++k) ie[k] = m.eocR[kC6 * ic + k];
 #pragma unroll
-    for (int k = 0; k < kC6; ++k) { ue[k] = u[ie[k]]; hh[k] = he[ie[k]]; }
+    for (int k = 0; k < kC6; ++k) { g.ue[k] = u[ie[k]]; g.hh[k] = he[ie[k]]; }
+    return g;
+}
+
+template <typename T>
+__device__ __forceinline__ void tend_cell_finish(const DMesh& m, int stage, T w, T sw, const TendCG<T>& g, T* xh,
+                                                 T* acch, T* ph, int ic, bool live)
+{
     T th = T(0);
 #pragma unroll
     for (int k = 0; k < kC6; ++k) {
-        const T flux = ue[k] * T(m.dvCR[kC6 * ic + k]) * hh[k];
+        const T flux = g.ue[k] * T(m.dvCR[kC6 * ic + k]) * g.hh[k];
         th = th + T(-m.sgnCR[kC6 * ic + k]) * flux;
     }
     th = th / T(m.areaC[ic]);
@@ -1689,57 +1784,104 @@ __device__ __forceinline__ void tend_cell_item(const DMesh& m, int stage, T w, T
 }
 
 template <typename T>
-__device__ __forceinline__ void tl_vertex_item(const DMesh& m, const BaseStage& bs, const T* h_tl, const T* u_tl,
-                                               T* pvv_tl, int iv, bool live)
+__device__ __forceinline__ void tend_cell_item(const DMesh& m, int stage, T w, T sw, const T* u, const T* he,
+                                               T* xh, T* acch, T* ph, int ic, bool live)
 {
-    int ie[3], ic[3];
-    T ue[3], hc[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) { ie[i] = m.eovR[3 * iv + i]; ic[i] = m.cov[3 * iv + i]; }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) { ue[i] = u_tl[ie[i]]; hc[i] = h_tl[ic[i]]; }
+    tend_cell_finish(m, stage, w, sw, tend_cell_gather(m, u, he, ic), xh, acch, ph, ic, live);
+}
+
+template <typename T>
+__device__ __forceinline__ VertexG<T> tl_vertex_gather(const DMesh& m, const T* h_tl, const T* u_tl, int iv)
+{
+    return pv_vertex_gather(m, h_tl, u_tl, iv);   // the same neighbours: 3 edges, 3 cells
+}
+
+template <typename T>
+__device__ __forceinline__ void tl_vertex_finish(const DMesh& m, const BaseStage& bs, const VertexG<T>& g,
+                                                 T* pvv_tl, int iv, bool live)
+{
     T ct = T(0);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) ct = ct + T(m.sdcV[3 * iv + i]) * ue[i];
+    for (int i = 0; i < 3; ++i) ct = ct + T(m.sdcV[3 * iv + i]) * g.ue[i];
     const T at = T(m.areaT[iv]);
     const T vt = ct / at;
     T ht = T(0);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) ht = ht + hc[i] * T(m.kite[3 * iv + i]);
+    for (int i = 0; i < 3; ++i) ht = ht + g.hc[i] * T(m.kite[3 * iv + i]);
     ht = ht / at;
     const T hv = T(bs.hv[iv]);
     if (live) stn(pvv_tl + iv, vt / hv - ht * (T(m.fV[iv]) + T(bs.vort[iv])) / (hv * hv));
 }
 
+template <typename T>
+__device__ __forceinline__ void tl_vertex_item(const DMesh& m, const BaseStage& bs, const T* h_tl, const T* u_tl,
+                                               T* pvv_tl, int iv, bool live)
+{
+    tl_vertex_finish(m, bs, tl_vertex_gather(m, h_tl, u_tl, iv), pvv_tl, iv, live);
+}
+
 // k_tl_cell with its two coefficients (2*u and 0.25*dcEdge*dvEdge per edge
 // slot, as k_tl_coef forms them) computed here
 template <typename T>
-__device__ __forceinline__ void tl_cell_item(const DMesh& m, const BaseStage& bs, const T* u_tl, const T* pvv_tl,
-                                             T* ke_tl, T* pc_tl, int ic, bool live)
+__device__ __forceinline__ CellG<T> tl_cell_gather(const DMesh& m, const T* u_tl, const T* pvv_tl, int ic)
 {
-    int ie[kC6], iv[kC6];
-    T ue[kC6], pv[kC6];
-#pragma unroll
-    for (int i = 0; i < kC6; ++i) { ie[i] = m.eoc[kC6 * ic + i]; iv[i] = m.vocR[kC6 * ic + i]; }
-#pragma unroll
-    for (int i = 0; i < kC6; ++i) { ue[i] = u_tl[ie[i]]; pv[i] = pvv_tl[iv[i]]; }
+    return pv_cell_gather(m, u_tl, pvv_tl, ic);   // the same neighbours: 6 edges, 6 vertices
+}
+
+template <typename T>
+__device__ __forceinline__ void tl_cell_finish(const DMesh& m, const BaseStage& bs, const CellG<T>& g, T* ke_tl,
+                                               T* pc_tl, int ic, bool live)
+{
     const T area = T(m.areaC[ic]);
     T k = T(0);
 #pragma unroll
     for (int i = 0; i < kC6; ++i) {
-        const int es = ie[i];
+        const int es = m.eoc[kC6 * ic + i];
         const double r24 = 0.25 * m.dc[es] * m.dv[es];
         const double r12 = 2.0 * bs.u[es];
         const T mask = T(fabs(m.sgnC[kC6 * ic + i]));
-        k = k + mask * (T(r24) * (T(r12) * ue[i]));
+        k = k + mask * (T(r24) * (T(r12) * g.ue[i]));
     }
     k = k / area;
     T p = T(0);
 #pragma unroll
-    for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * pv[i] / area;
+    for (int i = 0; i < kC6; ++i) p = p + T(m.kiteCR[kC6 * ic + i]) * g.pv[i] / area;
     if (!live) return;
     stn(ke_tl + ic, k);
     stn(pc_tl + ic, p);
+}
+
+template <typename T>
+__device__ __forceinline__ void tl_cell_item(const DMesh& m, const BaseStage& bs, const T* u_tl, const T* pvv_tl,
+                                             T* ke_tl, T* pc_tl, int ic, bool live)
+{
+    tl_cell_finish(m, bs, tl_cell_gather(m, u_tl, pvv_tl, ic), ke_tl, pc_tl, ic, live);
+}
+
+template <typename T>
+__device__ __forceinline__ EdgeG<T> tl_edge_gather(const DMesh& m, const T* h_tl, const T* u_tl, const T* pvv_tl,
+                                                   const T* pc_tl, int e)
+{
+    return pv_edge_gather(m, h_tl, u_tl, pvv_tl, pc_tl, e);   // the same neighbours
+}
+
+template <typename T>
+__device__ __forceinline__ void tl_edge_finish(const DMesh& m, const DConfig& c, const BaseStage& bs,
+                                               const EdgeG<T>& g, T* pe_tl, T* he_tl, int e, bool live)
+{
+    const T apvm = T(c.apvm), dt = T(c.dt);
+    T vt = T(0);
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) vt = vt + T(m.woe[kE2 * e + j]) * g.ux[j];
+    const T gtt = (g.p2 - g.p1) / T(m.dv[e]);
+    const T gnt = (g.q2 - g.q1) / T(m.dc[e]);
+    T p = T(0.5) * g.p1 + T(0.5) * g.p2;
+    p = p - apvm * vt * dt * T(bs.gpt[e]) - apvm * T(bs.v[e]) * dt * gtt;
+    p = p - apvm * g.ue * dt * T(bs.gpn[e]) - apvm * T(bs.u[e]) * dt * gnt;
+    if (!live) return;
+    stn(pe_tl + e, p);
+    const T het = T(0.5) * (g.h1 + g.h2);
+    stn(he_tl + e, c.tl_fused ? g.ue * T(bs.he[e]) + T(bs.u[e]) * het : het);
 }
 
 template <typename T>
@@ -1747,63 +1889,74 @@ __device__ __forceinline__ void tl_edge_item(const DMesh& m, const DConfig& c, c
                                              const T* u_tl, const T* pvv_tl, const T* pc_tl, T* pe_tl, T* he_tl,
                                              int e, bool live)
 {
-    const T apvm = T(c.apvm), dt = T(c.dt);
-    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
-    const int v1 = m.voe[2 * e], v2 = m.voe[2 * e + 1];
-    int ix[kEoE];
-    T ux[kEoE];
-#pragma unroll
-    for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
-#pragma unroll
-    for (int j = 0; j < kEoE; ++j) ux[j] = u_tl[ix[j]];
-    const T p1 = pvv_tl[v1], p2 = pvv_tl[v2], q1 = pc_tl[c1], q2 = pc_tl[c2], h1 = h_tl[c1], h2 = h_tl[c2];
-    const T ue = u_tl[e];
-    T vt = T(0);
-#pragma unroll
-    for (int j = 0; j < kEoE; ++j) vt = vt + T(m.woe[kE2 * e + j]) * ux[j];
-    const T gtt = (p2 - p1) / T(m.dv[e]);
-    const T gnt = (q2 - q1) / T(m.dc[e]);
-    T p = T(0.5) * p1 + T(0.5) * p2;
-    p = p - apvm * vt * dt * T(bs.gpt[e]) - apvm * T(bs.v[e]) * dt * gtt;
-    p = p - apvm * ue * dt * T(bs.gpn[e]) - apvm * T(bs.u[e]) * dt * gnt;
-    const T het = T(0.5) * (h1 + h2);
-    if (!live) return;
-    stn(pe_tl + e, p);
-    stn(he_tl + e, c.tl_fused ? ue * T(bs.he[e]) + T(bs.u[e]) * het : het);
+    tl_edge_finish(m, c, bs, tl_edge_gather(m, h_tl, u_tl, pvv_tl, pc_tl, e), pe_tl, he_tl, e, live);
 }
 
 // k_tl_tend's edge branch with its edgesOnEdge coefficients formed from the
 // neighbour records as k_tl_coef forms them
+template <typename T>
+struct TlTendEG {
+    double4 ro[kEoE];   // neighbour records: pe u he gpt
+    T fx[kEoE], px[kEoE], ux[kEoE];
+    T pte, k1, k2, h1, h2;
+};
+
+template <typename T, bool FUSED>
+__device__ __forceinline__ TlTendEG<T> tl_tend_edge_gather(const DMesh& m, const BaseStage& bs, const T* h_tl,
+                                                           const T* u_tl, const T* pe_tl, const T* he_tl,
+                                                           const T* ke_tl, int e)
+{
+    TlTendEG<T> g;
+    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+    int ix[kEoE];
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) {
+        g.ro[j] = *reinterpret_cast<const double4*>(bs.rec + 4L * ix[j]);
+        g.fx[j] = he_tl[ix[j]];
+        g.px[j] = pe_tl[ix[j]];
+        if constexpr (!FUSED) g.ux[j] = u_tl[ix[j]];
+    }
+    g.pte = pe_tl[e];
+    g.k1 = ke_tl[c1]; g.k2 = ke_tl[c2]; g.h1 = h_tl[c1]; g.h2 = h_tl[c2];
+    return g;
+}
+
+template <typename T, bool FUSED>
+__device__ __forceinline__ void tl_tend_edge_finish(const DMesh& m, const DConfig& c, int stage, T w, T sw,
+                                                    const BaseStage& bs, const TlTendEG<T>& g, T* xu, T* accu,
+                                                    T* pu, int e, bool live)
+{
+    const double peb = bs.pe[e];
+    T qt = T(0);
+#pragma unroll
+    for (int j = 0; j < kEoE; ++j) {
+        const double4 ro = g.ro[j];
+        const double wo = m.woe[kE2 * e + j];
+        const double wpv = 0.5 * (peb + ro.x);
+        const double kk = wo * ro.y;
+        if constexpr (FUSED) {
+            qt = qt + T(wo * wpv) * g.fx[j] + T(0.5 * (kk * ro.z)) * (g.pte + g.px[j]);
+        } else {
+            const T wpvt = T(0.5) * (g.pte + g.px[j]);
+            const T heb = T(ro.z);
+            qt = qt + T(wo) * g.ux[j] * T(wpv) * heb + T(kk) * wpvt * heb + T(kk * wpv) * g.fx[j];
+        }
+    }
+    const T tend = qt - (g.k2 - g.k1 + T(kGravity) * (g.h2 - g.h1)) / T(m.dc[e]);
+    if (live) rk_update(stage, w, sw, tend, xu, accu, pu, e, 1, 0, c.frozen_u != 0);
+}
+
 template <typename T, bool FUSED>
 __device__ __forceinline__ void tl_tend_edge_item(const DMesh& m, const DConfig& c, int stage, T w, T sw,
                                                   const BaseStage& bs, const T* h_tl, const T* u_tl, const T* pe_tl,
                                                   const T* he_tl, const T* ke_tl, T* xu, T* accu, T* pu, int e,
                                                   bool live)
 {
-    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
-    int ix[kEoE];
-#pragma unroll
-    for (int j = 0; j < kEoE; ++j) ix[j] = m.eoe[kE2 * e + j];
-    const T pte = pe_tl[e];
-    const T k1 = ke_tl[c1], k2 = ke_tl[c2], h1 = h_tl[c1], h2 = h_tl[c2];
-    const double peb = bs.pe[e];
-    T qt = T(0);
-#pragma unroll
-    for (int j = 0; j < kEoE; ++j) {
-        const double4 ro = *reinterpret_cast<const double4*>(bs.rec + 4L * ix[j]);   // pe u he gpt
-        const double wo = m.woe[kE2 * e + j];
-        const double wpv = 0.5 * (peb + ro.x);
-        const double kk = wo * ro.y;
-        if constexpr (FUSED) {
-            qt = qt + T(wo * wpv) * he_tl[ix[j]] + T(0.5 * (kk * ro.z)) * (pte + pe_tl[ix[j]]);
-        } else {
-            const T wpvt = T(0.5) * (pte + pe_tl[ix[j]]);
-            const T heb = T(ro.z);
-            qt = qt + T(wo) * u_tl[ix[j]] * T(wpv) * heb + T(kk) * wpvt * heb + T(kk * wpv) * he_tl[ix[j]];
-        }
-    }
-    const T tend = qt - (k2 - k1 + T(kGravity) * (h2 - h1)) / T(m.dc[e]);
-    if (live) rk_update(stage, w, sw, tend, xu, accu, pu, e, 1, 0, c.frozen_u != 0);
+    tl_tend_edge_finish<T, FUSED>(m, c, stage, w, sw, bs,
+                                  tl_tend_edge_gather<T, FUSED>(m, bs, h_tl, u_tl, pe_tl, he_tl, ke_tl, e), xu, accu,
+                                  pu, e, live);
 }
 
 template <typename T, bool FUSED>
@@ -1841,8 +1994,11 @@ __device__ __forceinline__ void pv_vertex_chunk(const DMesh& m, const T* __restr
                                                 T* __restrict__ pvv, T* __restrict__ hv, T* __restrict__ vort,
                                                 const ChunkI& ent, const ChunkB& lv)
 {
+    VertexG<T> g[kChunk];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) pv_vertex_item<T>(m, h, u, pvv, hv, vort, ent[j], lv[j]);
+    for (int j = 0; j < kChunk; ++j) g[j] = pv_vertex_gather(m, h, u, ent[j]);
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) pv_vertex_finish(m, g[j], pvv, hv, vort, ent[j], lv[j]);
 }
 
 template <typename T>
@@ -1850,8 +2006,11 @@ __device__ __forceinline__ void pv_cell_chunk(const DMesh& m, const T* __restric
                                               T* __restrict__ ke, T* __restrict__ pc, const ChunkI& ent,
                                               const ChunkB& lv)
 {
+    CellG<T> g[kChunk];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) pv_cell_item<T>(m, u, pvv, ke, pc, ent[j], lv[j]);
+    for (int j = 0; j < kChunk; ++j) g[j] = pv_cell_gather(m, u, pvv, ent[j]);
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) pv_cell_finish(m, g[j], ke, pc, ent[j], lv[j]);
 }
 
 template <typename T>
@@ -1861,11 +2020,18 @@ __device__ __forceinline__ void pv_edge_chunk(const DMesh& m, const DConfig& c, 
                                               T* __restrict__ vout, T* __restrict__ gpt, T* __restrict__ gpn,
                                               double* __restrict__ rec, const ChunkI& ent, const ChunkB& lv)
 {
+    // in pairs: the gathers of two edges in flight (four would not fit the registers of FP64)
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) pv_edge_item<T>(m, c, h, u, pvv, pc, he, pe, vout, gpt, gpn, rec, ent[j], lv[j]);
+    for (int j0 = 0; j0 < kChunk; j0 += 2) {
+        EdgeG<T> g[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) g[j] = pv_edge_gather(m, h, u, pvv, pc, ent[j0 + j]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) pv_edge_finish(m, c, g[j], he, pe, vout, gpt, gpn, rec, ent[j0 + j], lv[j0 + j]);
+    }
 }
 
-template <typename T>
+template <typename T, bool REC>
 __device__ __forceinline__ void tend_edge_chunk(const DMesh& m, const DConfig& c, int stage, T w, T sw,
                                                 const T* __restrict__ h, const T* __restrict__ u,
                                                 const T* __restrict__ he, const T* __restrict__ pe,
@@ -1874,7 +2040,13 @@ __device__ __forceinline__ void tend_edge_chunk(const DMesh& m, const DConfig& c
                                                 const ChunkI& ent, const ChunkB& lv)
 {
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) tend_edge_item<T>(m, c, stage, w, sw, h, u, he, pe, ke, rec, xu, accu, pu, ent[j], lv[j]);
+    for (int j0 = 0; j0 < kChunk; j0 += 2) {
+        TendEG<T> g[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) g[j] = tend_edge_gather<T, REC>(m, h, u, he, pe, ke, rec, ent[j0 + j]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) tend_edge_finish(m, c, stage, w, sw, g[j], xu, accu, pu, ent[j0 + j], lv[j0 + j]);
+    }
 }
 
 template <typename T>
@@ -1882,8 +2054,11 @@ __device__ __forceinline__ void tend_cell_chunk(const DMesh& m, int stage, T w, 
                                                 const T* __restrict__ he, T* __restrict__ xh, T* __restrict__ acch,
                                                 T* __restrict__ ph, const ChunkI& ent, const ChunkB& lv)
 {
+    TendCG<T> g[kChunk];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) tend_cell_item<T>(m, stage, w, sw, u, he, xh, acch, ph, ent[j], lv[j]);
+    for (int j = 0; j < kChunk; ++j) g[j] = tend_cell_gather(m, u, he, ent[j]);
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) tend_cell_finish(m, stage, w, sw, g[j], xh, acch, ph, ent[j], lv[j]);
 }
 
 template <typename T>
@@ -1891,8 +2066,11 @@ __device__ __forceinline__ void tl_vertex_chunk(const DMesh& m, const BaseStage&
                                                 const T* __restrict__ u_tl, T* __restrict__ pvv_tl,
                                                 const ChunkI& ent, const ChunkB& lv)
 {
+    VertexG<T> g[kChunk];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) tl_vertex_item<T>(m, bs, h_tl, u_tl, pvv_tl, ent[j], lv[j]);
+    for (int j = 0; j < kChunk; ++j) g[j] = tl_vertex_gather(m, h_tl, u_tl, ent[j]);
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) tl_vertex_finish(m, bs, g[j], pvv_tl, ent[j], lv[j]);
 }
 
 template <typename T>
@@ -1900,8 +2078,11 @@ __device__ __forceinline__ void tl_cell_chunk(const DMesh& m, const BaseStage& b
                                               const T* __restrict__ pvv_tl, T* __restrict__ ke_tl,
                                               T* __restrict__ pc_tl, const ChunkI& ent, const ChunkB& lv)
 {
+    CellG<T> g[kChunk];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) tl_cell_item<T>(m, bs, u_tl, pvv_tl, ke_tl, pc_tl, ent[j], lv[j]);
+    for (int j = 0; j < kChunk; ++j) g[j] = tl_cell_gather(m, u_tl, pvv_tl, ent[j]);
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) tl_cell_finish(m, bs, g[j], ke_tl, pc_tl, ent[j], lv[j]);
 }
 
 template <typename T, bool FUSED>
@@ -1913,8 +2094,10 @@ __device__ __forceinline__ void tl_tend_edge_chunk(const DMesh& m, const DConfig
                                                    const ChunkI& ent, const ChunkB& lv)
 {
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j)
-        tl_tend_edge_item<T, FUSED>(m, c, stage, w, sw, bs, h_tl, u_tl, pe_tl, he_tl, ke_tl, xu, accu, pu, ent[j], lv[j]);
+    for (int j = 0; j < kChunk; ++j) {
+        const TlTendEG<T> g = tl_tend_edge_gather<T, FUSED>(m, bs, h_tl, u_tl, pe_tl, he_tl, ke_tl, ent[j]);
+        tl_tend_edge_finish<T, FUSED>(m, c, stage, w, sw, bs, g, xu, accu, pu, ent[j], lv[j]);
+    }
 }
 
 template <typename T, bool FUSED>
@@ -1936,7 +2119,13 @@ __device__ __forceinline__ void tl_edge_chunk(const DMesh& m, const DConfig& c, 
                                               const ChunkB& lv)
 {
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) tl_edge_item<T>(m, c, bs, h_tl, u_tl, pvv_tl, pc_tl, pe_tl, he_tl, ent[j], lv[j]);
+    for (int j0 = 0; j0 < kChunk; j0 += 2) {
+        EdgeG<T> g[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) g[j] = tl_edge_gather(m, h_tl, u_tl, pvv_tl, pc_tl, ent[j0 + j]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) tl_edge_finish(m, c, bs, g[j], pe_tl, he_tl, ent[j0 + j], lv[j0 + j]);
+    }
 }
 
 // A chunk whose entities are all of kind ka goes to fa(ent, live), all of
@@ -2114,7 +2303,7 @@ __global__ __launch_bounds__(kPatch) void k_persist_fwd(PersistArgs a)
                 return;
             if (!persistent_pass(a, kEC, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
                     chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
-                        tend_edge_chunk<T>(m, c, s, w, sw, rh, ru, he, pe, ke, nullptr, u, acc + C, nb + C, e, lv); },
+                        tend_edge_chunk<T, false>(m, c, s, w, sw, rh, ru, he, pe, ke, nullptr, u, acc + C, nb + C, e, lv); },
                         1, [&](const ChunkI& e, const ChunkB& lv) {
                         tend_cell_chunk<T>(m, s, w, sw, ru, he, h, acc, nb, e, lv); },
                         [&](int kind, int i) {
@@ -2203,7 +2392,7 @@ __global__ __launch_bounds__(kPatch) void k_persist_tlm(PersistArgs a)
                 return;
             if (!persistent_pass(a, kEC, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
                     chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
-                        tend_edge_chunk<double>(m, c, s, a.k.w[s], a.k.sw[s], b.h, b.u, b.he, b.pe, b.ke, b.rec,
+                        tend_edge_chunk<double, true>(m, c, s, a.k.w[s], a.k.sw[s], b.h, b.u, b.he, b.pe, b.ke, b.rec,
                                                 a.bu, a.bacc + C, nh + C, e, lv);
                         tl_edge_chunk<T>(m, c, b, th, tu, pvv_tl, pc_tl, pe_tl, he_tl, e, lv); },
                         1, [&](const ChunkI& e, const ChunkB& lv) {
