@@ -1979,50 +1979,59 @@ __device__ __forceinline__ void tl_tend_cell_item(const DMesh& m, int stage, T w
     if (live) rk_update(stage, w, sw, th, xh, acch, ph, ic, 1, 0, false);
 }
 
-// Chunks of items.  No entity of a pass reads what the pass writes, so the
-// arrays a chunk function reads and writes are declared __restrict__: the
-// compiler may then issue the gathers of all of a thread's kChunk entities
-// before the first store, instead of walking them one after another with a
-// full memory latency each.  An entity with live false is entity 0, computed
-// and not stored.
-constexpr int kChunk = 4;
-using ChunkI = int[kChunk];
-using ChunkB = bool[kChunk];
+// Chunks of K items (K patches per queue atomic, one entity of each per
+// thread).  No entity of a pass reads what the pass writes, so the arrays a
+// chunk function reads and writes are declared __restrict__: the compiler
+// may then issue the gathers of all of a thread's K entities before the
+// first store, instead of walking them one after another with a full memory
+// latency each.  An entity with live false is entity 0, computed and not
+// stored.
+template <int K>
+using ChunkI = int[K];
+template <int K>
+using ChunkB = bool[K];
 
-template <typename T>
+// K of each persistent kernel (measured on the 85-instance batch of levels
+// 5-9, FP32: forward 9.6 / 9.3 / 8.3 ms per step and TLM 18.1 / 18.3 /
+// 22.5 ms for K = 2 / 4 / 8)
+constexpr int kChunkFwd = 8;
+constexpr int kChunkTlm = 2;
+
+template <typename T, int K>
 __device__ __forceinline__ void pv_vertex_chunk(const DMesh& m, const T* __restrict__ h, const T* __restrict__ u,
                                                 T* __restrict__ pvv, T* __restrict__ hv, T* __restrict__ vort,
-                                                const ChunkI& ent, const ChunkB& lv)
+                                                const ChunkI<K>& ent, const ChunkB<K>& lv)
 {
-    VertexG<T> g[kChunk];
+    VertexG<T> g[K];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) g[j] = pv_vertex_gather(m, h, u, ent[j]);
+    for (int j = 0; j < K; ++j) g[j] = pv_vertex_gather(m, h, u, ent[j]);
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) pv_vertex_finish(m, g[j], pvv, hv, vort, ent[j], lv[j]);
+    for (int j = 0; j < K; ++j) pv_vertex_finish(m, g[j], pvv, hv, vort, ent[j], lv[j]);
 }
 
-template <typename T>
+template <typename T, int K>
 __device__ __forceinline__ void pv_cell_chunk(const DMesh& m, const T* __restrict__ u, const T* __restrict__ pvv,
-                                              T* __restrict__ ke, T* __restrict__ pc, const ChunkI& ent,
-                                              const ChunkB& lv)
+                                              T* __restrict__ ke, T* __restrict__ pc, const ChunkI<K>& ent,
+                                              const ChunkB<K>& lv)
 {
-    CellG<T> g[kChunk];
+    CellG<T> g[K];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) g[j] = pv_cell_gather(m, u, pvv, ent[j]);
+    for (int j = 0; j < K; ++j) g[j] = pv_cell_gather(m, u, pvv, ent[j]);
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) pv_cell_finish(m, g[j], ke, pc, ent[j], lv[j]);
+    for (int j = 0; j < K; ++j) pv_cell_finish(m, g[j], ke, pc, ent[j], lv[j]);
 }
 
-template <typename T>
+template <typename T, int K>
 __device__ __forceinline__ void pv_edge_chunk(const DMesh& m, const DConfig& c, const T* __restrict__ h,
                                               const T* __restrict__ u, const T* __restrict__ pvv,
                                               const T* __restrict__ pc, T* __restrict__ he, T* __restrict__ pe,
                                               T* __restrict__ vout, T* __restrict__ gpt, T* __restrict__ gpn,
-                                              double* __restrict__ rec, const ChunkI& ent, const ChunkB& lv)
+                                              double* __restrict__ rec, const ChunkI<K>& ent, const ChunkB<K>& lv)
 {
     // in pairs: the gathers of two edges in flight (four would not fit the registers of FP64)
+    static_assert(K % 2 == 0, "edge items are taken in pairs");
 #pragma unroll
-    for (int j0 = 0; j0 < kChunk; j0 += 2) {
+    for (int j0 = 0; j0 < K; j0 += 2) {
         EdgeG<T> g[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) g[j] = pv_edge_gather(m, h, u, pvv, pc, ent[j0 + j]);
@@ -2031,16 +2040,17 @@ __device__ __forceinline__ void pv_edge_chunk(const DMesh& m, const DConfig& c, 
     }
 }
 
-template <typename T, bool REC>
+template <typename T, bool REC, int K>
 __device__ __forceinline__ void tend_edge_chunk(const DMesh& m, const DConfig& c, int stage, T w, T sw,
                                                 const T* __restrict__ h, const T* __restrict__ u,
                                                 const T* __restrict__ he, const T* __restrict__ pe,
                                                 const T* __restrict__ ke, const double* __restrict__ rec,
                                                 T* __restrict__ xu, T* __restrict__ accu, T* __restrict__ pu,
-                                                const ChunkI& ent, const ChunkB& lv)
+                                                const ChunkI<K>& ent, const ChunkB<K>& lv)
 {
+    static_assert(K % 2 == 0, "edge items are taken in pairs");
 #pragma unroll
-    for (int j0 = 0; j0 < kChunk; j0 += 2) {
+    for (int j0 = 0; j0 < K; j0 += 2) {
         TendEG<T> g[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) g[j] = tend_edge_gather<T, REC>(m, h, u, he, pe, ke, rec, ent[j0 + j]);
@@ -2049,77 +2059,78 @@ __device__ __forceinline__ void tend_edge_chunk(const DMesh& m, const DConfig& c
     }
 }
 
-template <typename T>
+template <typename T, int K>
 __device__ __forceinline__ void tend_cell_chunk(const DMesh& m, int stage, T w, T sw, const T* __restrict__ u,
                                                 const T* __restrict__ he, T* __restrict__ xh, T* __restrict__ acch,
-                                                T* __restrict__ ph, const ChunkI& ent, const ChunkB& lv)
+                                                T* __restrict__ ph, const ChunkI<K>& ent, const ChunkB<K>& lv)
 {
-    TendCG<T> g[kChunk];
+    TendCG<T> g[K];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) g[j] = tend_cell_gather(m, u, he, ent[j]);
+    for (int j = 0; j < K; ++j) g[j] = tend_cell_gather(m, u, he, ent[j]);
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) tend_cell_finish(m, stage, w, sw, g[j], xh, acch, ph, ent[j], lv[j]);
+    for (int j = 0; j < K; ++j) tend_cell_finish(m, stage, w, sw, g[j], xh, acch, ph, ent[j], lv[j]);
 }
 
-template <typename T>
+template <typename T, int K>
 __device__ __forceinline__ void tl_vertex_chunk(const DMesh& m, const BaseStage& bs, const T* __restrict__ h_tl,
                                                 const T* __restrict__ u_tl, T* __restrict__ pvv_tl,
-                                                const ChunkI& ent, const ChunkB& lv)
+                                                const ChunkI<K>& ent, const ChunkB<K>& lv)
 {
-    VertexG<T> g[kChunk];
+    VertexG<T> g[K];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) g[j] = tl_vertex_gather(m, h_tl, u_tl, ent[j]);
+    for (int j = 0; j < K; ++j) g[j] = tl_vertex_gather(m, h_tl, u_tl, ent[j]);
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) tl_vertex_finish(m, bs, g[j], pvv_tl, ent[j], lv[j]);
+    for (int j = 0; j < K; ++j) tl_vertex_finish(m, bs, g[j], pvv_tl, ent[j], lv[j]);
 }
 
-template <typename T>
+template <typename T, int K>
 __device__ __forceinline__ void tl_cell_chunk(const DMesh& m, const BaseStage& bs, const T* __restrict__ u_tl,
                                               const T* __restrict__ pvv_tl, T* __restrict__ ke_tl,
-                                              T* __restrict__ pc_tl, const ChunkI& ent, const ChunkB& lv)
+                                              T* __restrict__ pc_tl, const ChunkI<K>& ent, const ChunkB<K>& lv)
 {
-    CellG<T> g[kChunk];
+    CellG<T> g[K];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) g[j] = tl_cell_gather(m, u_tl, pvv_tl, ent[j]);
+    for (int j = 0; j < K; ++j) g[j] = tl_cell_gather(m, u_tl, pvv_tl, ent[j]);
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) tl_cell_finish(m, bs, g[j], ke_tl, pc_tl, ent[j], lv[j]);
+    for (int j = 0; j < K; ++j) tl_cell_finish(m, bs, g[j], ke_tl, pc_tl, ent[j], lv[j]);
 }
 
-template <typename T, bool FUSED>
+template <typename T, bool FUSED, int K>
 __device__ __forceinline__ void tl_tend_edge_chunk(const DMesh& m, const DConfig& c, int stage, T w, T sw,
                                                    const BaseStage& bs, const T* __restrict__ h_tl,
                                                    const T* __restrict__ u_tl, const T* __restrict__ pe_tl,
                                                    const T* __restrict__ he_tl, const T* __restrict__ ke_tl,
                                                    T* __restrict__ xu, T* __restrict__ accu, T* __restrict__ pu,
-                                                   const ChunkI& ent, const ChunkB& lv)
+                                                   const ChunkI<K>& ent, const ChunkB<K>& lv)
 {
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) {
+    for (int j = 0; j < K; ++j) {
         const TlTendEG<T> g = tl_tend_edge_gather<T, FUSED>(m, bs, h_tl, u_tl, pe_tl, he_tl, ke_tl, ent[j]);
         tl_tend_edge_finish<T, FUSED>(m, c, stage, w, sw, bs, g, xu, accu, pu, ent[j], lv[j]);
     }
 }
 
-template <typename T, bool FUSED>
+template <typename T, bool FUSED, int K>
 __device__ __forceinline__ void tl_tend_cell_chunk(const DMesh& m, int stage, T w, T sw, const BaseStage& bs,
                                                    const T* __restrict__ u_tl, const T* __restrict__ he_tl,
                                                    T* __restrict__ xh, T* __restrict__ acch, T* __restrict__ ph,
-                                                   const ChunkI& ent, const ChunkB& lv)
+                                                   const ChunkI<K>& ent, const ChunkB<K>& lv)
 {
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j)
+    for (int j = 0; j < K; ++j)
         tl_tend_cell_item<T, FUSED>(m, stage, w, sw, bs, u_tl, he_tl, xh, acch, ph, ent[j], lv[j]);
 }
 
-template <typename T>
+template <typename T, int K>
 __device__ __forceinline__ void tl_edge_chunk(const DMesh& m, const DConfig& c, const BaseStage& bs,
                                               const T* __restrict__ h_tl, const T* __restrict__ u_tl,
                                               const T* __restrict__ pvv_tl, const T* __restrict__ pc_tl,
-                                              T* __restrict__ pe_tl, T* __restrict__ he_tl, const ChunkI& ent,
-                                              const ChunkB& lv)
+                                              T* __restrict__ pe_tl, T* __restrict__ he_tl, const ChunkI<K>& ent,
+                                              const ChunkB<K>& lv)
 {
+    static_assert(K % 2 == 0, "edge items are taken in pairs");
 #pragma unroll
-    for (int j0 = 0; j0 < kChunk; j0 += 2) {
+    for (int j0 = 0; j0 < K; j0 += 2) {
         EdgeG<T> g[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) g[j] = tl_edge_gather(m, h_tl, u_tl, pvv_tl, pc_tl, ent[j0 + j]);
@@ -2132,13 +2143,13 @@ __device__ __forceinline__ void tl_edge_chunk(const DMesh& m, const DConfig& c, 
 // kind kb to fb(ent, live) (the usual case: a queue lists each kind's
 // patches in one run); a chunk that straddles two kinds goes one entity at
 // a time to one(kind, entity).  kb = -2: a pass of one kind.
-template <typename FA, typename FB, typename G>
-__device__ __forceinline__ void chunk_of(const ChunkI& kk, const ChunkI& ent, int ka, FA&& fa, int kb, FB&& fb,
+template <int K, typename FA, typename FB, typename G>
+__device__ __forceinline__ void chunk_of(const ChunkI<K>& kk, const ChunkI<K>& ent, int ka, FA&& fa, int kb, FB&& fb,
                                          G&& one)
 {
-    bool la[kChunk], lb[kChunk], onlya = true, onlyb = true;
+    bool la[K], lb[K], onlya = true, onlyb = true;
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) {
+    for (int j = 0; j < K; ++j) {
         la[j] = kk[j] == ka;
         lb[j] = kk[j] == kb;
         onlya = onlya && (la[j] || kk[j] < 0);
@@ -2150,7 +2161,7 @@ __device__ __forceinline__ void chunk_of(const ChunkI& kk, const ChunkI& ent, in
         fb(ent, lb);
     } else {
 #pragma unroll
-        for (int j = 0; j < kChunk; ++j)
+        for (int j = 0; j < K; ++j)
             if (kk[j] >= 0) one(kk[j], ent[j]);
     }
 }
@@ -2164,11 +2175,14 @@ __device__ bool grid_barrier(unsigned long long* bar, int* err, unsigned long lo
     __shared__ int s_ok;
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();   // this block's writes of the pass, to every XCD
+        // release: this block's writes of the pass (ordered before this
+        // point by the block barrier), to every XCD
         __hip_atomic_fetch_add(bar, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         int ok = 1;
         unsigned polls = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        // relaxed polls: an acquire on every poll would invalidate the
+        // caches that the blocks still working on the pass are reading
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
             if (++polls > limit) {
                 __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2177,7 +2191,8 @@ __device__ bool grid_barrier(unsigned long long* bar, int* err, unsigned long lo
             }
             __builtin_amdgcn_s_sleep(8);
         }
-        __threadfence();   // the other blocks' writes, before this block reads them
+        // acquire: the other blocks' writes, before this block reads them
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         s_ok = ok;
     }
     __syncthreads();
@@ -2191,14 +2206,14 @@ __device__ bool grid_barrier(unsigned long long* bar, int* err, unsigned long lo
 // The items are split into 8 queues, queue x holding the x-th contiguous
 // eighth of every kind's patches, so that the blocks of XCD x (hardware
 // block i runs on XCD i mod 8) work on one region of the space-filling curve
-// and find its neighbours in their own L2.  A block takes kChunk items per
+// and find its neighbours in their own L2.  A block takes K items per
 // atomic from its XCD's queue, then from the others in turn (work stealing)
 // until all are empty.  Passes alternate between two sets of queue counters;
 // block 0 clears the set of the previous pass, which the next barrier
 // publishes before that set is used again.
 constexpr int kQueues = 8;
 
-template <typename F>
+template <int K, typename F>
 __device__ __forceinline__ bool persistent_pass(const PersistArgs& a, const int (&kinds)[3], int nk,
                                                 unsigned long long& phase, F&& chunk)
 {
@@ -2223,16 +2238,16 @@ __device__ __forceinline__ bool persistent_pass(const PersistArgs& a, const int 
         }
         for (;;) {
             if (threadIdx.x == 0)
-                s_q = __hip_atomic_fetch_add(cnt + x, static_cast<unsigned long long>(kChunk), __ATOMIC_RELAXED,
+                s_q = __hip_atomic_fetch_add(cnt + x, static_cast<unsigned long long>(K), __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
             __syncthreads();
             const unsigned long long q = s_q;
             __syncthreads();
             if (q >= static_cast<unsigned long long>(total)) break;
             // the chunk's patches: kind and entity of this thread in each
-            int kk[kChunk], ent[kChunk];
+            int kk[K], ent[K];
 #pragma unroll
-            for (int j = 0; j < kChunk; ++j) {
+            for (int j = 0; j < K; ++j) {
                 const int qq = static_cast<int>(q) + j;
                 kk[j] = -1;
                 ent[j] = 0;
@@ -2262,6 +2277,8 @@ __device__ __forceinline__ bool persistent_pass(const PersistArgs& a, const int 
 template <typename T>
 __global__ __launch_bounds__(kPatch) void k_persist_fwd(PersistArgs a)
 {
+    using CI = ChunkI<kChunkFwd>;
+    using CB = ChunkB<kChunkFwd>;
     const DMesh& m = a.m;
     const DConfig& c = a.c;
     T* h = static_cast<T*>(a.h);
@@ -2281,30 +2298,30 @@ __global__ __launch_bounds__(kPatch) void k_persist_fwd(PersistArgs a)
         for (int s = 0; s < 4; ++s) {
             T* nb = static_cast<T*>((s % 2 == 0) ? a.pA : a.pB);
             const T w = T(a.k.w[s]), sw = T(a.k.sw[s]);
-            if (!persistent_pass(a, kV, 1, phase, [&](const ChunkI& kk, const ChunkI& en) {
-                    chunk_of(kk, en, 0, [&](const ChunkI& e, const ChunkB& lv) {
-                        pv_vertex_chunk<T>(m, rh, ru, pvv, nullptr, nullptr, e, lv); }, -2, [&](const ChunkI&, const ChunkB&) {},
+            if (!persistent_pass<kChunkFwd>(a, kV, 1, phase, [&](const CI& kk, const CI& en) {
+                    chunk_of(kk, en, 0, [&](const CI& e, const CB& lv) {
+                        pv_vertex_chunk<T>(m, rh, ru, pvv, nullptr, nullptr, e, lv); }, -2, [&](const CI&, const CB&) {},
                         [&](int, int i) { pv_vertex_item<T>(m, rh, ru, pvv, nullptr, nullptr, i, true); });
                 }))
                 return;
-            if (!persistent_pass(a, kC, 1, phase, [&](const ChunkI& kk, const ChunkI& en) {
-                    chunk_of(kk, en, 1, [&](const ChunkI& e, const ChunkB& lv) {
-                        pv_cell_chunk<T>(m, ru, pvv, ke, pc, e, lv); }, -2, [&](const ChunkI&, const ChunkB&) {},
+            if (!persistent_pass<kChunkFwd>(a, kC, 1, phase, [&](const CI& kk, const CI& en) {
+                    chunk_of(kk, en, 1, [&](const CI& e, const CB& lv) {
+                        pv_cell_chunk<T>(m, ru, pvv, ke, pc, e, lv); }, -2, [&](const CI&, const CB&) {},
                         [&](int, int i) { pv_cell_item<T>(m, ru, pvv, ke, pc, i, true); });
                 }))
                 return;
-            if (!persistent_pass(a, kE, 1, phase, [&](const ChunkI& kk, const ChunkI& en) {
-                    chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
+            if (!persistent_pass<kChunkFwd>(a, kE, 1, phase, [&](const CI& kk, const CI& en) {
+                    chunk_of(kk, en, 2, [&](const CI& e, const CB& lv) {
                         pv_edge_chunk<T>(m, c, rh, ru, pvv, pc, he, pe, nullptr, nullptr, nullptr, nullptr, e, lv); },
-                        -2, [&](const ChunkI&, const ChunkB&) {},
+                        -2, [&](const CI&, const CB&) {},
                         [&](int, int i) {
                             pv_edge_item<T>(m, c, rh, ru, pvv, pc, he, pe, nullptr, nullptr, nullptr, nullptr, i, true); });
                 }))
                 return;
-            if (!persistent_pass(a, kEC, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
-                    chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
+            if (!persistent_pass<kChunkFwd>(a, kEC, 2, phase, [&](const CI& kk, const CI& en) {
+                    chunk_of(kk, en, 2, [&](const CI& e, const CB& lv) {
                         tend_edge_chunk<T, false>(m, c, s, w, sw, rh, ru, he, pe, ke, nullptr, u, acc + C, nb + C, e, lv); },
-                        1, [&](const ChunkI& e, const ChunkB& lv) {
+                        1, [&](const CI& e, const CB& lv) {
                         tend_cell_chunk<T>(m, s, w, sw, ru, he, h, acc, nb, e, lv); },
                         [&](int kind, int i) {
                             if (kind == 2)
@@ -2328,6 +2345,8 @@ __global__ __launch_bounds__(kPatch) void k_persist_fwd(PersistArgs a)
 template <typename T, bool FUSED>
 __global__ __launch_bounds__(kPatch) void k_persist_tlm(PersistArgs a)
 {
+    using CI = ChunkI<kChunkTlm>;
+    using CB = ChunkB<kChunkTlm>;
     const DMesh& m = a.m;
     const DConfig& c = a.c;
     T* htl = static_cast<T*>(a.h);
@@ -2351,9 +2370,9 @@ __global__ __launch_bounds__(kPatch) void k_persist_tlm(PersistArgs a)
             const T w = T(a.k.w[s]), sw = T(a.k.sw[s]);
             const bool store = s == 0 && a.traj_h != nullptr;
             const size_t tro = static_cast<size_t>(step);
-            if (!persistent_pass(a, store ? kVCE : kV, store ? 3 : 1, phase, [&](const ChunkI& kk, const ChunkI& en) {
-                    chunk_of(kk, en, 0, [&](const ChunkI& e, const ChunkB& lv) {
-                        pv_vertex_chunk<double>(m, b.h, b.u, b.pvv, b.hv, b.vort, e, lv); }, -2, [&](const ChunkI&, const ChunkB&) {},
+            if (!persistent_pass<kChunkTlm>(a, store ? kVCE : kV, store ? 3 : 1, phase, [&](const CI& kk, const CI& en) {
+                    chunk_of(kk, en, 0, [&](const CI& e, const CB& lv) {
+                        pv_vertex_chunk<double>(m, b.h, b.u, b.pvv, b.hv, b.vort, e, lv); }, -2, [&](const CI&, const CB&) {},
                         [&](int kind, int i) {
                             if (kind == 0)
                                 pv_vertex_item<double>(m, b.h, b.u, b.pvv, b.hv, b.vort, i, true);
@@ -2364,10 +2383,10 @@ __global__ __launch_bounds__(kPatch) void k_persist_tlm(PersistArgs a)
                         });
                 }))
                 return;
-            if (!persistent_pass(a, kCV, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
-                    chunk_of(kk, en, 1, [&](const ChunkI& e, const ChunkB& lv) {
+            if (!persistent_pass<kChunkTlm>(a, kCV, 2, phase, [&](const CI& kk, const CI& en) {
+                    chunk_of(kk, en, 1, [&](const CI& e, const CB& lv) {
                         pv_cell_chunk<double>(m, b.u, b.pvv, b.ke, b.pc, e, lv); },
-                        0, [&](const ChunkI& e, const ChunkB& lv) { tl_vertex_chunk<T>(m, b, th, tu, pvv_tl, e, lv); },
+                        0, [&](const CI& e, const CB& lv) { tl_vertex_chunk<T>(m, b, th, tu, pvv_tl, e, lv); },
                         [&](int kind, int i) {
                             if (kind == 1)
                                 pv_cell_item<double>(m, b.u, b.pvv, b.ke, b.pc, i, true);
@@ -2376,10 +2395,10 @@ __global__ __launch_bounds__(kPatch) void k_persist_tlm(PersistArgs a)
                         });
                 }))
                 return;
-            if (!persistent_pass(a, kEC, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
-                    chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
+            if (!persistent_pass<kChunkTlm>(a, kEC, 2, phase, [&](const CI& kk, const CI& en) {
+                    chunk_of(kk, en, 2, [&](const CI& e, const CB& lv) {
                         pv_edge_chunk<double>(m, c, b.h, b.u, b.pvv, b.pc, b.he, b.pe, b.v, b.gpt, b.gpn, b.rec, e, lv); },
-                        1, [&](const ChunkI& e, const ChunkB& lv) {
+                        1, [&](const CI& e, const CB& lv) {
                         tl_cell_chunk<T>(m, b, tu, pvv_tl, ke_tl, pc_tl, e, lv); },
                         [&](int kind, int i) {
                             if (kind == 2)
@@ -2390,12 +2409,12 @@ __global__ __launch_bounds__(kPatch) void k_persist_tlm(PersistArgs a)
                         });
                 }))
                 return;
-            if (!persistent_pass(a, kEC, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
-                    chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
+            if (!persistent_pass<kChunkTlm>(a, kEC, 2, phase, [&](const CI& kk, const CI& en) {
+                    chunk_of(kk, en, 2, [&](const CI& e, const CB& lv) {
                         tend_edge_chunk<double, true>(m, c, s, a.k.w[s], a.k.sw[s], b.h, b.u, b.he, b.pe, b.ke, b.rec,
                                                 a.bu, a.bacc + C, nh + C, e, lv);
                         tl_edge_chunk<T>(m, c, b, th, tu, pvv_tl, pc_tl, pe_tl, he_tl, e, lv); },
-                        1, [&](const ChunkI& e, const ChunkB& lv) {
+                        1, [&](const CI& e, const CB& lv) {
                         tend_cell_chunk<double>(m, s, a.k.w[s], a.k.sw[s], b.u, b.he, a.bh, a.bacc, nh, e, lv); },
                         [&](int kind, int i) {
                             if (kind == 2) {
@@ -2408,11 +2427,11 @@ __global__ __launch_bounds__(kPatch) void k_persist_tlm(PersistArgs a)
                         });
                 }))
                 return;
-            if (!persistent_pass(a, kEC, 2, phase, [&](const ChunkI& kk, const ChunkI& en) {
-                    chunk_of(kk, en, 2, [&](const ChunkI& e, const ChunkB& lv) {
+            if (!persistent_pass<kChunkTlm>(a, kEC, 2, phase, [&](const CI& kk, const CI& en) {
+                    chunk_of(kk, en, 2, [&](const CI& e, const CB& lv) {
                         tl_tend_edge_chunk<T, FUSED>(m, c, s, w, sw, b, th, tu, pe_tl, he_tl, ke_tl, utl, acc + C,
                                                      nb + C, e, lv); },
-                        1, [&](const ChunkI& e, const ChunkB& lv) {
+                        1, [&](const CI& e, const CB& lv) {
                         tl_tend_cell_chunk<T, FUSED>(m, s, w, sw, b, tu, he_tl, htl, acc, nb, e, lv); },
                         [&](int kind, int i) {
                             if (kind == 2)
