@@ -656,6 +656,68 @@ int enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
     });
 }
 
+// Side stream: the FP64 base trajectory of one TLM step (B = 1) into set
+// (st, qs): the step-start state (h, u) is copied into x, the stage-0 input
+// the member kernels read, then each stage's diagnostics, TL coefficient
+// rows and RK update run; stage 3 writes the step's end state into (h, u).
+int side_tlm_base(msw_ctx* c, BaseStage* st, AdjCoef* qs, double* x, double* h, double* u)
+{
+    const DMesh& m = c->dm;
+    const RKCoeffs k = rk_coeffs(c->cfg.dt);
+    const size_t C = c->nC, E = c->nE;
+    hipStream_t ss = c->side;
+    HIP_TRY(hipMemcpyAsync(x, h, C * sizeof(double), hipMemcpyDeviceToDevice, ss));
+    HIP_TRY(hipMemcpyAsync(x + C, u, E * sizeof(double), hipMemcpyDeviceToDevice, ss));
+    st[0].h = x;
+    st[0].u = x + C;
+    for (int s = 0; s < 4; ++s) {
+        BaseStage& b = st[s];
+        fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
+                         b.gpt, b.gpn, b.rec, ss);
+        tl_coef(m, c->dcfg, b, qs[s], ss);
+        double* nh = s < 3 ? st[s + 1].h : c->bend;
+        base_tend(c, b, s, k, h, u, nh, ss);
+    }
+    return MSW_OK;
+}
+
+// One step of a multi-step TLM window, pipelined like the adjoint sweep: the
+// member stages use base set `par`, formed while the previous step ran (or
+// before the first), while the side stream forms the other set for the next
+// step (when `next`) from the base state, which it then advances.  With no
+// serial base prologue per step, every base kernel overlaps member work.
+template <typename T>
+int enqueue_tlm_step_piped(msw_ctx* c, int par, bool next, double* h, double* u, T* htl, T* utl)
+{
+    const DMesh& m = c->dm;
+    const RKCoeffs k = rk_coeffs(c->cfg.dt);
+    const size_t C = c->nC, B = c->B;
+    T* acc = static_cast<T*>(c->macc);
+    T* pA = static_cast<T*>(c->mpA);
+    T* pB = static_cast<T*>(c->mpB);
+    BaseStage* cur = par ? c->stB : c->st;
+    AdjCoef* qcur = par ? c->qsB : c->qs;
+    return forked(c, [&]() -> int {
+        if (next) {
+            const int rc = par ? side_tlm_base(c, c->st, c->qs, c->bx, h, u)
+                               : side_tlm_base(c, c->stB, c->qsB, c->bxB, h, u);
+            if (rc) return rc;
+        }
+        const T* th = htl;
+        const T* tu = utl;
+        for (int s = 0; s < 4; ++s) {
+            T* nb = (s % 2 == 0) ? pA : pB;
+            tl_stage<T>(m, c->dcfg, c->lb, s, k, cur[s], th, tu, static_cast<T*>(c->mv1), static_cast<T*>(c->mc1),
+                        static_cast<T*>(c->mc2), static_cast<T*>(c->me1), static_cast<T*>(c->me2),
+                        static_cast<T*>(c->xvort), member_ds<T>(c), qcur[s], htl, utl, acc, acc + C * B, nb,
+                        nb + C * B, static_cast<T*>(c->tout[2]), static_cast<T*>(c->tout[3]), c->stream);
+            th = nb;
+            tu = nb + C * B;
+        }
+        return MSW_OK;
+    });
+}
+
 // Side stream: the four base stages of an adjoint step whose start state is
 // in x (FP64, B = 1) into `st`, then the coefficient rows of stages 3, 2, 1, 0
 // into `qs`; with `events`, ev[1+s] is recorded once stage s's rows exist.
@@ -1163,6 +1225,30 @@ int msw_tlm_steps(msw_ctx* ctx, double* h, double* u, void* h_tl, void* u_tl, in
     if (ctx->persistent && !tend_kept(ctx))
         return run_persistent(ctx, true, h, u, h_tl, u_tl, nsteps, traj_h, traj_u);
     const size_t C = ctx->nC, E = ctx->nE;
+    if (!tend_kept(ctx)) {
+        if (nsteps == 0) return MSW_OK;
+        // step i runs with base set i % 2, formed while step i-1 ran; the
+        // first set is formed here.  (h, u) run a step ahead of the members:
+        // before step i they hold the start state of step i+1.
+        if (traj_h && (rc = store_state(ctx, traj_h, traj_u, h, u))) return rc;
+        if ((rc = forked(ctx, [&] { return side_tlm_base(ctx, ctx->st, ctx->qs, ctx->bx, h, u); }))) return rc;
+        std::function<int(int)> pre = [&](int i) {
+            return i + 1 < nsteps ? store_state(ctx, traj_h + (i + 1) * C, traj_u + (i + 1) * E, h, u)
+                                  : static_cast<int>(MSW_OK);
+        };
+        std::function<int(int)> variant = [&](int i) { return (i % 2) * 2 + (i + 1 < nsteps ? 1 : 0); };
+        if (ctx->prec == MSW_FP64)
+            return run_steps_var(ctx, 12, h, u, h_tl, u_tl, nsteps, [&](int v) {
+                return enqueue_tlm_step_piped<double>(ctx, v >> 1, (v & 1) != 0, h, u, static_cast<double*>(h_tl),
+                                                      static_cast<double*>(u_tl));
+            }, traj_h ? &pre : nullptr, &variant);
+        return run_steps_var(ctx, 13, h, u, h_tl, u_tl, nsteps, [&](int v) {
+            return enqueue_tlm_step_piped<float>(ctx, v >> 1, (v & 1) != 0, h, u, static_cast<float*>(h_tl),
+                                                 static_cast<float*>(u_tl));
+        }, traj_h ? &pre : nullptr, &variant);
+    }
+    // with tendency outputs, each step forms its own base set first (the
+    // outputs then hold that step's tendencies)
     std::function<int(int)> pre = [&](int i) { return store_state(ctx, traj_h + i * C, traj_u + i * E, h, u); };
     const std::function<int(int)>* pp = traj_h ? &pre : nullptr;
     if (ctx->prec == MSW_FP64)
