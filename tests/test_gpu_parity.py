@@ -805,6 +805,47 @@ def test_pipelined_adjoint_sweep_equals_single_steps(level, prec):
         assert torch.equal(a, outs[0][0]) and torch.equal(b, outs[0][1])
 
 
+@pytest.mark.parametrize("level,prec", [(LEVEL, "fp64"), (LEVEL, "fp32"), (7, "fp64")])
+def test_pipelined_tlm_window_equals_single_steps(level, prec):
+    """64 members: the multi-step TLM window (the next step's base stages
+    and coefficient rows formed on the side stream while the current step's
+    member stages run, alternating two buffer sets, the base state a step
+    ahead) gives the same TL state, base state and stored trajectory, bit
+    for bit, as one-step calls, and as the window with tendency outputs set
+    (each step forming its own base set first), eagerly and through
+    hipGraphs replayed twice."""
+    print(f"pipelined TLM window vs single steps L{level} B=64 {prec}", flush=True)
+    tc, B = 5, 64
+    dt, n = (900.0, 7) if level == LEVEL else (75.0, 4)
+    mesh, phys = case(level, tc)
+    dh, du = perturbations(mesh, B, seed=29)
+    outs = []
+    for graphs in (False, True):
+        m = model(level, tc, dt, batch=B, precision=prec, graphs=graphs, tl_fused_flux=True)
+        for mode in ("window", "window", "single", "tend_out"):
+            h = m.to_internal(phys["h"], "c", member=False)
+            u = m.to_internal(phys["u"], "e", member=False)
+            ht, ut = m.to_internal(dh, "c"), m.to_internal(du, "e")
+            th = torch.empty((n, m.nC), dtype=torch.float64, device=m.device)
+            tu = torch.empty((n, m.nE), dtype=torch.float64, device=m.device)
+            if mode == "single":
+                for k in range(n):
+                    m.tlm(h, u, ht, ut, 1, th[k:k + 1], tu[k:k + 1])
+            elif mode == "tend_out":
+                tht, tut = torch.empty_like(ht), torch.empty_like(ut)
+                m.set_tend_output(tend_h_tl=tht, tend_u_tl=tut)
+                m.tlm(h, u, ht, ut, n, th, tu)
+                m.set_tend_output()
+            else:
+                m.tlm(h, u, ht, ut, n, th, tu)
+            torch.cuda.synchronize()
+            outs.append(tuple(x.cpu() for x in (ht, ut, h, u, th, tu)))
+        del m
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert torch.equal(a, b)
+
+
 @pytest.mark.parametrize("B,prec,tol", [(1, "fp64", 1e-11), (64, "fp64", 1e-11), (4, "fp32", 1e-4)])
 def test_diagnostics_adjoint_matches_oracle(B, prec, tol):
     """msw_diagnostics_adj: seeds in all 13 diagnostic adjoints, different
