@@ -115,6 +115,8 @@ struct BaseStage {
 // member kernel then reads contiguous coefficient rows instead of gathering
 // base values through the connectivity.  The TL stage uses e48 and c32 with
 // its own layout (k_tl_coef); TL and adjoint steps never overlap.
+// The rows are FP64 in FP64 contexts and FP32 in FP32 contexts (the same
+// row strides in elements).
 struct AdjCoef {
     double* e48;   // adjoint [E][24]: woeT*workpv, woeT*(-apvm*dt*gradPVt(eo)) per
                    //   edgesOnEdge slot; the TL stage uses [E][48]
@@ -166,8 +168,10 @@ Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, co
 
 // Coefficient rows of the TL / adjoint stage operator from base fields `b`
 // (B = 1); the member launchers below read them from `q`.
-void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s);
-void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s);
+// f32: rows stored in FP32 (contexts with FP32 members, which round every
+// coefficient to FP32 where they read it, so the results are the same)
+void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, bool f32, hipStream_t s);
+void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, bool f32, hipStream_t s);
 
 // Tangent-linear stage on top of base fields `b` of the same stage.
 // vort_tl [V] and ds are used only when a viscosity is on; the last stage

@@ -576,7 +576,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_vertex(DMesh m, int lb, const 
 
 // :301-316, :401-421
 template <typename T, bool WU>
-__global__ __launch_bounds__(kMaxBlock) void k_tl_cell(DMesh m, int lb, const double* __restrict__ c32,
+__global__ __launch_bounds__(kMaxBlock) void k_tl_cell(DMesh m, int lb, const T* __restrict__ c32,
                                                     const T* __restrict__ u_tl,
                                                     const T* __restrict__ pvv_tl,
                                                     T* __restrict__ ke_tl, T* __restrict__ pc_tl)
@@ -592,7 +592,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_cell(DMesh m, int lb, const do
         for (int i = 0; i < kC6; ++i) { ie[i] = m.eoc[kC6 * ic + i]; iv[i] = m.vocR[kC6 * ic + i]; }
         ld_rows<T, PF>(u_tl, ie, B, ln, ue);
         ld_rows<T, PF>(pvv_tl, iv, B, ln, pv);
-        const double* r = c32 + 32L * ic;
+        const T* r = c32 + 32L * ic;
         const T area = T(m.areaC[ic]);
         T k = T(0);
 #pragma unroll
@@ -652,6 +652,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_edge(DMesh m, DConfig c, int l
     }
 }
 
+// Typed view of a coefficient buffer: FP64 rows in FP64 contexts, FP32 rows
+// in FP32 contexts (whose member kernels round every coefficient to FP32
+// where they read it, so storing it rounded gives the same results).
+template <typename T>
+__host__ __device__ inline T* coef_rows(double* p)
+{
+    return reinterpret_cast<T*>(p);
+}
+
 // Base-state coefficients of the TL stage (B = 1, FP64), in the products
 // the reference forms, so the member kernels read contiguous rows and keep
 // the reference's rounding:
@@ -659,9 +668,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_edge(DMesh m, DConfig c, int l
 //            for the 12 edgesOnEdge slots (src/mpas_sw_dyncore_tlm.f90:564-595)
 //   c32[c] = h_edge, u*dvEdge, dvEdge over the reference-ordered edges of the
 //            cell (:541-561); 2*u and 0.25*dcEdge*dvEdge over its edge slots (:301-316)
+template <typename CT>
 __global__ __launch_bounds__(kMaxBlock) void k_tl_coef(DMesh m, BaseStage bs, int fused,
-                                                    double* __restrict__ e48,
-                                                    double* __restrict__ c32)
+                                                    CT* __restrict__ e48,
+                                                    CT* __restrict__ c32)
 {
     // 16 lanes per edge (slot j = lane), then 8 lanes per cell (slot k)
     const unsigned ne = 16u * static_cast<unsigned>(m.nE);
@@ -674,7 +684,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_coef(DMesh m, BaseStage bs, in
             const double4 ro = *reinterpret_cast<const double4*>(bs.rec + 4L * eo);  // pe u he gpt
             const double wpv = 0.5 * (bs.pe[e] + ro.x);
             const double k2 = m.woe[kE2 * e + j] * ro.y;
-            double* r = e48 + 48L * e;
+            CT* r = e48 + 48L * e;
             if (fused) {
                 // q_tl = sum_j w*workpv*F_tl(eo) + 0.5*w*u(eo)*he(eo)*(pv_tl(e)+pv_tl(eo))
                 r[j] = m.woe[kE2 * e + j] * wpv;
@@ -688,7 +698,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_coef(DMesh m, BaseStage bs, in
         } else {
             const unsigned u = t - ne;
             const int ic = static_cast<int>(u >> 3), k = static_cast<int>(u & 7);
-            double* r = c32 + 32L * ic;
+            CT* r = c32 + 32L * ic;
             if (k < kC6) {
                 const int e = m.eocR[kC6 * ic + k], es = m.eoc[kC6 * ic + k];
                 r[k] = bs.he[e];
@@ -732,8 +742,8 @@ __device__ __forceinline__ T tend_forcing_tl(const DMesh& m, const DConfig& c, c
 // tend_u_tl (:564-595) and tend_h_tl (:541-561), then RK update
 template <typename T, bool WU, bool FUSED>
 __global__ __launch_bounds__(kMaxBlock) void k_tl_tend(DMesh m, DConfig c, int lb, int stage, T w,
-                                                    T sw, BaseStage bs, const double* __restrict__ e48,
-                                                    const double* __restrict__ c32,
+                                                    T sw, BaseStage bs, const T* __restrict__ e48,
+                                                    const T* __restrict__ c32,
                                                     const T* __restrict__ h_tl,
                                                     const T* __restrict__ u_tl,
                                                     const T* __restrict__ pe_tl,
@@ -756,7 +766,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_tend(DMesh m, DConfig c, int l
             T k1, k2, h1, h2;
             ld2<T, PF>(ke_tl, c1, ke_tl, c2, B, ln, k1, k2);
             ld2<T, PF>(h_tl, c1, h_tl, c2, B, ln, h1, h2);
-            const double* r = e48 + 48L * e;
+            const T* r = e48 + 48L * e;
             T qt = T(0);
             if constexpr (FUSED) {
                 // he_tl holds the TL flux F_tl = u_tl*he + u*he_tl
@@ -789,7 +799,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_tend(DMesh m, DConfig c, int l
             int ie[kC6];
 #pragma unroll
             for (int k = 0; k < kC6; ++k) ie[k] = m.eocR[kC6 * ic + k];
-            const double* r = c32 + 32L * ic;
+            const T* r = c32 + 32L * ic;
             T th = T(0);
             if constexpr (FUSED) {
                 T fx[kC6];
@@ -824,9 +834,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_tend(DMesh m, DConfig c, int l
 // edges: tend_u/q adjoint (src/mpas_sw_dyncore_adj.f90:689-717), tend_h
 // adjoint (:720-736), normal upwinding (:193-200), drag and wind (:549-572),
 // v -> u through the transposed edgesOnEdge (:296-304), ke -> u (:306-318)
+template <typename CT>
 __global__ __launch_bounds__(kMaxBlock) void k_adj_coef_edge(DMesh m, DConfig c, BaseStage bs,
-                                                          double* __restrict__ e24,
-                                                          double* __restrict__ e16)
+                                                          CT* __restrict__ e24,
+                                                          CT* __restrict__ e16)
 {
     // 16 lanes per edge (all of them active through the reduction below):
     // lanes 0-9 fill the edgesOnEdge slots, lanes 12-15 the per-edge values
@@ -840,7 +851,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_coef_edge(DMesh m, DConfig c,
         const int eo = m.eoe[kE2 * e + j];
         const double4 ro = *reinterpret_cast<const double4*>(bs.rec + 4L * eo);  // pe u he gpt
         const double wt = m.woeT[kE2 * e + j];
-        double* r = e24 + 24L * e;
+        CT* r = e24 + 24L * e;
         r[j] = wt * (0.5 * (bs.pe[e] + ro.x));
         r[12 + j] = wt * (-c.apvm * c.dt * ro.w);
         term = 0.5 * (m.woe[kE2 * e + j] * ro.y * ro.z);
@@ -849,7 +860,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_coef_edge(DMesh m, DConfig c,
     for (int off = 8; off > 0; off >>= 1) term += __shfl_xor(term, off, 16);
     if (j >= kE2) {
         const double ub = bs.u[e], heb = bs.he[e], dv = m.dv[e];
-        double* q = e16 + 16L * e;
+        CT* q = e16 + 16L * e;
         const int k = j - kE2;
         if (k == 0) {
             q[0] = ub * dv;
@@ -882,8 +893,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_coef_edge(DMesh m, DConfig c,
 
 // vertices: pv_cell -> pv_vertex (:215-229), pv_edge and gradPVt ->
 // pv_vertex (:235-262), pv_vertex -> h_vertex, vorticity (:268-284)
+template <typename CT>
 __global__ __launch_bounds__(kMaxBlock) void k_adj_coef_vertex(DMesh m, DConfig c, BaseStage bs,
-                                                            double* __restrict__ v8)
+                                                            CT* __restrict__ v8)
 {
     FLAT_ITEM(8u * static_cast<unsigned>(m.nV))
     {
@@ -904,14 +916,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_coef_vertex(DMesh m, DConfig 
 
 // cells: gradPVn -> pv_cell (:202-213), gravity -> h (:696-697), ke
 // gradient and drag -> ke (:555-562, :693-695), h_edge -> h (:358-369)
+template <typename CT>
 __global__ __launch_bounds__(kMaxBlock) void k_adj_coef_cell(DMesh m, DConfig c, BaseStage bs,
-                                                          double* __restrict__ c32)
+                                                          CT* __restrict__ c32)
 {
     // 8 lanes per cell: lanes 0-5 fill the edge slots, lane 6 the area
     FLAT_ITEM(8u * static_cast<unsigned>(m.nC))
     {
         const int ic = static_cast<int>(t >> 3), i = static_cast<int>(t & 7);
-        double* q = c32 + 32L * ic;
+        CT* q = c32 + 32L * ic;
         if (i < kC6) {
             const int e = m.eoc[kC6 * ic + i];
             const double s = m.sgnC[kC6 * ic + i], mask = fabs(s), dc = m.dc[e];
@@ -961,8 +974,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_edge1(DMesh m, int lb, AdjCoe
         gh1 = gs * gh1;
         gh2 = gs * gh2;
         const T ge = gs * ld(gu, e, B, b);
-        const double* r = q.e48 + 24L * e;
-        const double* s = q.e8 + 16L * e;
+        const T* r = coef_rows<T>(q.e48) + 24L * e;
+        const T* s = coef_rows<T>(q.e8) + 16L * e;
         const T flux_ad = gh2 * T(s[11]) - gh1 * T(s[10]);
         // edge eo's q contains this edge as a neighbour: with wt the weight of
         // this edge in eo's list, d q(eo) / d u(e) = wt*workpv*h_edge(e) and
@@ -1009,7 +1022,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_cell1(DMesh m, DConfig c, int
         ld_rows<T, PF>(he_ad, ie, B, ln, ha);
 #pragma unroll
         for (int i = 0; i < kC6; ++i) ge[i] = gs * ge[i];
-        const double* r = q.c32 + 32L * ic;
+        const T* r = coef_rows<T>(q.c32) + 32L * ic;
         T pcad = T(0), had = T(0), kad = T(0);
 #pragma unroll
         for (int i = 0; i < kC6; ++i) {
@@ -1059,7 +1072,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_vertex(DMesh m, DConfig c, in
         ld2<T, PF>(pc_ad, ic[0], pc_ad, ic[1], B, ln, pc[0], pc[1]);
         ld2<T, PF>(pc_ad, ic[2], pe_ad, ie[0], B, ln, pc[2], pe[0]);
         ld2<T, PF>(pe_ad, ie[1], pe_ad, ie[2], B, ln, pe[1], pe[2]);
-        const double* r = q.v8 + 8L * iv;
+        const T* r = coef_rows<T>(q.v8) + 8L * iv;
         T pa = T(0);
 #pragma unroll
         for (int i = 0; i < 3; ++i) pa = pa + T(r[i]) * pc[i] + T(r[3 + i]) * pe[i];
@@ -1201,12 +1214,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_final(DMesh m, DConfig c, int
             T cv1, cv2, k1, k2;
             ld2<T, PF>(circ_ad, m.voe[2 * e + 1], circ_ad, m.voe[2 * e], B, ln, cv2, cv1);
             ld2<T, PF>(ke_ad, c1, ke_ad, c2, B, ln, k1, k2);
-            const double* r = q.e48 + 24L * e + 12;
+            const T* r = coef_rows<T>(q.e48) + 24L * e + 12;
             T ua = ld(ua_p, e, B, b);
 #pragma unroll
             for (int j = 0; j < kEoE; ++j) ua = ua + T(r[j]) * px[j];
             ua = ua + T(m.dc[e]) * (cv2 - cv1);
-            const T kc = T(q.e8[16L * e + 6]);
+            const T kc = coef_rows<T>(q.e8)[16L * e + 6];
             ua = ua + kc * k1 + kc * k2;
             // divergence(c) = sum(+-u dv) / area (:221-242)
             if (c.visc2 > 0.0 || c.visc4 > 0.0)
@@ -2518,20 +2531,38 @@ Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, co
     return dp;
 }
 
-void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s)
+template <typename CT>
+void tl_coef_as(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_tl_coef, dim3(grid_items(16L * m.nE + 8L * m.nC, m.stripe_log2, m.block)), dim3(m.block), 0, s, m,
-                       b, c.tl_fused, q.e48, q.c32);
+    hipLaunchKernelGGL(k_tl_coef<CT>, dim3(grid_items(16L * m.nE + 8L * m.nC, m.stripe_log2, m.block)), dim3(m.block), 0, s,
+                       m, b, c.tl_fused, coef_rows<CT>(q.e48), coef_rows<CT>(q.c32));
 }
 
-void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s)
+template <typename CT>
+void adj_coef_as(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adj_coef_edge, dim3(grid_items(16L * m.nE, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
-                       q.e48, q.e8);
-    hipLaunchKernelGGL(k_adj_coef_vertex, dim3(grid_items(8L * m.nV, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
-                       q.v8);
-    hipLaunchKernelGGL(k_adj_coef_cell, dim3(grid_items(8L * m.nC, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
-                       q.c32);
+    hipLaunchKernelGGL(k_adj_coef_edge<CT>, dim3(grid_items(16L * m.nE, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
+                       coef_rows<CT>(q.e48), coef_rows<CT>(q.e8));
+    hipLaunchKernelGGL(k_adj_coef_vertex<CT>, dim3(grid_items(8L * m.nV, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
+                       coef_rows<CT>(q.v8));
+    hipLaunchKernelGGL(k_adj_coef_cell<CT>, dim3(grid_items(8L * m.nC, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
+                       coef_rows<CT>(q.c32));
+}
+
+void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, bool f32, hipStream_t s)
+{
+    if (f32)
+        tl_coef_as<float>(m, c, b, q, s);
+    else
+        tl_coef_as<double>(m, c, b, q, s);
+}
+
+void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, bool f32, hipStream_t s)
+{
+    if (f32)
+        adj_coef_as<float>(m, c, b, q, s);
+    else
+        adj_coef_as<double>(m, c, b, q, s);
 }
 
 template <typename T>
@@ -2543,7 +2574,8 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
     const bool visc = c.visc2 > 0.0 || c.visc4 > 0.0;
     LAUNCH_WU(k_tl_vertex, T, lb, grid_items(static_cast<long>(m.nV) << lb, m.stripe_log2, m.block), m.block, s, m, lb, b.hv, b.vort,
               h_tl, u_tl, pvv_tl, visc ? vort_tl : nullptr);
-    LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2, m.block), m.block, s, m, lb, q.c32, u_tl,
+    LAUNCH_WU(k_tl_cell, T, lb, grid_items(static_cast<long>(m.nC) << lb, m.stripe_log2, m.block), m.block, s, m, lb,
+              coef_rows<T>(q.c32), u_tl,
               pvv_tl, ke_tl, pc_tl);
     LAUNCH_WU(k_tl_edge, T, lb, grid_items(static_cast<long>(m.nE) << lb, m.stripe_log2, m.block), m.block, s, m, c, lb, b, h_tl,
               u_tl, pvv_tl, pc_tl, pe_tl, he_tl);
@@ -2551,7 +2583,7 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
     // the fused and reference-order forms are separate instances, so each
     // gets its own register allocation
     const int g = grid_edge_cell(m.nE, m.nC, lb, m.stripe_log2, m.block);
-#define TL_TEND_ARGS s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), b, q.e48, q.c32, h_tl, u_tl, \
+#define TL_TEND_ARGS s, m, c, lb, stage, T(k.w[stage]), T(k.sw[stage]), b, coef_rows<T>(q.e48), coef_rows<T>(q.c32), h_tl, u_tl, \
                      pe_tl, he_tl, ke_tl, dp, xh, xu, acch, accu, ph, pu, stage == 3 ? toh : nullptr, \
                      stage == 3 ? tou : nullptr
     if (lb >= 6) {

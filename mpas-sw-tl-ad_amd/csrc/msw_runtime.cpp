@@ -99,7 +99,6 @@ struct msw_ctx {
     double* usrc = nullptr;           // [E]  wind-stress source, internal order
     double* msd = nullptr;            // [2E] meshScalingDel2, meshScalingDel4 (internal order)
     bool msd_on = false;              // false: the factors are all 1
-    AdjCoef adjq{};                   // stage coefficients (B = 1) of the eager single-stream path
     AdjCoef qs[4]{};                  // per-stage coefficients of the two-stream steps
     // second base-stage / coefficient set and step-start state: a multi-step
     // adjoint sweep forms the next step's set while the current one is used
@@ -450,12 +449,6 @@ int alloc_buffers(msw_ctx* c)
     take(&c->macc, C + E); take(&c->mpA, C + E); take(&c->mpB, C + E);
     take(&c->mv1, V); take(&c->mv2, V); take(&c->mc1, C); take(&c->mc2, C); take(&c->mc3, C);
     take(&c->me1, E); take(&c->me2, E); take(&c->me3, E);
-    double* q = nullptr;
-    if ((rc = c->alloc(&q, 64 * E + 8 * V + 32 * C))) return rc;
-    c->adjq.e48 = q;
-    c->adjq.e8 = q + 48 * E;
-    c->adjq.v8 = q + 64 * E;
-    c->adjq.c32 = q + 64 * E + 8 * V;
     for (AdjCoef* set : {c->qs, c->qsB})
         for (int st = 0; st < 4; ++st) {
             double* qq = nullptr;
@@ -635,7 +628,7 @@ int enqueue_tlm_step(msw_ctx* c, double* h, double* u, T* htl, T* utl)
         BaseStage& b = c->st[s];
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
                          b.gpt, b.gpn, b.rec, ss);
-        tl_coef(m, c->dcfg, b, c->qs[s], ss);
+        tl_coef(m, c->dcfg, b, c->qs[s], c->prec == MSW_FP32, ss);
         HIP_TRY(hipEventRecord(c->ev[1 + s], ss));
         double* nh = s < 3 ? c->st[s + 1].h : c->bend;
         base_tend(c, b, s, k, h, u, nh, ss);
@@ -674,7 +667,7 @@ int side_tlm_base(msw_ctx* c, BaseStage* st, AdjCoef* qs, double* x, double* h, 
         BaseStage& b = st[s];
         fwd_diag<double>(m, c->dcfg, 0, b.h, b.u, b.pvv, b.hv, b.vort, b.ke, b.pc, b.he, b.pe, b.v,
                          b.gpt, b.gpn, b.rec, ss);
-        tl_coef(m, c->dcfg, b, qs[s], ss);
+        tl_coef(m, c->dcfg, b, qs[s], c->prec == MSW_FP32, ss);
         double* nh = s < 3 ? st[s + 1].h : c->bend;
         base_tend(c, b, s, k, h, u, nh, ss);
     }
@@ -742,7 +735,7 @@ int side_adj_base(msw_ctx* c, BaseStage* st, AdjCoef* qs, double* x, bool want_e
         }
     }
     for (int s = 3; s >= 0; --s) {
-        adj_coef(m, c->dcfg, st[s], qs[s], ss);
+        adj_coef(m, c->dcfg, st[s], qs[s], c->prec == MSW_FP32, ss);
         if (events) HIP_TRY(hipEventRecord(c->ev[1 + s], ss));
     }
     return MSW_OK;

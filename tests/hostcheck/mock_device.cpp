@@ -791,36 +791,38 @@ Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, co
     return dp;
 }
 
-void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& bs, const AdjCoef& q, hipStream_t s)
+void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& bs, const AdjCoef& q, bool f32, hipStream_t s)
 {
     check_shape(m, "tl_coef");
     const size_t C = m.nC, E = m.nE, V = m.nV;
+    const size_t cs = f32 ? sizeof(float) : sizeof(double);   // coefficient rows
     std::vector<Buf> b;
     mesh_reads(m, b);
     base_reads(bs, C, E, V, b, false);
     b.push_back({bs.rec, 4 * E * D, false, "base edge records", false, 32});
     b.push_back(R(bs.pe, E * D, "base pv_edge"));
-    b.push_back(W(q.e48, 48 * E * D, "TL edge coefficients"));
-    b.push_back(W(q.c32, 32 * C * D, "TL cell coefficients"));
+    b.push_back(W(q.e48, 48 * E * cs, "TL edge coefficients"));
+    b.push_back(W(q.c32, 32 * C * cs, "TL cell coefficients"));
     launch(s, "k_tl_coef", b);
     (void)c;
 }
 
-void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& bs, const AdjCoef& q, hipStream_t s)
+void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& bs, const AdjCoef& q, bool f32, hipStream_t s)
 {
     check_shape(m, "adj_coef");
     const size_t C = m.nC, E = m.nE, V = m.nV;
+    const size_t cs = f32 ? sizeof(float) : sizeof(double);   // coefficient rows
     std::vector<Buf> b;
     mesh_reads(m, b);
     config_reads(m, c, b);
     base_reads(bs, C, E, V, b, true);
     std::vector<Buf> be = b, bv = b, bc = b;
-    be.insert(be.end(), {W(q.e48, 24 * E * D, "adjoint edge slot coefficients"),
-                         W(q.e8, 16 * E * D, "adjoint edge coefficients")});
+    be.insert(be.end(), {W(q.e48, 24 * E * cs, "adjoint edge slot coefficients"),
+                         W(q.e8, 16 * E * cs, "adjoint edge coefficients")});
     launch(s, "k_adj_coef_edge", be);
-    bv.push_back(W(q.v8, 8 * V * D, "adjoint vertex coefficients"));
+    bv.push_back(W(q.v8, 8 * V * cs, "adjoint vertex coefficients"));
     launch(s, "k_adj_coef_vertex", bv);
-    bc.push_back(W(q.c32, 32 * C * D, "adjoint cell coefficients"));
+    bc.push_back(W(q.c32, 32 * C * cs, "adjoint cell coefficients"));
     launch(s, "k_adj_coef_cell", bc);
 }
 
@@ -840,7 +842,7 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
                          MR(u_tl, E, lb, "u_tl"), MW(pvv_tl, V, lb, "pv_vertex_tl")});
     if (visc) b1.push_back(MW(vort_tl, V, lb, "vorticity_tl"));
     launch(s, "k_tl_vertex", b1);
-    b2.insert(b2.end(), {R(q.c32, 32 * C * D, "TL cell coefficients"), MR(u_tl, E, lb, "u_tl"),
+    b2.insert(b2.end(), {R(q.c32, 32 * C * sizeof(T), "TL cell coefficients"), MR(u_tl, E, lb, "u_tl"),
                          MR(pvv_tl, V, lb, "pv_vertex_tl"), MW(ke_tl, C, lb, "ke_tl"), MW(pc_tl, C, lb, "pv_cell_tl")});
     launch(s, "k_tl_cell", b2);
     b3.insert(b3.end(), {R(bs.gpt, E * D, "base gradPVt"), R(bs.v, E * D, "base v"), R(bs.gpn, E * D, "base gradPVn"),
@@ -856,7 +858,7 @@ void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeff
         b4.push_back(R(bs.u, E * D, "base u"));
         b4.push_back(R(bs.ke, C * D, "base ke"));
     }
-    b4.insert(b4.end(), {R(q.e48, 48 * E * D, "TL edge coefficients"), R(q.c32, 32 * C * D, "TL cell coefficients"),
+    b4.insert(b4.end(), {R(q.e48, 48 * E * sizeof(T), "TL edge coefficients"), R(q.c32, 32 * C * sizeof(T), "TL cell coefficients"),
                          MR(h_tl, C, lb, "h_tl"), MR(u_tl, E, lb, "u_tl"), MR(pe_tl, E, lb, "pv_edge_tl"),
                          MR(he_tl, E, lb, "h_edge_tl"), MR(ke_tl, C, lb, "ke_tl")});
     rk_bufs(b4, stage, xu, accu, pu, E, lb, c.frozen_u != 0, "u_tl");
@@ -892,22 +894,22 @@ void adj_stage(const DMesh& m, const DConfig& c, int lb, int stage, bool first, 
         launch(s, "k_adj_del4_edge", d3);
     }
     std::vector<Buf> b1 = b, b2 = b, b3 = b, b4 = b;
-    b1.insert(b1.end(), {R(q.e48, 24 * E * D, "adjoint edge slot coefficients"), R(q.e8, 16 * E * D, "adjoint edge coefficients"),
+    b1.insert(b1.end(), {R(q.e48, 24 * E * sizeof(T), "adjoint edge slot coefficients"), R(q.e8, 16 * E * sizeof(T), "adjoint edge coefficients"),
                          MR(rgh, C, lb, "h tendency adjoint"), MR(rgu, E, lb, "u tendency adjoint"),
                          MW(pe_ad, E, lb, "pv_edge_ad"), MW(he_ad, E, lb, "h_edge_ad"), MW(ua_p, E, lb, "partial u_ad")});
     launch(s, "k_adj_edge1", b1);
-    b2.insert(b2.end(), {R(q.c32, 32 * C * D, "adjoint cell coefficients"), MR(rgu, E, lb, "u tendency adjoint"),
+    b2.insert(b2.end(), {R(q.c32, 32 * C * sizeof(T), "adjoint cell coefficients"), MR(rgu, E, lb, "u tendency adjoint"),
                          MR(pe_ad, E, lb, "pv_edge_ad"), MR(he_ad, E, lb, "h_edge_ad"), MW(pc_ad, C, lb, "pv_cell_ad"),
                          MW(ha_p, C, lb, "partial h_ad"), MW(ke_ad, C, lb, "ke_ad")});
     if (c.visc4 > 0.0) b2.push_back(MR(ds.dsu, E, lb, "del2 u adjoint"));
     if (visc) b2.push_back(MW(ds.div, C, lb, "divergence adjoint"));
     launch(s, "k_adj_cell1", b2);
-    b3.insert(b3.end(), {R(q.v8, 8 * V * D, "adjoint vertex coefficients"), MR(pe_ad, E, lb, "pv_edge_ad"),
+    b3.insert(b3.end(), {R(q.v8, 8 * V * sizeof(T), "adjoint vertex coefficients"), MR(pe_ad, E, lb, "pv_edge_ad"),
                          MR(pc_ad, C, lb, "pv_cell_ad"), MW(hv_ad, V, lb, "h_vertex_ad"), MW(circ_ad, V, lb, "circulation_ad")});
     if (c.visc2 > 0.0) b3.push_back(MR(rgu, E, lb, "u tendency adjoint"));
     if (c.visc4 > 0.0) b3.push_back(MR(ds.dsu, E, lb, "del2 u adjoint"));
     launch(s, "k_adj_vertex", b3);
-    b4.insert(b4.end(), {R(q.e48, 24 * E * D, "adjoint edge slot coefficients"), R(q.e8, 16 * E * D, "adjoint edge coefficients"),
+    b4.insert(b4.end(), {R(q.e48, 24 * E * sizeof(T), "adjoint edge slot coefficients"), R(q.e8, 16 * E * sizeof(T), "adjoint edge coefficients"),
                          MR(pe_ad, E, lb, "pv_edge_ad"), MR(ua_p, E, lb, "partial u_ad"), MR(ha_p, C, lb, "partial h_ad"),
                          MR(hv_ad, V, lb, "h_vertex_ad"), MR(circ_ad, V, lb, "circulation_ad"), MR(ke_ad, C, lb, "ke_ad")});
     if (visc) b4.push_back(MR(ds.div, C, lb, "divergence adjoint"));
