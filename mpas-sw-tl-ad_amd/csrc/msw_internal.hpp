@@ -172,6 +172,8 @@ Dissip<T> dissip_fields(const DMesh& m, const DConfig& c, int lb, const T* u, co
 // coefficient to FP32 where they read it, so the results are the same)
 void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, bool f32, hipStream_t s);
 void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, bool f32, hipStream_t s);
+// the four stages' rows in one pass over the mesh tables (b[s], q[s] for s = 0..3)
+void adj_coef4(const DMesh& m, const DConfig& c, const BaseStage* b, const AdjCoef* q, bool f32, hipStream_t s);
 
 // Tangent-linear stage on top of base fields `b` of the same stage.
 // vort_tl [V] and ds are used only when a viscosity is on; the last stage

@@ -834,10 +834,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_tl_tend(DMesh m, DConfig c, int l
 // edges: tend_u/q adjoint (src/mpas_sw_dyncore_adj.f90:689-717), tend_h
 // adjoint (:720-736), normal upwinding (:193-200), drag and wind (:549-572),
 // v -> u through the transposed edgesOnEdge (:296-304), ke -> u (:306-318)
-template <typename CT>
-__global__ __launch_bounds__(kMaxBlock) void k_adj_coef_edge(DMesh m, DConfig c, BaseStage bs,
-                                                          CT* __restrict__ e24,
-                                                          CT* __restrict__ e16)
+// The base stages and coefficient rows of NS stages of one step, formed by
+// one launch: the mesh tables are read once for all of them (a multi-step
+// adjoint sweep forms the four stages of the next step together).
+template <int NS>
+struct AdjCoefStages {
+    BaseStage bs[NS];
+    AdjCoef q[NS];
+};
+
+template <typename CT, int NS>
+__global__ __launch_bounds__(kMaxBlock) void k_adj_coef_edge(DMesh m, DConfig c, AdjCoefStages<NS> a)
 {
     // 16 lanes per edge (all of them active through the reduction below):
     // lanes 0-9 fill the edgesOnEdge slots, lanes 12-15 the per-edge values
@@ -846,101 +853,121 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj_coef_edge(DMesh m, DConfig c,
     if (!it_.live) return;   // whole 16-lane groups only, so the shuffles stay inside live lanes
     const unsigned t = it_.t;
     const int e = static_cast<int>(t >> 4), j = static_cast<int>(t & 15);
-    double term = 0.0;   // 0.5*w*u(eo)*he(eo), summed over the slots
-    if (j < kEoE) {      // the member kernels read the 10 real slots only
-        const int eo = m.eoe[kE2 * e + j];
-        const double4 ro = *reinterpret_cast<const double4*>(bs.rec + 4L * eo);  // pe u he gpt
-        const double wt = m.woeT[kE2 * e + j];
-        CT* r = e24 + 24L * e;
-        r[j] = wt * (0.5 * (bs.pe[e] + ro.x));
-        r[12 + j] = wt * (-c.apvm * c.dt * ro.w);
-        term = 0.5 * (m.woe[kE2 * e + j] * ro.y * ro.z);
-    }
+    const bool slot = j < kEoE;   // the member kernels read the 10 real slots only
+    const int eo = slot ? m.eoe[kE2 * e + j] : 0;
+    const double wt = slot ? m.woeT[kE2 * e + j] : 0.0, wo = slot ? m.woe[kE2 * e + j] : 0.0;
+    const double dv = m.dv[e];
 #pragma unroll
-    for (int off = 8; off > 0; off >>= 1) term += __shfl_xor(term, off, 16);
-    if (j >= kE2) {
-        const double ub = bs.u[e], heb = bs.he[e], dv = m.dv[e];
-        CT* q = e16 + 16L * e;
-        const int k = j - kE2;
-        if (k == 0) {
-            q[0] = ub * dv;
-            q[1] = dv * heb;
-            q[7] = ub;
-            q[8] = heb;
-        } else if (k == 1) {
-            q[2] = term;
-            q[3] = c.apvm * c.dt * bs.gpn[e];
-            q[9] = 0.5 * (ub * heb);
-        } else if (k == 2) {
-            double hge = 0.0, uge = 0.0;
-            if (c.drag) {
-                const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
-                const double sq = sqrt(2.0 * (0.5 * (bs.ke[c1] + bs.ke[c2])));
-                hge = kDragCoef * ub * sq / (heb * heb);
-                uge = -kDragCoef * sq / heb;
+    for (int st = 0; st < NS; ++st) {
+        const BaseStage& bs = a.bs[st];
+        double term = 0.0;   // 0.5*w*u(eo)*he(eo), summed over the slots
+        if (slot) {
+            const double4 ro = *reinterpret_cast<const double4*>(bs.rec + 4L * eo);  // pe u he gpt
+            CT* r = coef_rows<CT>(a.q[st].e48) + 24L * e;
+            r[j] = wt * (0.5 * (bs.pe[e] + ro.x));
+            r[12 + j] = wt * (-c.apvm * c.dt * ro.w);
+            term = 0.5 * (wo * ro.y * ro.z);
+        }
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) term += __shfl_xor(term, off, 16);
+        if (j >= kE2) {
+            const double ub = bs.u[e], heb = bs.he[e];
+            CT* q = coef_rows<CT>(a.q[st].e8) + 16L * e;
+            const int k = j - kE2;
+            if (k == 0) {
+                q[0] = ub * dv;
+                q[1] = dv * heb;
+                q[7] = ub;
+                q[8] = heb;
+            } else if (k == 1) {
+                q[2] = term;
+                q[3] = c.apvm * c.dt * bs.gpn[e];
+                q[9] = 0.5 * (ub * heb);
+            } else if (k == 2) {
+                double hge = 0.0, uge = 0.0;
+                if (c.drag) {
+                    const int c1 = m.coe[2 * e], c2 = m.coe[2 * e + 1];
+                    const double sq = sqrt(2.0 * (0.5 * (bs.ke[c1] + bs.ke[c2])));
+                    hge = kDragCoef * ub * sq / (heb * heb);
+                    uge = -kDragCoef * sq / heb;
+                }
+                if (c.wind) hge = hge - c.usrc[e] / kRhoRef / (heb * heb);
+                q[4] = hge;
+                q[5] = uge;
+            } else {
+                q[6] = 0.25 * m.dc[e] * dv * 2.0 * ub;
+                q[10] = 1.0 / m.areaC[m.coe[2 * e]];
+                q[11] = 1.0 / m.areaC[m.coe[2 * e + 1]];
+                for (int i = 12; i < 16; ++i) q[i] = 0.0;
             }
-            if (c.wind) hge = hge - c.usrc[e] / kRhoRef / (heb * heb);
-            q[4] = hge;
-            q[5] = uge;
-        } else {
-            q[6] = 0.25 * m.dc[e] * dv * 2.0 * ub;
-            q[10] = 1.0 / m.areaC[m.coe[2 * e]];
-            q[11] = 1.0 / m.areaC[m.coe[2 * e + 1]];
-            for (int i = 12; i < 16; ++i) q[i] = 0.0;
         }
     }
 }
 
 // vertices: pv_cell -> pv_vertex (:215-229), pv_edge and gradPVt ->
 // pv_vertex (:235-262), pv_vertex -> h_vertex, vorticity (:268-284)
-template <typename CT>
-__global__ __launch_bounds__(kMaxBlock) void k_adj_coef_vertex(DMesh m, DConfig c, BaseStage bs,
-                                                            CT* __restrict__ v8)
+template <typename CT, int NS>
+__global__ __launch_bounds__(kMaxBlock) void k_adj_coef_vertex(DMesh m, DConfig c, AdjCoefStages<NS> a)
 {
     FLAT_ITEM(8u * static_cast<unsigned>(m.nV))
     {
         const int iv = static_cast<int>(t / 8), k = static_cast<int>(t % 8);
-        double v;
         if (k < 3) {
-            v = m.kite[3 * iv + k] / m.areaC[m.cov[3 * iv + k]];
+            const double v = m.kite[3 * iv + k] / m.areaC[m.cov[3 * iv + k]];
+#pragma unroll
+            for (int st = 0; st < NS; ++st) coef_rows<CT>(a.q[st].v8)[t] = v;
         } else if (k < 6) {
             const int e = m.eovR[3 * iv + k - 3];
-            v = 0.5 + m.sgnV[3 * iv + k - 3] * (-c.apvm * bs.v[e] * c.dt) / m.dv[e];
+            const double sg = m.sgnV[3 * iv + k - 3], dv = m.dv[e];
+#pragma unroll
+            for (int st = 0; st < NS; ++st)
+                coef_rows<CT>(a.q[st].v8)[t] = 0.5 + sg * (-c.apvm * a.bs[st].v[e] * c.dt) / dv;
         } else {
-            const double hv = bs.hv[iv], at = m.areaT[iv];
-            v = k == 6 ? -(m.fV[iv] + bs.vort[iv]) / (hv * hv) / at : 1.0 / hv / at;
+            const double at = m.areaT[iv], fv = m.fV[iv];
+#pragma unroll
+            for (int st = 0; st < NS; ++st) {
+                const double hv = a.bs[st].hv[iv];
+                coef_rows<CT>(a.q[st].v8)[t] = k == 6 ? -(fv + a.bs[st].vort[iv]) / (hv * hv) / at : 1.0 / hv / at;
+            }
         }
-        v8[t] = v;
     }
 }
 
 // cells: gradPVn -> pv_cell (:202-213), gravity -> h (:696-697), ke
 // gradient and drag -> ke (:555-562, :693-695), h_edge -> h (:358-369)
-template <typename CT>
-__global__ __launch_bounds__(kMaxBlock) void k_adj_coef_cell(DMesh m, DConfig c, BaseStage bs,
-                                                          CT* __restrict__ c32)
+template <typename CT, int NS>
+__global__ __launch_bounds__(kMaxBlock) void k_adj_coef_cell(DMesh m, DConfig c, AdjCoefStages<NS> a)
 {
     // 8 lanes per cell: lanes 0-5 fill the edge slots, lane 6 the area
     FLAT_ITEM(8u * static_cast<unsigned>(m.nC))
     {
         const int ic = static_cast<int>(t >> 3), i = static_cast<int>(t & 7);
-        CT* q = c32 + 32L * ic;
         if (i < kC6) {
             const int e = m.eoc[kC6 * ic + i];
             const double s = m.sgnC[kC6 * ic + i], mask = fabs(s), dc = m.dc[e];
-            q[i] = s * c.apvm * bs.u[e] * c.dt / dc;
-            q[6 + i] = s * (kGravity / dc);
-            double kk = s / dc;
-            if (c.drag) {
-                const double kee = 0.5 * (bs.ke[m.coe[2 * e]] + bs.ke[m.coe[2 * e + 1]]);
-                kk = kk + mask * 0.5 * (-0.5 * kDragCoef * bs.u[e] * 2.0 / sqrt(2.0 * kee) / bs.he[e]);
+            const double grav = s * (kGravity / dc);
+#pragma unroll
+            for (int st = 0; st < NS; ++st) {
+                const BaseStage& bs = a.bs[st];
+                CT* q = coef_rows<CT>(a.q[st].c32) + 32L * ic;
+                q[i] = s * c.apvm * bs.u[e] * c.dt / dc;
+                q[6 + i] = grav;
+                double kk = s / dc;
+                if (c.drag) {
+                    const double kee = 0.5 * (bs.ke[m.coe[2 * e]] + bs.ke[m.coe[2 * e + 1]]);
+                    kk = kk + mask * 0.5 * (-0.5 * kDragCoef * bs.u[e] * 2.0 / sqrt(2.0 * kee) / bs.he[e]);
+                }
+                q[12 + i] = kk;
+                q[18 + i] = mask * 0.5;
             }
-            q[12 + i] = kk;
-            q[18 + i] = mask * 0.5;
         } else if (i == 6) {
-            q[24] = 1.0 / m.areaC[ic];
+            const double ra = 1.0 / m.areaC[ic];
+#pragma unroll
+            for (int st = 0; st < NS; ++st) coef_rows<CT>(a.q[st].c32)[32L * ic + 24] = ra;
         } else {
-            for (int k = 25; k < 32; ++k) q[k] = 0.0;
+#pragma unroll
+            for (int st = 0; st < NS; ++st)
+                for (int k = 25; k < 32; ++k) coef_rows<CT>(a.q[st].c32)[32L * ic + k] = 0.0;
         }
     }
 }
@@ -2538,15 +2565,20 @@ void tl_coef_as(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjC
                        m, b, c.tl_fused, coef_rows<CT>(q.e48), coef_rows<CT>(q.c32));
 }
 
-template <typename CT>
-void adj_coef_as(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, hipStream_t s)
+template <typename CT, int NS>
+void adj_coef_as(const DMesh& m, const DConfig& c, const BaseStage* b, const AdjCoef* q, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adj_coef_edge<CT>, dim3(grid_items(16L * m.nE, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
-                       coef_rows<CT>(q.e48), coef_rows<CT>(q.e8));
-    hipLaunchKernelGGL(k_adj_coef_vertex<CT>, dim3(grid_items(8L * m.nV, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
-                       coef_rows<CT>(q.v8));
-    hipLaunchKernelGGL(k_adj_coef_cell<CT>, dim3(grid_items(8L * m.nC, m.stripe_log2, m.block)), dim3(m.block), 0, s, m, c, b,
-                       coef_rows<CT>(q.c32));
+    AdjCoefStages<NS> a;
+    for (int i = 0; i < NS; ++i) {
+        a.bs[i] = b[i];
+        a.q[i] = q[i];
+    }
+    hipLaunchKernelGGL((k_adj_coef_edge<CT, NS>), dim3(grid_items(16L * m.nE, m.stripe_log2, m.block)), dim3(m.block), 0, s,
+                       m, c, a);
+    hipLaunchKernelGGL((k_adj_coef_vertex<CT, NS>), dim3(grid_items(8L * m.nV, m.stripe_log2, m.block)), dim3(m.block), 0, s,
+                       m, c, a);
+    hipLaunchKernelGGL((k_adj_coef_cell<CT, NS>), dim3(grid_items(8L * m.nC, m.stripe_log2, m.block)), dim3(m.block), 0, s,
+                       m, c, a);
 }
 
 void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, bool f32, hipStream_t s)
@@ -2560,9 +2592,17 @@ void tl_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef
 void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& b, const AdjCoef& q, bool f32, hipStream_t s)
 {
     if (f32)
-        adj_coef_as<float>(m, c, b, q, s);
+        adj_coef_as<float, 1>(m, c, &b, &q, s);
     else
-        adj_coef_as<double>(m, c, b, q, s);
+        adj_coef_as<double, 1>(m, c, &b, &q, s);
+}
+
+void adj_coef4(const DMesh& m, const DConfig& c, const BaseStage* b, const AdjCoef* q, bool f32, hipStream_t s)
+{
+    if (f32)
+        adj_coef_as<float, 4>(m, c, b, q, s);
+    else
+        adj_coef_as<double, 4>(m, c, b, q, s);
 }
 
 template <typename T>

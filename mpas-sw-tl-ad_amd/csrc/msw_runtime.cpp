@@ -734,9 +734,14 @@ int side_adj_base(msw_ctx* c, BaseStage* st, AdjCoef* qs, double* x, bool want_e
             base_tend(c, b, s, k, wxh, wxu, nh, ss);
         }
     }
-    for (int s = 3; s >= 0; --s) {
-        adj_coef(m, c->dcfg, st[s], qs[s], c->prec == MSW_FP32, ss);
-        if (events) HIP_TRY(hipEventRecord(c->ev[1 + s], ss));
+    if (events) {
+        // stage by stage, so each member stage starts once its rows exist
+        for (int s = 3; s >= 0; --s) {
+            adj_coef(m, c->dcfg, st[s], qs[s], c->prec == MSW_FP32, ss);
+            HIP_TRY(hipEventRecord(c->ev[1 + s], ss));
+        }
+    } else {
+        adj_coef4(m, c->dcfg, st, qs, c->prec == MSW_FP32, ss);
     }
     return MSW_OK;
 }

@@ -826,6 +826,13 @@ void adj_coef(const DMesh& m, const DConfig& c, const BaseStage& bs, const AdjCo
     launch(s, "k_adj_coef_cell", bc);
 }
 
+void adj_coef4(const DMesh& m, const DConfig& c, const BaseStage* bs, const AdjCoef* q, bool f32, hipStream_t s)
+{
+    // one launch per kernel over the four stages: the same buffers as four
+    // single-stage launches
+    for (int i = 0; i < 4; ++i) adj_coef(m, c, bs[i], q[i], f32, s);
+}
+
 template <typename T>
 void tl_stage(const DMesh& m, const DConfig& c, int lb, int stage, const RKCoeffs& k, const BaseStage& bs,
               const T* h_tl, const T* u_tl, T* pvv_tl, T* ke_tl, T* pc_tl, T* pe_tl, T* he_tl, T* vort_tl,
