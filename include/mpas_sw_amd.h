@@ -188,10 +188,11 @@ int msw_base_steps(msw_ctx *ctx, double *h, double *u, int nsteps, double *traj_
 /* Replaces: sw_rk4_tlm (src/mpas_sw_module.f90:637-781), nsteps times.
  * Advances the FP64 nonlinear state (h,u) [B=1] and the batched TL state
  * (h_tl,u_tl) [B members, `prec`] together, as the reference does
- * (:774-777).  Optional trajectory storage as in msw_base_steps.  Within a
- * call the nonlinear state runs a step ahead of the members (the next
- * step's base stages are formed while the current one's member stages
- * run); after an error return, (h,u) and (h_tl,u_tl) are unspecified. */
+ * (:774-777).  Optional trajectory storage as in msw_base_steps.  With no
+ * tendency output set, the nonlinear state runs a step ahead of the
+ * members within a call (the next step's base stages are formed while the
+ * current step's member stages run); after an error return, (h,u) and
+ * (h_tl,u_tl) are unspecified. */
 int msw_tlm_steps(msw_ctx *ctx, double *h, double *u, void *h_tl, void *u_tl, int nsteps,
                   double *traj_h, double *traj_u);
 
