@@ -418,7 +418,10 @@ def main():
         log(f"plain fp64: {plain}")
     f32 = None
     if not args.no_fp32:
-        f32 = gpu_rate(mesh, phys, dh, du, BATCH, "fp32", args.steps, args.warmup, g)
+        # at the default launch shape (stripe 2, 256 threads), without the
+        # autotuner's option changes: the 64-member FP32 test of those changes
+        # is kept off the GPU (a host fault whose cause is not found)
+        f32 = gpu_rate(mesh, phys, dh, du, BATCH, "fp32", args.steps, args.warmup, g, tune=False)
         log(f"batched fp32: {f32}")
     physf = None
     if not args.no_physical_f:
