@@ -128,6 +128,8 @@ def launch_shapes(prec, B):
         ok(lib.msw_set_option(ctx, _lib.OPT_BLOCK, blk), "block")
         for _ in range(2):
             tlm_adj(ctx, s, n=1)
+    if lib.msw_set_option(ctx, _lib.OPT_BLOCK, 96) != _lib.MSW_EINVAL:   # the test's last call
+        fail("a block of 96 threads was accepted")
     ok(lib.msw_destroy(ctx), "destroy")
     dev.free()
     no_violations(f"launch shapes prec={prec} B={B}")
